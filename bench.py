@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Throughput of the DPM timestep on one MI355X: the 3D monolayer.
+
+Workload (one "step" = one overdamped Euler step of every vertex):
+  3D monolayer of 256 cells x 162 vertices (icosphere subdivision 2,
+  320 triangles) with radius U(0.9, 1.1) on a 16x16 jittered grid on the
+  substrate plane, periodic box L = 38.4; volume, surface-area (edge spring)
+  and bending springs Kv=1, Ka=0.5, Kb=0.01; winding-number contact Kre=1;
+  vertex adhesion Kat=0.1; substrate Ks=0.2; dt=0.002.  Forces in float,
+  per-cell volume / centre-of-mass sums in double.
+  * plain   — one tissue per handle (the reference's Tissue3D);
+  * batched — `--batch` independent tissues of that shape (seeds 0..B-1)
+              advanced together by the same launches.
+`value` is the batched whole-job rate in vertex-steps/s with all inputs
+resident in HBM; the plain rate is reported beside it.
+
+roofline: the dominant (only) kernel of a step is k3d_step.  Algorithmic
+bytes per vertex-step = 24 (position read + write, float xyz) + 24/162
+(per-cell state amortised) = 24.15 B; `achieved` = that x vertices per
+launch / the average launch duration measured here with HIP events
+bracketing single launches on the library's own stream.
+
+cpu_baseline: the reference algorithm restated (oracle ref32: all-pairs,
+serial float, the four terms the reference launches), OpenMP over cells on
+the host cores, one step of the same 256-cell tissue.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "vertex-steps/s + HBM GB/s, 3D monolayer; batched FP32 vs plain at 1e-5 err"
+B_ALG = 24.0 + 24.0 / 162.0   # bytes per vertex-step
+HBM_PEAK = 8000.0             # GB/s, MI355X
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_tissue(capi, ms):
+    cb = np.concatenate([[0], np.cumsum([m.C for m in ms])])
+    cat = lambda f: np.concatenate([np.broadcast_to(np.asarray(f(m), np.float32), (m.C,)) for m in ms])
+    m0 = ms[0]
+    t = capi.Tissues3D(m0.mesh.faces, cb, cat(lambda m: m.Kv), cat(lambda m: m.Ka), cat(lambda m: m.Ks),
+                       cat(lambda m: m.v0), cat(lambda m: m.a0), [m.L for m in ms], [m.Kre for m in ms],
+                       Kat=[m.Kat for m in ms], Kb=cat(lambda m: m.Kb), mask=capi.DPM3D_ALL, nv=m0.mesh.nv)
+    t.set_positions(np.concatenate([m.X for m in ms]))
+    return t
+
+
+def time_steps(t, hiptime, steps, warmup, dt):
+    t.euler_async(warmup, dt)
+    t.synchronize()
+    t.counters(reset=True)
+    e0, e1 = hiptime.Event(), hiptime.Event()
+    s = t.stream()
+    wall0 = time.perf_counter()
+    e0.record(s)
+    t.euler_async(steps, dt)
+    e1.record(s)
+    ms = e0.elapsed_ms(e1)
+    wall = time.perf_counter() - wall0
+    return ms, wall, t.counters()
+
+
+def time_launches(t, hiptime, n, dt):
+    """Average duration of single k3d_step launches, events around each."""
+    s = t.stream()
+    evs = [(hiptime.Event(), hiptime.Event()) for _ in range(n)]
+    for a, b in evs:
+        a.record(s)
+        t.euler_async(1, dt)
+        b.record(s)
+    t.synchronize()
+    d = [a.elapsed_ms(b) for a, b in evs]
+    return float(np.mean(d)), float(np.min(d))
+
+
+def cpu_baseline(m, steps):
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    oracle.run3d(m.X, m.mesh.faces, m.mesh.nv, m.Kv, m.Ka, m.Ks, m.v0, m.a0, m.L, m.Kre, steps, m.dt,
+                 mask=15, prec=32, lib="liboracle_omp.so")
+    sec = time.perf_counter() - t0
+    nvt = m.C * m.mesh.nv
+    return {"value": nvt * steps / sec, "unit": "vertex-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} step(s) of the 256-cell x 162-vertex monolayer, the reference's four "
+                      f"terms (volume, surface, substrate, all-pairs winding contact), float, "
+                      f"OpenMP over cells: {sec:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--plain-steps", type=int, default=1000)
+    ap.add_argument("--launches", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from opencl_dpm_amd import capi, hiptime
+    from opencl_dpm_amd import workloads as W
+
+    if capi.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    dt = 0.002
+    m = W.monolayer_config("cfg2", seed=0)
+    nvt = m.C * m.mesh.nv
+
+    log(f"[bench] plain: {m.C} cells x {m.mesh.nv} vertices, {args.plain_steps} steps")
+    plain = build_tissue(capi, [m])
+    p_ms, p_wall, p_cnt = time_steps(plain, hiptime, args.plain_steps, args.warmup, dt)
+    p_launch, p_launch_min = time_launches(plain, hiptime, args.launches, dt)
+    plain_rate = nvt * args.plain_steps / (p_ms / 1e3)
+    log(f"[bench] plain {plain_rate:.4g} vertex-steps/s ({p_ms / args.plain_steps * 1e3:.1f} us/step, "
+        f"launch {p_launch * 1e3:.1f} us)")
+    del plain
+
+    B = args.batch
+    log(f"[bench] batched: {B} tissues, {args.steps} steps")
+    ms = [W.monolayer_config("cfg2", seed=s) for s in range(B)]
+    bt = build_tissue(capi, ms)
+    b_ms, b_wall, b_cnt = time_steps(bt, hiptime, args.steps, args.warmup, dt)
+    b_launch, b_launch_min = time_launches(bt, hiptime, args.launches, dt)
+    nvb = nvt * B
+    rate = nvb * args.steps / (b_ms / 1e3)
+    log(f"[bench] batched {rate:.4g} vertex-steps/s ({b_ms / args.steps:.3f} ms/step, "
+        f"launch {b_launch:.3f} ms)")
+    achieved = B_ALG * nvb / (b_launch / 1e3) / 1e9
+    windings = float(b_cnt[1]) / max(1, int(b_cnt[0]))
+
+    out = {
+        "metric": METRIC,
+        "value": rate,
+        "unit": "vertex-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": b_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D monolayer, 256 cells x 162 vertices (icosphere s=2), batch of {B} "
+                        f"independent tissues, all force terms, dt=0.002",
+            "tissues": B, "cells_per_tissue": m.C, "verts_per_cell": m.mesh.nv,
+            "vertices": nvb, "per_cell_sums": "f64",
+        },
+        "plain": {"value": plain_rate, "unit": "vertex-steps/s", "steps": args.plain_steps,
+                  "ms_per_step": p_ms / args.plain_steps, "launch_ms": p_launch},
+        "batched_vs_plain": rate / plain_rate,
+        "hbm_gbps_algorithmic": B_ALG * rate / 1e9,
+        "winding_evals_per_step": windings,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k3d_step", "launch_ms": b_launch, "bytes_per_vertex_step": B_ALG},
+    }
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline: {args.cpu_steps} step(s)")
+        out["cpu_baseline"] = cpu_baseline(m, args.cpu_steps)
+        log(f"[bench] cpu {out['cpu_baseline']['value']:.4g} vertex-steps/s")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,286 @@
+"""ctypes binding of the C ABI in include/dpm.h (libdpm_hip.so).
+
+This is the binding a Python caller uses for the batched forms (many
+tissues in one handle) and for timing; the drop-in classes of the reference
+(`clDPM.Tissue2D/3D`) reach the same library through the pybind11 module.
+The library is the HIP implementation only: if it is missing or no GPU is
+visible, calls fail loudly; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdpm_hip.so")
+
+DPM_OK, DPM_EINVAL, DPM_ENONFINITE, DPM_EDEVICE, DPM_ENOMEM, DPM_ERANGE = range(6)
+
+DPM2D_AREA, DPM2D_PERIMETER, DPM2D_BENDING, DPM2D_ATTRACTION, DPM2D_REPULSION = 1, 2, 4, 8, 16
+DPM2D_ALL = 31
+DPM3D_VOLUME, DPM3D_SURFACE, DPM3D_SUBSTRATE, DPM3D_CONTACT = 1, 2, 4, 8
+DPM3D_ADHESION, DPM3D_BENDING = 16, 32
+DPM3D_REFERENCE, DPM3D_ALL = 15, 63
+
+# Every symbol include/dpm.h declares (checked by the CPU tests).
+EXPORTS = [
+    "dpm_last_error", "dpm_version", "dpm_device_count",
+    "dpm3d_create", "dpm3d_set_positions", "dpm3d_get_positions", "dpm3d_get_forces",
+    "dpm3d_set_cell_params", "dpm3d_set_tissue_params", "dpm3d_cell_stats", "dpm3d_probe",
+    "dpm2d_create", "dpm2d_set_positions", "dpm2d_get_positions", "dpm2d_get_forces",
+    "dpm2d_set_cell_params", "dpm2d_set_tissue_params", "dpm2d_probe",
+    "dpm_euler", "dpm_euler_async", "dpm_synchronize", "dpm_stream", "dpm_counters",
+    "dpm_destroy",
+]
+
+
+class DPMError(RuntimeError):
+    pass
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m opencl_dpm_amd.build` (HIP path required)")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, i, sz, fp = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_float
+    lib.dpm_last_error.restype = ctypes.c_char_p
+    lib.dpm_stream.restype = vp
+    lib.dpm_stream.argtypes = [vp]
+    for name in ["dpm3d_set_positions", "dpm3d_get_positions", "dpm3d_get_forces",
+                 "dpm2d_set_positions", "dpm2d_get_positions", "dpm2d_get_forces"]:
+        getattr(lib, name).argtypes = [vp, vp, sz]
+    lib.dpm_euler.argtypes = [vp, i, fp]
+    lib.dpm_euler_async.argtypes = [vp, i, fp]
+    lib.dpm_synchronize.argtypes = [vp]
+    lib.dpm_counters.argtypes = [vp, vp, i]
+    lib.dpm_destroy.argtypes = [vp]
+    lib.dpm_destroy.restype = None
+    lib.dpm3d_create.argtypes = [vp, vp]
+    lib.dpm2d_create.argtypes = [vp, vp]
+    lib.dpm3d_set_cell_params.argtypes = [vp] + [vp] * 6
+    lib.dpm3d_set_tissue_params.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_uint]
+    lib.dpm2d_set_cell_params.argtypes = [vp] + [vp] * 6
+    lib.dpm2d_set_tissue_params.argtypes = [vp, vp, vp, vp, vp, ctypes.c_uint]
+    lib.dpm3d_cell_stats.argtypes = [vp, vp, vp]
+    lib.dpm3d_probe.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.dpm2d_probe.argtypes = [vp, vp, vp, vp]
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != DPM_OK:
+        msg = load().dpm_last_error().decode()
+        if rc == DPM_EINVAL:
+            raise ValueError(msg)
+        raise DPMError(f"[{rc}] {msg}")
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    load().dpm_device_count(ctypes.byref(n))
+    return n.value
+
+
+class _Desc3D(ctypes.Structure):
+    _fields_ = [
+        ("ntissues", ctypes.c_int), ("cell_begin", ctypes.c_void_p),
+        ("nv", ctypes.c_int), ("nf", ctypes.c_int), ("faces", ctypes.c_void_p),
+        ("Kv", ctypes.c_void_p), ("Ka", ctypes.c_void_p), ("Ks", ctypes.c_void_p),
+        ("Kb", ctypes.c_void_p), ("v0", ctypes.c_void_p), ("a0", ctypes.c_void_p),
+        ("L", ctypes.c_void_p), ("pbc", ctypes.c_void_p), ("Kre", ctypes.c_void_p),
+        ("Kat", ctypes.c_void_p), ("ks_grad", ctypes.c_void_p), ("mask", ctypes.c_uint),
+    ]
+
+
+class _Desc2D(ctypes.Structure):
+    _fields_ = [
+        ("ntissues", ctypes.c_int), ("cell_begin", ctypes.c_void_p), ("nv", ctypes.c_void_p),
+        ("Ka", ctypes.c_void_p), ("Kl", ctypes.c_void_p), ("Kb", ctypes.c_void_p),
+        ("a0", ctypes.c_void_p), ("l0", ctypes.c_void_p), ("r0", ctypes.c_void_p),
+        ("L", ctypes.c_void_p), ("pbc", ctypes.c_void_p), ("Kre", ctypes.c_void_p),
+        ("Kat", ctypes.c_void_p), ("mask", ctypes.c_uint),
+    ]
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+class _Handle:
+    def __init__(self):
+        self._h = ctypes.c_void_p(None)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            load().dpm_destroy(self._h)
+            self._h = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def euler(self, nsteps: int, dt: float) -> None:
+        _check(load().dpm_euler(self._h, int(nsteps), float(dt)))
+
+    def euler_async(self, nsteps: int, dt: float) -> None:
+        _check(load().dpm_euler_async(self._h, int(nsteps), float(dt)))
+
+    def synchronize(self) -> None:
+        _check(load().dpm_synchronize(self._h))
+
+    def stream(self) -> int:
+        return load().dpm_stream(self._h) or 0
+
+    def counters(self, reset: bool = False) -> np.ndarray:
+        out = np.zeros(4, np.uint64)
+        _check(load().dpm_counters(self._h, _p(out), int(reset)))
+        return out
+
+
+class Tissues3D(_Handle):
+    """A batch of 3D tissues sharing one mesh (include/dpm.h, dpm3d_*)."""
+
+    def __init__(self, faces, cell_begin: Sequence[int], Kv, Ka, Ks, v0, a0, L, Kre, Kat=0.0,
+                 pbc=1, Kb=None, ks_grad=None, mask: int = DPM3D_REFERENCE, nv: int = None):
+        super().__init__()
+        self._faces = _i32(faces).reshape(-1, 3)
+        self.nf = self._faces.shape[0]
+        self.nv = int(nv if nv is not None else self._faces.max() + 1)
+        self.cell_begin = _i32(cell_begin)
+        self.T = len(self.cell_begin) - 1
+        self.C = int(self.cell_begin[-1])
+        bc = lambda v, n: _f32(np.broadcast_to(np.asarray(v, np.float32), (n,)))
+        self._cell = [bc(Kv, self.C), bc(Ka, self.C), bc(Ks, self.C),
+                      bc(0.0 if Kb is None else Kb, self.C), bc(v0, self.C), bc(a0, self.C)]
+        self._tis = [bc(L, self.T), _i32(np.broadcast_to(np.asarray(pbc), (self.T,))),
+                     bc(Kre, self.T), bc(Kat, self.T),
+                     bc(0.0 if ks_grad is None else ks_grad, self.T)]
+        d = _Desc3D()
+        d.ntissues = self.T
+        d.cell_begin = _p(self.cell_begin)
+        d.nv, d.nf, d.faces = self.nv, self.nf, _p(self._faces)
+        d.Kv, d.Ka, d.Ks, d.Kb, d.v0, d.a0 = [_p(a) for a in self._cell]
+        d.L, d.pbc, d.Kre, d.Kat, d.ks_grad = [_p(a) for a in self._tis]
+        d.mask = mask
+        _check(load().dpm3d_create(ctypes.byref(d), ctypes.byref(self._h)))
+
+    @property
+    def nverts(self) -> int:
+        return self.C * self.nv
+
+    def set_positions(self, xyz) -> None:
+        a = _f32(xyz).reshape(-1, 3)
+        _check(load().dpm3d_set_positions(self._h, _p(a), a.shape[0]))
+
+    def positions(self) -> np.ndarray:
+        out = np.empty((self.nverts, 3), np.float32)
+        _check(load().dpm3d_get_positions(self._h, _p(out), self.nverts))
+        return out
+
+    def forces(self) -> np.ndarray:
+        out = np.empty((self.nverts, 3), np.float32)
+        _check(load().dpm3d_get_forces(self._h, _p(out), self.nverts))
+        return out
+
+    def set_tissue_params(self, L=None, pbc=None, Kre=None, Kat=None, ks_grad=None, mask=None):
+        bc = lambda v: None if v is None else _f32(np.broadcast_to(np.asarray(v, np.float32), (self.T,)))
+        arrs = [bc(L), None if pbc is None else _i32(np.broadcast_to(np.asarray(pbc), (self.T,))),
+                bc(Kre), bc(Kat), bc(ks_grad)]
+        self._keep = arrs
+        _check(load().dpm3d_set_tissue_params(self._h, *[_p(a) for a in arrs],
+                                              ctypes.c_uint(DPM3D_REFERENCE if mask is None else mask)))
+
+    def set_cell_params(self, Kv=None, Ka=None, Ks=None, Kb=None, v0=None, a0=None):
+        bc = lambda v: None if v is None else _f32(np.broadcast_to(np.asarray(v, np.float32), (self.C,)))
+        arrs = [bc(Kv), bc(Ka), bc(Ks), bc(Kb), bc(v0), bc(a0)]
+        self._keep2 = arrs
+        _check(load().dpm3d_set_cell_params(self._h, *[_p(a) for a in arrs]))
+
+    def cell_stats(self):
+        vol = np.empty(self.C, np.float64)
+        com = np.empty((self.C, 3), np.float32)
+        _check(load().dpm3d_cell_stats(self._h, _p(vol), _p(com)))
+        return vol, com
+
+    def probe(self, winding: bool = False):
+        """Forces at the current positions plus the recorded pair sets."""
+        ncmax = int(np.max(np.diff(self.cell_begin)))
+        f = np.empty((self.nverts, 3), np.float32)
+        contact = np.zeros((self.nverts, ncmax), np.uint8)
+        w = np.zeros((self.nverts, ncmax), np.float32) if winding else None
+        down = np.zeros((self.C, self.nf), np.uint8)
+        adh = np.zeros(self.nverts, np.int32)
+        _check(load().dpm3d_probe(self._h, _p(f), _p(contact), _p(w), _p(down), _p(adh)))
+        return {"forces": f, "contact": contact, "winding": w, "down": down, "adhesion": adh}
+
+
+class Tissues2D(_Handle):
+    """A batch of 2D tissues (include/dpm.h, dpm2d_*); vertices packed."""
+
+    def __init__(self, cell_begin, nv, Ka, Kl, Kb, a0, l0, r0, L, Kre=1.0, Kat=0.0, pbc=1,
+                 mask: int = DPM2D_ALL):
+        super().__init__()
+        self.cell_begin = _i32(cell_begin)
+        self.T = len(self.cell_begin) - 1
+        self.nv = _i32(nv)
+        self.C = int(self.cell_begin[-1])
+        self.V = int(self.nv.sum())
+        bc = lambda v, n: _f32(np.broadcast_to(np.asarray(v, np.float32), (n,)))
+        self._cell = [bc(Ka, self.C), bc(Kl, self.C), bc(Kb, self.C), bc(a0, self.C),
+                      bc(l0, self.C), bc(r0, self.C)]
+        self._tis = [bc(L, self.T), _i32(np.broadcast_to(np.asarray(pbc), (self.T,))),
+                     bc(Kre, self.T), bc(Kat, self.T)]
+        d = _Desc2D()
+        d.ntissues = self.T
+        d.cell_begin = _p(self.cell_begin)
+        d.nv = _p(self.nv)
+        d.Ka, d.Kl, d.Kb, d.a0, d.l0, d.r0 = [_p(a) for a in self._cell]
+        d.L, d.pbc, d.Kre, d.Kat = [_p(a) for a in self._tis]
+        d.mask = mask
+        _check(load().dpm2d_create(ctypes.byref(d), ctypes.byref(self._h)))
+
+    def set_positions(self, xy) -> None:
+        a = _f32(xy).reshape(-1, 2)
+        _check(load().dpm2d_set_positions(self._h, _p(a), a.shape[0]))
+
+    def positions(self) -> np.ndarray:
+        out = np.empty((self.V, 2), np.float32)
+        _check(load().dpm2d_get_positions(self._h, _p(out), self.V))
+        return out
+
+    def forces(self) -> np.ndarray:
+        out = np.empty((self.V, 2), np.float32)
+        _check(load().dpm2d_get_forces(self._h, _p(out), self.V))
+        return out
+
+    def probe(self):
+        f = np.empty((self.V, 2), np.float32)
+        ov = np.zeros(self.V, np.int32)
+        at = np.zeros(self.V, np.int32)
+        _check(load().dpm2d_probe(self._h, _p(f), _p(ov), _p(at)))
+        return {"forces": f, "overlap": ov, "attraction": at}

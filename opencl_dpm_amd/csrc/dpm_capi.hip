@@ -1,0 +1,832 @@
+// C ABI of the DPM timestep (include/dpm.h): handle management, device
+// state, launch orchestration.  One translation unit holds the kernels and
+// their host side so that the library is one code object for gfx950.
+#include "dpm.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dpm2d.hip"
+#include "dpm3d.hip"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHECK(call)                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(DPM_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+template <class T>
+int dev_alloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+  if (e != hipSuccess) return fail(DPM_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return DPM_OK;
+}
+
+template <class T>
+int upload(T* dst, const T* src, size_t n, hipStream_t s) {
+  HIPCHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return DPM_OK;
+}
+
+#define TRY(x)                \
+  do {                        \
+    int rc_ = (x);            \
+    if (rc_ != DPM_OK) return rc_; \
+  } while (0)
+
+int max_lds_per_block(int dev) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return 65536;
+  return v;
+}
+
+// Steps replayed per hipGraph launch (even, so the position ping-pong ends
+// where it started).
+constexpr int kGraphSteps = 64;
+
+}  // namespace
+
+struct dpm_tissue {
+  int dim = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  unsigned long long* counters = nullptr;  // device [4]
+  std::vector<void*> allocs;
+
+  // ------------------------------------------------------------------ 3D
+  int T = 0, C = 0, nv = 0, nf = 0, NT = 0, ncmax = 0;
+  size_t lds = 0;
+  float *x[2] = {}, *y[2] = {}, *z[2] = {}, *fx = nullptr, *fy = nullptr, *fz = nullptr;
+  float4* st[2] = {};
+  double* vol[2] = {};
+  float *Kv = nullptr, *Ka = nullptr, *Ks = nullptr, *Kb = nullptr, *v0 = nullptr, *a0 = nullptr,
+        *l0 = nullptr;
+  int *cell_tissue = nullptr, *tissue_begin = nullptr;
+  float *tL = nullptr, *tKre = nullptr, *tKat = nullptr, *tksg = nullptr;
+  int* tpbc = nullptr;
+  int *faces = nullptr, *vf = nullptr, *adj = nullptr;
+  unsigned mask = 0;
+  int cur = 0;
+  bool stats_valid = false;
+  bool forces_valid = false;
+  std::vector<int> tissue_begin_h;
+  std::vector<int> faces_h;
+  hipGraphExec_t graph = nullptr;
+  int graph_cur = -1;
+  float graph_dt = 0.0f;
+  int* flag = nullptr;  // device non-finite flag
+
+  // ------------------------------------------------------------------ 2D
+  int V = 0, NT2 = 0;
+  size_t lds2 = 0;
+  float *x2 = nullptr, *y2 = nullptr, *fx2 = nullptr, *fy2 = nullptr;
+  int *tcb = nullptr, *cvb = nullptr;
+  float *Kl = nullptr, *l0_2 = nullptr, *r0 = nullptr;
+  std::vector<int> cvb_h;
+
+  template <class T_>
+  int alloc(T_** p, size_t n) {
+    TRY(dev_alloc(p, n));
+    allocs.push_back(*p);
+    return DPM_OK;
+  }
+  ~dpm_tissue() {
+    if (graph) hipGraphExecDestroy(graph);
+    for (void* p : allocs) hipFree(p);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+// ------------------------------------------------------------------ common
+
+extern "C" const char* dpm_last_error(void) { return g_err.c_str(); }
+extern "C" int dpm_version(void) { return 1; }
+
+extern "C" int dpm_device_count(int* n) {
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  if (n) *n = c;
+  return DPM_OK;
+}
+
+static int init_common(dpm_tissue* h) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(DPM_EDEVICE, "no HIP device is visible");
+  HIPCHECK(hipGetDevice(&h->device));
+  HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  TRY(h->alloc(&h->counters, 4));
+  TRY(h->alloc(&h->flag, 1));
+  HIPCHECK(hipMemsetAsync(h->counters, 0, 4 * sizeof(unsigned long long), h->stream));
+  return DPM_OK;
+}
+
+extern "C" void dpm_destroy(dpm_tissue* h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  delete h;
+}
+
+extern "C" void* dpm_stream(dpm_tissue* h) { return h ? (void*)h->stream : nullptr; }
+
+extern "C" int dpm_synchronize(dpm_tissue* h) {
+  if (!h) return fail(DPM_EINVAL, "null handle");
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return DPM_OK;
+}
+
+extern "C" int dpm_counters(dpm_tissue* h, unsigned long long* out4, int reset) {
+  if (!h || !out4) return fail(DPM_EINVAL, "null argument");
+  HIPCHECK(hipMemcpyAsync(out4, h->counters, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  if (reset) HIPCHECK(hipMemsetAsync(h->counters, 0, 4 * sizeof(unsigned long long), h->stream));
+  return DPM_OK;
+}
+
+__global__ void k_nonfinite(const float* a, const float* b, const float* c, size_t n, int* flag) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const bool bad = !isfinite(a[i]) || !isfinite(b[i]) || (c && !isfinite(c[i]));
+    if (bad) atomicOr(flag, 1);
+  }
+}
+
+// Positions of all vertices finite?  (Tissue3D.cpp:481-488)
+static int check_finite(dpm_tissue* h, const float* a, const float* b, const float* c, size_t n) {
+  HIPCHECK(hipMemsetAsync(h->flag, 0, sizeof(int), h->stream));
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 2048);
+  k_nonfinite<<<blocks, 256, 0, h->stream>>>(a, b, c, n, h->flag);
+  HIPCHECK(hipGetLastError());
+  int f = 0;
+  HIPCHECK(hipMemcpyAsync(&f, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  if (f) return fail(DPM_ENONFINITE, "Non-finite simulation results");
+  return DPM_OK;
+}
+
+// ================================================================== 3D ====
+
+static bool all_finite(const float* p, size_t n) {
+  for (size_t i = 0; i < n; i++)
+    if (!std::isfinite(p[i])) return false;
+  return true;
+}
+
+extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) {
+  if (!d || !out) return fail(DPM_EINVAL, "null argument");
+  *out = nullptr;
+  if (d->ntissues <= 0 || !d->cell_begin) return fail(DPM_EINVAL, "NCELLS must be positive");
+  const int T = d->ntissues;
+  if (d->cell_begin[0] != 0) return fail(DPM_EINVAL, "cell_begin[0] must be 0");
+  int ncmax = 0;
+  for (int t = 0; t < T; t++) {
+    const int n = d->cell_begin[t + 1] - d->cell_begin[t];
+    if (n <= 0) return fail(DPM_EINVAL, "NCELLS must be positive");
+    ncmax = std::max(ncmax, n);
+  }
+  const int C = d->cell_begin[T];
+  const int nv = d->nv, nf = d->nf;
+  if (nv < 4 || nf < 4 || !d->faces) return fail(DPM_EINVAL, "mesh needs nv >= 4 and nf >= 4");
+  if (nv > 1024) return fail(DPM_ERANGE, "nv > 1024 vertices per cell is not supported");
+  if (!d->Kv || !d->Ka || !d->Ks || !d->v0 || !d->a0 || !d->L || !d->pbc || !d->Kre || !d->Kat)
+    return fail(DPM_EINVAL, "missing parameter array");
+  for (int f = 0; f < nf; f++)
+    for (int k = 0; k < 3; k++) {
+      const int v = d->faces[3 * f + k];
+      if (v < 0 || v >= nv) {
+        char buf[160];
+        snprintf(buf, sizeof(buf), "Invalid face indices (face %d: %d,%d,%d)", f, d->faces[3 * f],
+                 d->faces[3 * f + 1], d->faces[3 * f + 2]);
+        return fail(DPM_ERANGE, buf);
+      }
+    }
+  for (int c = 0; c < C; c++) {
+    if (!(d->Kv[c] > 0) || !(d->Ka[c] > 0) || !(d->Ks[c] > 0))
+      return fail(DPM_ERANGE, "Invalid spring constants");
+    if (!(d->v0[c] > 0) || !(d->a0[c] > 0)) return fail(DPM_ERANGE, "Invalid reference values");
+  }
+  // incidence (face*3 + corner) and adjacency, ascending
+  std::vector<std::vector<int>> inc(nv), nb(nv);
+  for (int f = 0; f < nf; f++)
+    for (int k = 0; k < 3; k++) {
+      const int v = d->faces[3 * f + k];
+      inc[v].push_back(3 * f + k);
+      const int a = d->faces[3 * f + (k + 1) % 3];
+      nb[v].push_back(a);
+      nb[a].push_back(v);
+    }
+  std::vector<int> vf((size_t)nv * DPM3D_MAXDEG, -1), adj((size_t)nv * DPM3D_MAXDEG, -1);
+  for (int v = 0; v < nv; v++) {
+    std::sort(nb[v].begin(), nb[v].end());
+    nb[v].erase(std::unique(nb[v].begin(), nb[v].end()), nb[v].end());
+    if ((int)inc[v].size() > DPM3D_MAXDEG || (int)nb[v].size() > DPM3D_MAXDEG)
+      return fail(DPM_ERANGE, "vertex degree above 8 is not supported");
+    for (size_t k = 0; k < inc[v].size(); k++) vf[(size_t)v * DPM3D_MAXDEG + k] = inc[v][k];
+    for (size_t k = 0; k < nb[v].size(); k++) adj[(size_t)v * DPM3D_MAXDEG + k] = nb[v][k];
+  }
+
+  dpm_tissue* h = new dpm_tissue();
+  h->dim = 3;
+  int rc = init_common(h);
+  if (rc) { delete h; return rc; }
+  h->T = T;
+  h->C = C;
+  h->nv = nv;
+  h->nf = nf;
+  h->ncmax = ncmax;
+  h->NT = std::max(256, (nv + 63) / 64 * 64);
+  h->lds = lds3d_bytes(nv, h->NT);
+  if ((int)h->lds > max_lds_per_block(h->device)) {
+    delete h;
+    return fail(DPM_ERANGE, "cell mesh does not fit the LDS of one workgroup");
+  }
+  const size_t NVT = (size_t)C * nv;
+  auto A = [&](auto** p, size_t n) { return h->alloc(p, n); };
+  rc = 0;
+  for (int b = 0; b < 2 && !rc; b++) {
+    rc |= A(&h->x[b], NVT);
+    rc |= A(&h->y[b], NVT);
+    rc |= A(&h->z[b], NVT);
+    rc |= A(&h->st[b], (size_t)C);
+    rc |= A(&h->vol[b], (size_t)C);
+  }
+  rc |= A(&h->fx, NVT);
+  rc |= A(&h->fy, NVT);
+  rc |= A(&h->fz, NVT);
+  rc |= A(&h->Kv, C);
+  rc |= A(&h->Ka, C);
+  rc |= A(&h->Ks, C);
+  rc |= A(&h->Kb, C);
+  rc |= A(&h->v0, C);
+  rc |= A(&h->a0, C);
+  rc |= A(&h->l0, C);
+  rc |= A(&h->cell_tissue, C);
+  rc |= A(&h->tissue_begin, T + 1);
+  rc |= A(&h->tL, T);
+  rc |= A(&h->tpbc, T);
+  rc |= A(&h->tKre, T);
+  rc |= A(&h->tKat, T);
+  rc |= A(&h->tksg, T);
+  rc |= A(&h->faces, (size_t)nf * 3);
+  rc |= A(&h->vf, vf.size());
+  rc |= A(&h->adj, adj.size());
+  if (rc) { delete h; return DPM_ENOMEM; }
+  h->tissue_begin_h.assign(d->cell_begin, d->cell_begin + T + 1);
+  h->faces_h.assign(d->faces, d->faces + 3 * nf);
+  std::vector<int> ct(C);
+  for (int t = 0; t < T; t++)
+    for (int c = d->cell_begin[t]; c < d->cell_begin[t + 1]; c++) ct[c] = t;
+  hipStream_t s = h->stream;
+  rc = 0;
+  rc |= upload(h->cell_tissue, ct.data(), C, s);
+  rc |= upload(h->tissue_begin, d->cell_begin, T + 1, s);
+  rc |= upload(h->faces, d->faces, (size_t)nf * 3, s);
+  rc |= upload(h->vf, vf.data(), vf.size(), s);
+  rc |= upload(h->adj, adj.data(), adj.size(), s);
+  HIPCHECK(hipMemsetAsync(h->fx, 0, NVT * sizeof(float), s));
+  HIPCHECK(hipMemsetAsync(h->fy, 0, NVT * sizeof(float), s));
+  HIPCHECK(hipMemsetAsync(h->fz, 0, NVT * sizeof(float), s));
+  if (rc) { delete h; return rc; }
+  rc = dpm3d_set_cell_params(h, d->Kv, d->Ka, d->Ks, d->Kb, d->v0, d->a0);
+  if (!rc) rc = dpm3d_set_tissue_params(h, d->L, d->pbc, d->Kre, d->Kat, d->ks_grad, d->mask);
+  if (!d->Kb && !rc) {
+    std::vector<float> z(C, 0.0f);
+    rc = upload(h->Kb, z.data(), C, s);
+  }
+  if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(DPM_EDEVICE, "stream sync failed");
+  if (rc) { delete h; return rc; }
+  hipFuncSetAttribute((const void*)k3d_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
+  hipFuncSetAttribute((const void*)k3d_stats, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
+  *out = h;
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_set_cell_params(dpm_tissue* h, const float* Kv, const float* Ka,
+                                     const float* Ks, const float* Kb, const float* v0,
+                                     const float* a0) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  const int C = h->C;
+  for (int c = 0; c < C; c++) {
+    if ((Kv && !(Kv[c] > 0)) || (Ka && !(Ka[c] > 0)) || (Ks && !(Ks[c] > 0)))
+      return fail(DPM_ERANGE, "Invalid spring constants");
+    if ((v0 && !(v0[c] > 0)) || (a0 && !(a0[c] > 0)))
+      return fail(DPM_ERANGE, "Invalid reference values");
+  }
+  hipStream_t s = h->stream;
+  if (Kv) TRY(upload(h->Kv, Kv, C, s));
+  if (Ka) TRY(upload(h->Ka, Ka, C, s));
+  if (Ks) TRY(upload(h->Ks, Ks, C, s));
+  if (Kb) TRY(upload(h->Kb, Kb, C, s));
+  if (v0) TRY(upload(h->v0, v0, C, s));
+  if (a0) {
+    TRY(upload(h->a0, a0, C, s));
+    // l0 = sqrt(4 a0)/sqrt(3), Tissue3D.cpp:177, in float.
+    std::vector<float> l0(C);
+    for (int c = 0; c < C; c++) l0[c] = sqrtf(4.0f * a0[c]) / sqrtf(3.0f);
+    TRY(upload(h->l0, l0.data(), C, s));
+    HIPCHECK(hipStreamSynchronize(s));
+  }
+  HIPCHECK(hipStreamSynchronize(s));
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_set_tissue_params(dpm_tissue* h, const float* L, const int* pbc,
+                                       const float* Kre, const float* Kat, const float* ks_grad,
+                                       unsigned mask) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  const int T = h->T;
+  hipStream_t s = h->stream;
+  if (L) {
+    for (int t = 0; t < T; t++)
+      if (!(L[t] > 0) || !std::isfinite(L[t])) return fail(DPM_EINVAL, "box length must be positive");
+    TRY(upload(h->tL, L, T, s));
+  }
+  if (pbc) TRY(upload(h->tpbc, pbc, T, s));
+  if (Kre) TRY(upload(h->tKre, Kre, T, s));
+  if (Kat) TRY(upload(h->tKat, Kat, T, s));
+  std::vector<float> zero(T, 0.0f);
+  TRY(upload(h->tksg, ks_grad ? ks_grad : zero.data(), T, s));
+  h->mask = mask & DPM3D_ALL;
+  HIPCHECK(hipStreamSynchronize(s));
+  if (h->graph) {
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_set_positions(dpm_tissue* h, const float* xyz, size_t n) {
+  if (!h || h->dim != 3 || !xyz) return fail(DPM_EINVAL, "bad arguments");
+  const size_t NVT = (size_t)h->C * h->nv;
+  if (n != NVT) return fail(DPM_EINVAL, "position count does not match the tissue");
+  if (!all_finite(xyz, 3 * n)) return fail(DPM_ENONFINITE, "Non-finite vertex coordinates");
+  std::vector<float> a(n), b(n), c(n);
+  for (size_t i = 0; i < n; i++) {
+    a[i] = xyz[3 * i];
+    b[i] = xyz[3 * i + 1];
+    c[i] = xyz[3 * i + 2];
+  }
+  const int k = h->cur;
+  TRY(upload(h->x[k], a.data(), n, h->stream));
+  TRY(upload(h->y[k], b.data(), n, h->stream));
+  TRY(upload(h->z[k], c.data(), n, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  h->stats_valid = false;
+  return DPM_OK;
+}
+
+static int download3(dpm_tissue* h, const float* a, const float* b, const float* c, float* xyz,
+                     size_t n) {
+  std::vector<float> A(n), B(n), Cc(n);
+  HIPCHECK(hipMemcpyAsync(A.data(), a, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipMemcpyAsync(B.data(), b, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipMemcpyAsync(Cc.data(), c, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  for (size_t i = 0; i < n; i++) {
+    xyz[3 * i] = A[i];
+    xyz[3 * i + 1] = B[i];
+    xyz[3 * i + 2] = Cc[i];
+  }
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_get_positions(dpm_tissue* h, float* xyz, size_t n) {
+  if (!h || h->dim != 3 || !xyz) return fail(DPM_EINVAL, "bad arguments");
+  if (n != (size_t)h->C * h->nv) return fail(DPM_EINVAL, "position count does not match the tissue");
+  const int k = h->cur;
+  return download3(h, h->x[k], h->y[k], h->z[k], xyz, n);
+}
+
+extern "C" int dpm3d_get_forces(dpm_tissue* h, float* xyz, size_t n) {
+  if (!h || h->dim != 3 || !xyz) return fail(DPM_EINVAL, "bad arguments");
+  if (n != (size_t)h->C * h->nv) return fail(DPM_EINVAL, "force count does not match the tissue");
+  return download3(h, h->fx, h->fy, h->fz, xyz, n);
+}
+
+static Step3D step_args(dpm_tissue* h, int in, float dt, int flags) {
+  Step3D a{};
+  a.xin = h->x[in];
+  a.yin = h->y[in];
+  a.zin = h->z[in];
+  a.xout = h->x[1 - in];
+  a.yout = h->y[1 - in];
+  a.zout = h->z[1 - in];
+  a.fx = h->fx;
+  a.fy = h->fy;
+  a.fz = h->fz;
+  a.st_in = h->st[in];
+  a.vol_in = h->vol[in];
+  a.st_out = h->st[1 - in];
+  a.vol_out = h->vol[1 - in];
+  a.Kv = h->Kv;
+  a.Ka = h->Ka;
+  a.Ks = h->Ks;
+  a.Kb = h->Kb;
+  a.v0 = h->v0;
+  a.a0 = h->a0;
+  a.l0 = h->l0;
+  a.cell_tissue = h->cell_tissue;
+  a.tissue_begin = h->tissue_begin;
+  a.tL = h->tL;
+  a.tpbc = h->tpbc;
+  a.tKre = h->tKre;
+  a.tKat = h->tKat;
+  a.tksg = h->tksg;
+  a.faces = h->faces;
+  a.vf = h->vf;
+  a.adj = h->adj;
+  a.nv = h->nv;
+  a.nf = h->nf;
+  a.ncells = h->C;
+  a.dt = dt;
+  a.mask = h->mask;
+  a.flags = flags;
+  a.counters = h->counters;
+  a.ncmax = h->ncmax;
+  return a;
+}
+
+static int ensure_stats(dpm_tissue* h) {
+  if (h->stats_valid) return DPM_OK;
+  Step3D a = step_args(h, h->cur, 0.0f, 0);
+  a.st_out = h->st[h->cur];
+  a.vol_out = h->vol[h->cur];
+  hipLaunchKernelGGL(k3d_stats, dim3(h->C), dim3(h->NT), h->lds, h->stream, a);
+  HIPCHECK(hipGetLastError());
+  h->stats_valid = true;
+  return DPM_OK;
+}
+
+static int launch_step3d(dpm_tissue* h, int in, float dt, int flags) {
+  Step3D a = step_args(h, in, dt, flags);
+  hipLaunchKernelGGL(k3d_step, dim3(h->C), dim3(h->NT), h->lds, h->stream, a);
+  HIPCHECK(hipGetLastError());
+  return DPM_OK;
+}
+
+// A graph of kGraphSteps steps starting from buffer 0 (ends on buffer 0).
+static int build_graph3d(dpm_tissue* h, float dt) {
+  if (h->graph) {
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = DPM_OK;
+  for (int s = 0; s < kGraphSteps && !rc; s++) rc = launch_step3d(h, s & 1, dt, F_MOVE);
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(DPM_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    h->graph = nullptr;
+    return fail(DPM_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  h->graph_dt = dt;
+  return DPM_OK;
+}
+
+static int euler3d(dpm_tissue* h, int nsteps, float dt) {
+  if (nsteps <= 0) return fail(DPM_EINVAL, "nsteps must be positive");
+  if (!(dt > 0.0f) || dt > 0.1f) return fail(DPM_EINVAL, "dt must be positive and reasonable");
+  TRY(ensure_stats(h));
+  int left = nsteps - 1;  // the last step also writes the forces
+  if (left >= kGraphSteps) {
+    // Bring the ping-pong to buffer 0 so that the graph (captured from 0)
+    // applies.
+    if (h->cur != 0) {
+      TRY(launch_step3d(h, h->cur, dt, F_MOVE));
+      h->cur = 1 - h->cur;
+      left--;
+    }
+    if (left >= kGraphSteps) {
+      if (!h->graph || h->graph_dt != dt) TRY(build_graph3d(h, dt));
+      while (left >= kGraphSteps) {
+        HIPCHECK(hipGraphLaunch(h->graph, h->stream));
+        left -= kGraphSteps;
+      }
+    }
+  }
+  for (; left > 0; left--) {
+    TRY(launch_step3d(h, h->cur, dt, F_MOVE));
+    h->cur = 1 - h->cur;
+  }
+  TRY(launch_step3d(h, h->cur, dt, F_MOVE | F_WRITE_FORCES));
+  h->cur = 1 - h->cur;
+  h->forces_valid = true;
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_cell_stats(dpm_tissue* h, double* volume, float* com) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  TRY(ensure_stats(h));
+  const int C = h->C;
+  std::vector<float4> st(C);
+  HIPCHECK(hipMemcpyAsync(st.data(), h->st[h->cur], C * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+  if (volume)
+    HIPCHECK(hipMemcpyAsync(volume, h->vol[h->cur], C * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  if (com)
+    for (int c = 0; c < C; c++) {
+      com[3 * c] = st[c].x;
+      com[3 * c + 1] = st[c].y;
+      com[3 * c + 2] = st[c].z;
+    }
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_probe(dpm_tissue* h, float* forces, unsigned char* contact, float* winding,
+                           unsigned char* down, int* adhesion) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  TRY(ensure_stats(h));
+  const size_t NVT = (size_t)h->C * h->nv;
+  const size_t npair = NVT * (size_t)h->ncmax;
+  unsigned char *dc = nullptr, *dd = nullptr;
+  float* dw = nullptr;
+  int* da = nullptr;
+  hipStream_t s = h->stream;
+  int rc = DPM_OK;
+  if (contact) rc |= dev_alloc(&dc, npair);
+  if (winding) rc |= dev_alloc(&dw, npair);
+  if (down) rc |= dev_alloc(&dd, (size_t)h->C * h->nf);
+  if (adhesion) rc |= dev_alloc(&da, NVT);
+  if (!rc) {
+    if (dc) hipMemsetAsync(dc, 0, npair, s);
+    if (dw) hipMemsetAsync(dw, 0, npair * sizeof(float), s);
+    if (dd) hipMemsetAsync(dd, 0, (size_t)h->C * h->nf, s);
+    if (da) hipMemsetAsync(da, 0, NVT * sizeof(int), s);
+    Step3D a = step_args(h, h->cur, 0.0f, F_WRITE_FORCES);
+    a.counters = nullptr;
+    a.p_contact = dc;
+    a.p_winding = dw;
+    a.p_down = dd;
+    a.p_adh = da;
+    hipLaunchKernelGGL(k3d_step, dim3(h->C), dim3(h->NT), h->lds, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail(DPM_EDEVICE, std::string("k3d_step: ") + hipGetErrorString(e));
+  }
+  if (!rc && dc) rc = hipMemcpyAsync(contact, dc, npair, hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
+  if (!rc && dw)
+    rc = hipMemcpyAsync(winding, dw, npair * sizeof(float), hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
+  if (!rc && dd)
+    rc = hipMemcpyAsync(down, dd, (size_t)h->C * h->nf, hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
+  if (!rc && da)
+    rc = hipMemcpyAsync(adhesion, da, NVT * sizeof(int), hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(DPM_EDEVICE, "stream sync failed");
+  if (dc) hipFree(dc);
+  if (dw) hipFree(dw);
+  if (dd) hipFree(dd);
+  if (da) hipFree(da);
+  if (rc) return rc;
+  h->forces_valid = true;
+  if (forces) TRY(dpm3d_get_forces(h, forces, NVT));
+  return DPM_OK;
+}
+
+// ================================================================== 2D ====
+
+extern "C" int dpm2d_create(const dpm2d_desc* d, dpm_tissue** out) {
+  if (!d || !out) return fail(DPM_EINVAL, "null argument");
+  *out = nullptr;
+  const int T = d->ntissues;
+  if (T <= 0 || !d->cell_begin || d->cell_begin[0] != 0) return fail(DPM_EINVAL, "bad cell_begin");
+  const int C = d->cell_begin[T];
+  if (!d->nv || !d->Ka || !d->Kl || !d->Kb || !d->a0 || !d->l0 || !d->r0 || !d->L || !d->pbc ||
+      !d->Kre || !d->Kat)
+    return fail(DPM_EINVAL, "missing parameter array");
+  std::vector<int> cvb(C + 1, 0);
+  for (int c = 0; c < C; c++) {
+    if (d->nv[c] < 3) return fail(DPM_EINVAL, "a cell needs at least 3 vertices");
+    cvb[c + 1] = cvb[c] + d->nv[c];
+  }
+  int maxv = 0, maxc = 0;
+  for (int t = 0; t < T; t++) {
+    const int nc = d->cell_begin[t + 1] - d->cell_begin[t];
+    if (nc <= 0) return fail(DPM_EINVAL, "NCELLS must be positive");
+    if (nc > 32767) return fail(DPM_ERANGE, "more than 32767 cells in one tissue");
+    maxc = std::max(maxc, nc);
+    maxv = std::max(maxv, cvb[d->cell_begin[t + 1]] - cvb[d->cell_begin[t]]);
+  }
+  dpm_tissue* h = new dpm_tissue();
+  h->dim = 2;
+  int rc = init_common(h);
+  if (rc) { delete h; return rc; }
+  h->T = T;
+  h->C = C;
+  h->V = cvb[C];
+  h->NT2 = std::min(1024, std::max(64, (maxv + 63) / 64 * 64));
+  if (maxv > DPM2D_VPT * h->NT2) {
+    delete h;
+    return fail(DPM_ERANGE, "tissue has more than 16384 vertices");
+  }
+  h->lds2 = lds2d_bytes(maxv, maxc);
+  if ((int)h->lds2 > max_lds_per_block(h->device)) {
+    delete h;
+    return fail(DPM_ERANGE, "2D tissue does not fit the LDS of one workgroup");
+  }
+  h->cvb_h = cvb;
+  h->tissue_begin_h.assign(d->cell_begin, d->cell_begin + T + 1);
+  rc = 0;
+  rc |= h->alloc(&h->x2, h->V);
+  rc |= h->alloc(&h->y2, h->V);
+  rc |= h->alloc(&h->fx2, h->V);
+  rc |= h->alloc(&h->fy2, h->V);
+  rc |= h->alloc(&h->tcb, T + 1);
+  rc |= h->alloc(&h->cvb, C + 1);
+  rc |= h->alloc(&h->Ka, C);
+  rc |= h->alloc(&h->Kl, C);
+  rc |= h->alloc(&h->Kb, C);
+  rc |= h->alloc(&h->a0, C);
+  rc |= h->alloc(&h->l0_2, C);
+  rc |= h->alloc(&h->r0, C);
+  rc |= h->alloc(&h->tL, T);
+  rc |= h->alloc(&h->tpbc, T);
+  rc |= h->alloc(&h->tKre, T);
+  rc |= h->alloc(&h->tKat, T);
+  if (rc) { delete h; return DPM_ENOMEM; }
+  hipStream_t s = h->stream;
+  rc |= upload(h->tcb, d->cell_begin, T + 1, s);
+  rc |= upload(h->cvb, cvb.data(), C + 1, s);
+  hipMemsetAsync(h->fx2, 0, h->V * sizeof(float), s);
+  hipMemsetAsync(h->fy2, 0, h->V * sizeof(float), s);
+  if (!rc) rc = dpm2d_set_cell_params(h, d->Ka, d->Kl, d->Kb, d->a0, d->l0, d->r0);
+  if (!rc) rc = dpm2d_set_tissue_params(h, d->L, d->pbc, d->Kre, d->Kat, d->mask);
+  if (rc) { delete h; return rc; }
+  hipFuncSetAttribute((const void*)k2d_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds2);
+  *out = h;
+  return DPM_OK;
+}
+
+extern "C" int dpm2d_set_cell_params(dpm_tissue* h, const float* Ka, const float* Kl,
+                                     const float* Kb, const float* a0, const float* l0,
+                                     const float* r0) {
+  if (!h || h->dim != 2) return fail(DPM_EINVAL, "not a 2D tissue");
+  const int C = h->C;
+  hipStream_t s = h->stream;
+  if (Ka) TRY(upload(h->Ka, Ka, C, s));
+  if (Kl) TRY(upload(h->Kl, Kl, C, s));
+  if (Kb) TRY(upload(h->Kb, Kb, C, s));
+  if (a0) TRY(upload(h->a0, a0, C, s));
+  if (l0) TRY(upload(h->l0_2, l0, C, s));
+  if (r0) TRY(upload(h->r0, r0, C, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  return DPM_OK;
+}
+
+extern "C" int dpm2d_set_tissue_params(dpm_tissue* h, const float* L, const int* pbc,
+                                       const float* Kre, const float* Kat, unsigned mask) {
+  if (!h || h->dim != 2) return fail(DPM_EINVAL, "not a 2D tissue");
+  const int T = h->T;
+  hipStream_t s = h->stream;
+  if (L) TRY(upload(h->tL, L, T, s));
+  if (pbc) TRY(upload(h->tpbc, pbc, T, s));
+  if (Kre) TRY(upload(h->tKre, Kre, T, s));
+  if (Kat) TRY(upload(h->tKat, Kat, T, s));
+  h->mask = mask & DPM2D_ALL;
+  HIPCHECK(hipStreamSynchronize(s));
+  return DPM_OK;
+}
+
+extern "C" int dpm2d_set_positions(dpm_tissue* h, const float* xy, size_t n) {
+  if (!h || h->dim != 2 || !xy) return fail(DPM_EINVAL, "bad arguments");
+  if (n != (size_t)h->V) return fail(DPM_EINVAL, "position count does not match the tissue");
+  std::vector<float> a(n), b(n);
+  for (size_t i = 0; i < n; i++) {
+    a[i] = xy[2 * i];
+    b[i] = xy[2 * i + 1];
+  }
+  TRY(upload(h->x2, a.data(), n, h->stream));
+  TRY(upload(h->y2, b.data(), n, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return DPM_OK;
+}
+
+static int download2(dpm_tissue* h, const float* a, const float* b, float* xy, size_t n) {
+  std::vector<float> A(n), B(n);
+  HIPCHECK(hipMemcpyAsync(A.data(), a, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipMemcpyAsync(B.data(), b, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  for (size_t i = 0; i < n; i++) {
+    xy[2 * i] = A[i];
+    xy[2 * i + 1] = B[i];
+  }
+  return DPM_OK;
+}
+
+extern "C" int dpm2d_get_positions(dpm_tissue* h, float* xy, size_t n) {
+  if (!h || h->dim != 2 || !xy) return fail(DPM_EINVAL, "bad arguments");
+  if (n != (size_t)h->V) return fail(DPM_EINVAL, "position count does not match the tissue");
+  return download2(h, h->x2, h->y2, xy, n);
+}
+
+extern "C" int dpm2d_get_forces(dpm_tissue* h, float* xy, size_t n) {
+  if (!h || h->dim != 2 || !xy) return fail(DPM_EINVAL, "bad arguments");
+  if (n != (size_t)h->V) return fail(DPM_EINVAL, "force count does not match the tissue");
+  return download2(h, h->fx2, h->fy2, xy, n);
+}
+
+static Run2D run2d_args(dpm_tissue* h, int nsteps, float dt) {
+  Run2D a{};
+  a.x = h->x2;
+  a.y = h->y2;
+  a.fx = h->fx2;
+  a.fy = h->fy2;
+  a.tissue_cell_begin = h->tcb;
+  a.cell_vert_begin = h->cvb;
+  a.Ka = h->Ka;
+  a.Kl = h->Kl;
+  a.Kb = h->Kb;
+  a.a0 = h->a0;
+  a.l0 = h->l0_2;
+  a.r0 = h->r0;
+  a.tL = h->tL;
+  a.tpbc = h->tpbc;
+  a.tKre = h->tKre;
+  a.tKat = h->tKat;
+  a.mask = h->mask;
+  a.nsteps = nsteps;
+  a.dt = dt;
+  a.counters = h->counters;
+  return a;
+}
+
+static int euler2d(dpm_tissue* h, int nsteps, float dt) {
+  if (nsteps <= 0) return fail(DPM_EINVAL, "nsteps must be positive");
+  Run2D a = run2d_args(h, nsteps, dt);
+  hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds2, h->stream, a);
+  HIPCHECK(hipGetLastError());
+  return DPM_OK;
+}
+
+extern "C" int dpm2d_probe(dpm_tissue* h, float* forces, int* overlap, int* attraction) {
+  if (!h || h->dim != 2) return fail(DPM_EINVAL, "not a 2D tissue");
+  hipStream_t s = h->stream;
+  int *dov = nullptr, *dat = nullptr;
+  int rc = dev_alloc(&dov, h->V);
+  rc |= dev_alloc(&dat, h->V);
+  if (!rc) {
+    hipMemsetAsync(dov, 0, h->V * sizeof(int), s);
+    hipMemsetAsync(dat, 0, h->V * sizeof(int), s);
+    Run2D a = run2d_args(h, 0, 0.0f);
+    a.counters = nullptr;
+    a.p_overlap = dov;
+    a.p_attr = dat;
+    hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds2, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail(DPM_EDEVICE, std::string("k2d_run: ") + hipGetErrorString(e));
+  }
+  if (!rc && overlap)
+    rc = hipMemcpyAsync(overlap, dov, h->V * sizeof(int), hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
+  if (!rc && attraction)
+    rc = hipMemcpyAsync(attraction, dat, h->V * sizeof(int), hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(DPM_EDEVICE, "stream sync failed");
+  if (dov) hipFree(dov);
+  if (dat) hipFree(dat);
+  if (rc) return rc;
+  if (forces) TRY(dpm2d_get_forces(h, forces, h->V));
+  return DPM_OK;
+}
+
+// ================================================================ common ==
+
+extern "C" int dpm_euler_async(dpm_tissue* h, int nsteps, float dt) {
+  if (!h) return fail(DPM_EINVAL, "null handle");
+  if (h->dim == 3) return euler3d(h, nsteps, dt);
+  return euler2d(h, nsteps, dt);
+}
+
+extern "C" int dpm_euler(dpm_tissue* h, int nsteps, float dt) {
+  TRY(dpm_euler_async(h, nsteps, dt));
+  if (h->dim == 3) {
+    const size_t n = (size_t)h->C * h->nv;
+    return check_finite(h, h->x[h->cur], h->y[h->cur], h->z[h->cur], n);
+  }
+  // The reference's 2D path performs no result check (Tissue2D.cpp:231-240).
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return DPM_OK;
+}

@@ -1,0 +1,127 @@
+// Device helpers shared by the 2D and 3D DPM kernels (gfx950, wave64).
+//
+// Arithmetic contract: every kernel source is compiled with
+// -ffp-contract=off and HIP's correctly rounded float divide and sqrt, so
+// each + - * / sqrt below is one IEEE-754 operation.  The expressions that
+// decide a pair or a face (winding denominators, crossing tests, distance
+// cut-offs, the facing-down test) are written in the same operation order
+// as the reference kernels, which makes those decisions bit-identical to a
+// CPU evaluation of the same order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define DPM_WAVE 64
+
+namespace dpm {
+
+struct f3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 sub(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 add(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 scale(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+// dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z, unfused.
+__device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// normalize(v) = v * (1/sqrt(dot(v,v))), normalize(0) = 0.
+__device__ __forceinline__ f3 normalize(f3 a) {
+  float d = dot(a, a);
+  if (d == 0.0f) return {0.0f, 0.0f, 0.0f};
+  float inv = 1.0f / sqrtf(d);
+  return {a.x * inv, a.y * inv, a.z * inv};
+}
+__device__ __forceinline__ void normalize2(float x, float y, float& ux, float& uy) {
+  float d = x * x + y * y;
+  if (d == 0.0f) { ux = 0.0f; uy = 0.0f; return; }
+  float inv = 1.0f / sqrtf(d);
+  ux = x * inv;
+  uy = y * inv;
+}
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, DPM_WAVE);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_down(v, o, DPM_WAVE));
+  return v;
+}
+
+// Block-wide sum of one double per thread; every thread gets the result.
+// scratch: >= 17 doubles of LDS.  Contains __syncthreads: call from all threads.
+__device__ __forceinline__ double block_sum(double v, double* scratch) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
+  const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
+  v = wave_sum(v);
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < nw; w++) s += scratch[w];
+    scratch[16] = s;
+  }
+  __syncthreads();
+  double r = scratch[16];
+  __syncthreads();
+  return r;
+}
+__device__ __forceinline__ float block_max(float v, float* scratch) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
+  const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
+  v = wave_max(v);
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float s = scratch[0];
+    for (int w = 1; w < nw; w++) s = fmaxf(s, scratch[w]);
+    scratch[16] = s;
+  }
+  __syncthreads();
+  float r = scratch[16];
+  __syncthreads();
+  return r;
+}
+
+// Order-preserving block compaction: threads with flag set write `value`
+// to list[] in thread order.  Returns the count to every thread.
+// counts: >= 17 ints of LDS.  Contains __syncthreads.
+__device__ __forceinline__ int block_compact(bool flag, int value, int* list, int* counts) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
+  const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
+  const unsigned long long m = __ballot(flag);
+  const int before = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) counts[wave] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < nw; w++) {
+      int c = counts[w];
+      counts[w] = s;
+      s += c;
+    }
+    counts[16] = s;
+  }
+  __syncthreads();
+  if (flag) list[counts[wave] + before] = value;
+  const int total = counts[16];
+  __syncthreads();
+  return total;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X
+// (workgroup b runs on XCD b % 8).  This bijection hands each XCD a
+// contiguous range of work items, so spatially neighbouring cells — which
+// read each other's vertices — share one XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  const int q = n / 8, r = n % 8;
+  const int x = b % 8, i = b / 8;
+  return x * q + (x < r ? x : r) + i;
+}
+
+}  // namespace dpm

@@ -1,0 +1,591 @@
+/*
+ * TEST INFRASTRUCTURE ONLY — the oracle.  Nothing in the product links, loads
+ * or calls this code; only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg use it, and only as the checker.
+ *
+ * Body of the CPU restatement of the reference DPM kernels, written once and
+ * instantiated twice by dpm_oracle.c:
+ *   R = float   -> "ref32": the reference arithmetic.  Variables are float and
+ *                  every unsuffixed literal of the reference is kept as a
+ *                  double literal here, so C's usual arithmetic conversions
+ *                  (FLT_EVAL_METHOD 0 on x86-64 SSE) reproduce the same
+ *                  float/double promotions the OpenCL C kernels get.
+ *   R = double -> "ref64": the same formulas, all in double.
+ * Built with -ffp-contract=off: every a*b+c is two rounded operations.
+ *
+ * Stated choices where the reference leaves arithmetic to its OpenCL runtime
+ * (parity unpinned at the bit level; pinned by the tests to closed forms,
+ * invariants and finite differences instead):
+ *   - normalize(v) = v * (1 / sqrt(dot(v,v))), and normalize(0) = 0;
+ *   - dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z, no fused multiply-add;
+ *   - length(v) = sqrt(dot(v,v));
+ *   - atan2 = the C library's atan2f / atan2;
+ *   - Euler update x += F*dt is fused (fmaf), as clang emits fmuladd there;
+ *   - the 3D volume is the volume of the current positions (the reference's
+ *     per-cell volume buffer is written by one work-item and read by the
+ *     others without a global barrier, Cell3D_Kernel.cl:78-83; the intended
+ *     value is the current volume).
+ */
+
+#ifndef FN
+#error "include from dpm_oracle.c"
+#endif
+
+/* ----------------------------------------------------------------- helpers */
+
+static inline R FN(dot2)(R ax, R ay, R bx, R by) { return ax * bx + ay * by; }
+static inline R FN(dot3)(const R a[3], const R b[3]) {
+  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+static inline void FN(cross3)(const R a[3], const R b[3], R c[3]) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline void FN(normalize3)(const R a[3], R u[3]) {
+  R d = FN(dot3)(a, a);
+  if (d == (R)0) { u[0] = u[1] = u[2] = (R)0; return; }
+  R inv = (R)1 / SQRT(d);
+  u[0] = a[0] * inv; u[1] = a[1] * inv; u[2] = a[2] * inv;
+}
+static inline void FN(normalize2)(R x, R y, R* ux, R* uy) {
+  R d = FN(dot2)(x, y, x, y);
+  if (d == (R)0) { *ux = *uy = (R)0; return; }
+  R inv = (R)1 / SQRT(d);
+  *ux = x * inv; *uy = y * inv;
+}
+
+/* ======================================================================= 2D
+ * Layout: padded AoS like the reference, V[(ci*stride + vi)*2 + {0,1}]
+ * (Tissue2D.cpp:117-140).  Only vi < NV[ci] is read or written.
+ */
+
+/* GetCOM, Cell2D_kernel.cl:3-11: float2 sum then divide by NV. */
+static void FN(com2)(const R* V, int stride, int ci, int nv, R* cx, R* cy) {
+  R sx = 0, sy = 0;
+  for (int i = 0; i < nv; i++) {
+    sx += V[(ci * stride + i) * 2 + 0];
+    sy += V[(ci * stride + i) * 2 + 1];
+  }
+  *cx = sx / (R)nv;
+  *cy = sy / (R)nv;
+}
+
+/* Point-in-polygon crossing parity of vertex p against cell cj with the
+ * reference's periodic quirk (floor in x, round in y, only when a component
+ * exceeds L), Cell2D_kernel.cl:171-196. */
+static int FN(crossing_parity)(const R* V, int stride, const int* NV, int cj,
+                               R px, R py, int PBC, R L) {
+  int odd = 0;
+  int n = NV[cj];
+  for (int i = 0, j = n - 1; i < n; j = i++) {
+    R dix = px - V[(cj * stride + i) * 2 + 0];
+    R diy = py - V[(cj * stride + i) * 2 + 1];
+    R djx = px - V[(cj * stride + j) * 2 + 0];
+    R djy = py - V[(cj * stride + j) * 2 + 1];
+    if (PBC) {
+      if (FABS(dix) > L || FABS(djx) > L) {
+        dix -= L * FLOOR(dix / L);
+        djx -= L * FLOOR(djx / L);
+      }
+      if (FABS(diy) > L || FABS(djy) > L) {
+        diy -= L * ROUND(diy / L);
+        djy -= L * ROUND(djy / L);
+      }
+    }
+    if ((diy > 0) != (djy > 0)) {
+      R t = (djx - dix) * (0 - diy) / (djy - diy) + dix;
+      if (0 < t) odd = !odd;
+    }
+  }
+  return odd;
+}
+
+/* One evaluation of all 2D force terms at the given positions, in the
+ * reference's kernel order Area, Perimeter, Bending, Attraction, Repulsion
+ * (Tissue2D.cpp:216-222).  F is overwritten.  overlap (optional, per padded
+ * slot) receives 1 where the repulsion pair (vertex, some other cell) exists;
+ * attr_count (optional) the number of attraction partners of each vertex. */
+int FN(ora2d_forces)(int ncells, int stride, const int* NV, const R* V,
+                     const R* Ka, const R* Kl, const R* Kb, const R* a0,
+                     const R* l0, const R* r0, R L, int PBC, R Kre, R Kat,
+                     unsigned mask, R* F, int* overlap, int* attr_count) {
+  for (int k = 0; k < ncells * stride * 2; k++) F[k] = 0;
+  if (overlap)
+    for (int k = 0; k < ncells * stride; k++) overlap[k] = 0;
+  if (attr_count)
+    for (int k = 0; k < ncells * stride; k++) attr_count[k] = 0;
+
+  ORA_PARALLEL_FOR
+  for (int ci = 0; ci < ncells; ci++) {
+    const int n = NV[ci];
+    const int base = ci * stride;
+#define X(k) V[(k) * 2 + 0]
+#define Y(k) V[(k) * 2 + 1]
+    /* AreaForceUpdates, Cell2D_kernel.cl:13-42.  The area is recomputed per
+     * work-item in the reference; it is the same value for every vertex. */
+    if (mask & 1u) {
+      R Area = 0.0f;
+      for (int vj = 0; vj < n; vj++) {
+        int idx = base + vj;
+        int im = (vj == 0) ? (base + n - 1) : (idx - 1);
+        Area += 0.5 * ((X(im) + X(idx)) * (Y(im) - Y(idx)));
+      }
+      if (Area < 0.0) Area = -Area;
+      R areaStrain = (Area / a0[ci]) - 1.0;
+      for (int vi = 0; vi < n; vi++) {
+        int index = base + vi;
+        int im1 = (vi == 0) ? (base + n - 1) : (index - 1);
+        int ip1 = (vi == n - 1) ? base : (index + 1);
+        F[index * 2 + 0] += (Ka[ci] / SQRT(a0[ci])) * 0.5 * areaStrain * (Y(im1) - Y(ip1));
+        F[index * 2 + 1] += (Ka[ci] / SQRT(a0[ci])) * 0.5 * areaStrain * (X(im1) - X(ip1));
+      }
+    }
+    for (int vi = 0; vi < n; vi++) {
+      int index = base + vi;
+      int im1 = (vi == 0) ? (base + n - 1) : (index - 1);
+      int ip1 = (vi == n - 1) ? base : (index + 1);
+      /* PerimeterForceUpdates, Cell2D_kernel.cl:92-119. */
+      if (mask & 2u) {
+        R lvx = X(ip1) - X(index), lvy = Y(ip1) - Y(index);
+        R lvmx = X(index) - X(im1), lvmy = Y(index) - Y(im1);
+        R length = SQRT(FN(dot2)(lvx, lvy, lvx, lvy));
+        R lengthm = SQRT(FN(dot2)(lvmx, lvmy, lvmx, lvmy));
+        R ulvx = lvx / length, ulvy = lvy / length;
+        R ulvmx = lvmx / lengthm, ulvmy = lvmy / lengthm;
+        R dli = length / l0[ci] - 1.0;
+        R dlim1 = lengthm / l0[ci] - 1.0;
+        R pref = Kl[ci] * SQRT(a0[ci] / l0[ci]);
+        F[index * 2 + 0] += pref * (dli * ulvx - dlim1 * ulvmx);
+        F[index * 2 + 1] += pref * (dli * ulvy - dlim1 * ulvmy);
+      }
+      /* BendingForceUpdates, Cell2D_kernel.cl:44-90. */
+      if (mask & 4u) {
+        int ip2 = (vi == n - 1) ? base + 1 : ((vi == n - 2) ? base : index + 2);
+        int im2 = (vi == 0) ? (base + n - 2) : ((vi == 1) ? (base + n - 1) : index - 2);
+        R lvx = X(ip1) - X(index), lvy = Y(ip1) - Y(index);
+        R lvxm = X(index) - X(im1), lvym = Y(index) - Y(im1);
+        R six = lvx - lvxm, siy = lvy - lvym;
+        R sixp = (X(ip2) - X(ip1)) - lvx, siyp = (Y(ip2) - Y(ip1)) - lvy;
+        R sixm = lvxm - (X(im1) - X(im2)), siym = lvym - (Y(im1) - Y(im2));
+        F[index * 2 + 0] += Kb[ci] * (2.0 * six - sixm - sixp);
+        F[index * 2 + 1] += Kb[ci] * (2.0 * siy - siym - siyp);
+      }
+      /* AttractionForceUpdate, Cell2D_kernel.cl:222-268. */
+      if (mask & 8u) {
+        for (int cj = 0; cj < ncells; cj++) {
+          if (cj == ci) continue;
+          for (int vj = 0; vj < NV[cj]; vj++) {
+            int j = cj * stride + vj;
+            R rx = X(j) - X(index), ry = Y(j) - Y(index);
+            if (PBC) {
+              rx -= L * ROUND(rx / L);
+              ry -= L * ROUND(ry / L);
+            }
+            R dist = SQRT(FN(dot2)(rx, ry, rx, ry));
+            if (dist < l0[ci]) {
+              R ftmp = Kat / (R)NV[ci] * dist / l0[ci];
+              R ux, uy;
+              FN(normalize2)(rx, ry, &ux, &uy);
+              F[index * 2 + 0] += ftmp * ux;
+              F[index * 2 + 1] += ftmp * uy;
+              if (attr_count) attr_count[index]++;
+            }
+          }
+        }
+      }
+      /* RepulsionForceUpdate, Cell2D_kernel.cl:121-220: overlap with the
+       * first cell whose crossing parity is odd, then a push toward the own
+       * centre of mass. */
+      if (mask & 16u) {
+        int overlaps = 0;
+        for (int cj = 0; cj < ncells && !overlaps; cj++) {
+          if (cj == ci) continue;
+          overlaps = FN(crossing_parity)(V, stride, NV, cj, X(index), Y(index), PBC, L);
+        }
+        if (overlaps) {
+          if (overlap) overlap[index] = 1;
+          R cx, cy;
+          FN(com2)(V, stride, ci, n, &cx, &cy);
+          R dx = cx - X(index), dy = cy - Y(index);
+          if (PBC) {
+            dx -= L * ROUND(dx / L);
+            dy -= L * ROUND(dy / L);
+          }
+          R dist = SQRT(FN(dot2)(dx, dy, dx, dy));
+          R xij = dist / (2 * r0[ci]);
+          R ftmp = Kre * (1 - xij);
+          R ux, uy;
+          FN(normalize2)(dx, dy, &ux, &uy);
+          F[index * 2 + 0] += 0.5f * ftmp * ux;
+          F[index * 2 + 1] += 0.5f * ftmp * uy;
+        }
+      }
+    }
+#undef X
+#undef Y
+  }
+  return 0;
+}
+
+/* Tissue2D::CLEulerUpdate's step loop, Tissue2D.cpp:215-229 with
+ * EulerUpdate, Cell2D_kernel.cl:270-281.  Fout = forces of the last step
+ * (read before the Euler kernel zeroes them). */
+int FN(ora2d_run)(int ncells, int stride, const int* NV, R* V, const R* Ka,
+                  const R* Kl, const R* Kb, const R* a0, const R* l0,
+                  const R* r0, R L, int PBC, R Kre, R Kat, unsigned mask,
+                  int nsteps, R dt, R* Fout) {
+  R* F = (R*)calloc((size_t)ncells * stride * 2, sizeof(R));
+  if (!F) return -1;
+  for (int s = 0; s < nsteps; s++) {
+    FN(ora2d_forces)(ncells, stride, NV, V, Ka, Kl, Kb, a0, l0, r0, L, PBC, Kre,
+                     Kat, mask, F, NULL, NULL);
+    for (int ci = 0; ci < ncells; ci++)
+      for (int vi = 0; vi < NV[ci]; vi++) {
+        int k = (ci * stride + vi) * 2;
+        V[k + 0] = FMA(F[k + 0], dt, V[k + 0]);
+        V[k + 1] = FMA(F[k + 1], dt, V[k + 1]);
+      }
+  }
+  if (Fout) memcpy(Fout, F, sizeof(R) * (size_t)ncells * stride * 2);
+  free(F);
+  return 0;
+}
+
+/* ======================================================================= 3D
+ * Layout: AoS X[(ci*nv + v)*3 + {0,1,2}], faces[f*3 + k] shared by all cells
+ * (Tissue3D.cpp:139-155).
+ */
+
+/* GetCOM, Cell3D_Kernel.cl:35-44: sum then multiply by 1/NV. */
+static void FN(com3)(const R* X, int nv, int ci, R c[3]) {
+  R s0 = 0, s1 = 0, s2 = 0;
+  for (int v = 0; v < nv; v++) {
+    s0 += X[(ci * nv + v) * 3 + 0];
+    s1 += X[(ci * nv + v) * 3 + 1];
+    s2 += X[(ci * nv + v) * 3 + 2];
+  }
+  R inv = (R)1.0f / (R)nv;
+  c[0] = s0 * inv; c[1] = s1 * inv; c[2] = s2 * inv;
+}
+
+/* getVolume, Cell3D_Kernel.cl:46-64. */
+static R FN(volume3)(const int* faces, int nf, const R* X, int nv, int ci) {
+  R vol = 0.0f;
+  for (int f = 0; f < nf; f++) {
+    const R* P0 = &X[(ci * nv + faces[f * 3 + 0]) * 3];
+    const R* P1 = &X[(ci * nv + faces[f * 3 + 1]) * 3];
+    const R* P2 = &X[(ci * nv + faces[f * 3 + 2]) * 3];
+    R c[3];
+    FN(cross3)(P0, P1, c);
+    vol += FN(dot3)(c, P2) / (R)6.0f;
+  }
+  return FABS(vol);
+}
+
+/* Solid angle sum of cell cj (shifted by `shift`) seen from point p,
+ * RepellingForces Cell3D_Kernel.cl:283-300.  Returns totalOmega. */
+static R FN(omega_sum)(const int* faces, int nf, const R* X, int nv, int cj,
+                       const R shift[3], const R p[3]) {
+  R total = 0.0f;
+  for (int f = 0; f < nf; f++) {
+    R a[3], b[3], c[3], u[3], v[3], w[3], vw[3];
+    const R* A = &X[(cj * nv + faces[f * 3 + 0]) * 3];
+    const R* B = &X[(cj * nv + faces[f * 3 + 1]) * 3];
+    const R* C = &X[(cj * nv + faces[f * 3 + 2]) * 3];
+    for (int k = 0; k < 3; k++) {
+      a[k] = A[k] + shift[k] - p[k];
+      b[k] = B[k] + shift[k] - p[k];
+      c[k] = C[k] + shift[k] - p[k];
+    }
+    FN(normalize3)(a, u);
+    FN(normalize3)(b, v);
+    FN(normalize3)(c, w);
+    R denom = (R)1.0f + FN(dot3)(u, v) + FN(dot3)(v, w) + FN(dot3)(w, u);
+    if (denom < 1e-8f) continue;
+    FN(cross3)(v, w, vw);
+    R num = FN(dot3)(u, vw);
+    total += (R)2.0f * ATAN2(num, denom);
+  }
+  return total;
+}
+
+/* Options of the 3D step that have no reference counterpart. */
+typedef struct {
+  unsigned mask;  /* 1 volume, 2 surface, 4 substrate, 8 contact,
+                     16 vertex-vertex adhesion, 32 bending */
+  R ks_grad;      /* substrate stiffness gradient along x (0 = uniform) */
+} FN(opts3d);
+
+/* One evaluation of the 3D forces, in the reference's launch order
+ * Volume, SurfaceArea, StickToSurface, Repelling (Tissue3D.cpp:374-414),
+ * then the two terms the reference does not launch: vertex-vertex adhesion
+ * (AllVertAttraction, Cell3D_Kernel.cl:313-364) and bending.  Face
+ * contributions are scattered in face order, one valid order of the
+ * reference's atomics.
+ * pairs (optional, nv*ncells*ncells bytes): contact pair flags
+ *   pairs[(ci*nv+v)*ncells + cj]
+ * wnum (optional, same shape, R): winding numbers of every evaluated pair
+ * down (optional, ncells*nf bytes): faces that pass the facing-down test
+ * adh_count (optional, ncells*nv ints): adhesion partners (own side) */
+int FN(ora3d_forces)(int ncells, int nv, int nf, const int* faces,
+                     const int* adj, int maxdeg, const R* X, const R* Kv,
+                     const R* Ka, const R* Ks, const R* Kb, const R* v0,
+                     const R* a0, R L, int PBC, R Kre, R Kat, unsigned mask,
+                     R ks_grad, R* F, unsigned char* pairs, R* wnum,
+                     unsigned char* down, int* adh_count) {
+  const size_t nvt = (size_t)ncells * nv;
+  for (size_t k = 0; k < nvt * 3; k++) F[k] = 0;
+  if (adh_count)
+    for (size_t k = 0; k < nvt; k++) adh_count[k] = 0;
+  R* l0 = (R*)malloc(sizeof(R) * ncells);
+  R* com = (R*)malloc(sizeof(R) * ncells * 3);
+  if (!l0 || !com) return -1;
+  for (int ci = 0; ci < ncells; ci++) {
+    /* Tissue3D.cpp:177 */
+    l0[ci] = SQRT((R)4.0f * a0[ci]) / SQRT((R)3.0f);
+    FN(com3)(X, nv, ci, &com[ci * 3]);
+  }
+
+  /* VolumeForceUpdate, Cell3D_Kernel.cl:66-112 */
+  if (mask & 1u) {
+    for (int ci = 0; ci < ncells; ci++) {
+      if (Kv[ci] == 0.0f) continue;
+      R volume = FN(volume3)(faces, nf, X, nv, ci);
+      R volumeStrain = (volume / v0[ci]) - 1.0f;
+      const R* C = &com[ci * 3];
+      for (int f = 0; f < nf; f++) {
+        int i0 = ci * nv + faces[f * 3 + 0];
+        int i1 = ci * nv + faces[f * 3 + 1];
+        int i2 = ci * nv + faces[f * 3 + 2];
+        R A[3], B[3], Cc[3], g0[3], g1[3], g2[3];
+        for (int k = 0; k < 3; k++) {
+          A[k] = X[i1 * 3 + k] - C[k];
+          B[k] = X[i2 * 3 + k] - C[k];
+          Cc[k] = X[i0 * 3 + k] - C[k];
+        }
+        FN(cross3)(A, B, g0);
+        FN(cross3)(B, Cc, g1);
+        FN(cross3)(Cc, A, g2);
+        R pre = -Kv[ci] * volumeStrain;
+        for (int k = 0; k < 3; k++) {
+          F[i0 * 3 + k] += pre * g0[k] / (R)6.0f;
+          F[i1 * 3 + k] += pre * g1[k] / (R)6.0f;
+          F[i2 * 3 + k] += pre * g2[k] / (R)6.0f;
+        }
+      }
+    }
+  }
+
+  /* SurfaceAreaForceUpdate, Cell3D_Kernel.cl:114-177 */
+  if (mask & 2u) {
+    for (int ci = 0; ci < ncells; ci++) {
+      if (Ka[ci] < 1e-8f) continue;
+      for (int f = 0; f < nf; f++) {
+        int idx[3] = {ci * nv + faces[f * 3 + 0], ci * nv + faces[f * 3 + 1],
+                      ci * nv + faces[f * 3 + 2]};
+        R lv[3][3], len[3], un[3][3], d[3];
+        for (int k = 0; k < 3; k++) {
+          lv[0][k] = X[idx[1] * 3 + k] - X[idx[0] * 3 + k];
+          lv[1][k] = X[idx[2] * 3 + k] - X[idx[1] * 3 + k];
+          lv[2][k] = X[idx[0] * 3 + k] - X[idx[2] * 3 + k];
+        }
+        for (int e = 0; e < 3; e++) len[e] = SQRT(FN(dot3)(lv[e], lv[e]));
+        if (len[0] < 1e-12f || len[1] < 1e-12f || len[2] < 1e-12f) continue;
+        for (int e = 0; e < 3; e++)
+          for (int k = 0; k < 3; k++) un[e][k] = lv[e][k] / len[e];
+        R target = l0[ci];
+        for (int e = 0; e < 3; e++) d[e] = (len[e] / target) - 1.0f;
+        R scale = Ka[ci] * SQRT(a0[ci]) / target * 0.3f;
+        for (int k = 0; k < 3; k++) {
+          R f0 = (un[0][k] * d[0]) - (un[2][k] * d[2]);
+          R f1 = (un[1][k] * d[1]) - (un[0][k] * d[0]);
+          R f2 = (un[2][k] * d[2]) - (un[1][k] * d[1]);
+          F[idx[0] * 3 + k] += f0 * scale;
+          F[idx[1] * 3 + k] += f1 * scale;
+          F[idx[2] * 3 + k] += f2 * scale;
+        }
+      }
+    }
+  }
+
+  /* StickToSurface, Cell3D_Kernel.cl:180-247 */
+  if (down)
+    for (int k = 0; k < ncells * nf; k++) down[k] = 0;
+  if (mask & 4u) {
+    for (int ci = 0; ci < ncells; ci++) {
+      if (Ks[ci] < 1e-12f) continue;
+      for (int f = 0; f < nf; f++) {
+        int idx[3] = {ci * nv + faces[f * 3 + 0], ci * nv + faces[f * 3 + 1],
+                      ci * nv + faces[f * 3 + 2]};
+        const R* P0 = &X[idx[0] * 3];
+        const R* P1 = &X[idx[1] * 3];
+        const R* P2 = &X[idx[2] * 3];
+        R A[3], B[3], nrm[3], un[3];
+        for (int k = 0; k < 3; k++) { A[k] = P1[k] - P0[k]; B[k] = P2[k] - P0[k]; }
+        FN(cross3)(A, B, nrm);
+        FN(normalize3)(nrm, un);
+        if (!(un[2] < -0.1f)) continue;
+        if (down) down[ci * nf + f] = 1;
+        const R* C = &com[ci * 3];
+        for (int i = 0; i < 3; i++) {
+          const R* pos = &X[idx[i] * 3];
+          R ks = Ks[ci];
+          if (ks_grad != (R)0) {
+            R xw = pos[0] - L * FLOOR(pos[0] / L);
+            ks = Ks[ci] * ((R)1 + ks_grad * xw / L);
+          }
+          if (pos[2] < 0.0f) F[idx[i] * 3 + 2] += ks * FABS(pos[2]);
+          R height = FABS(pos[2]);
+          if (height < l0[ci] * 2.0f) {
+            R c2v[3] = {pos[0] - C[0], pos[1] - C[1], (R)0.0f - C[2]};
+            R dz[3] = {(R)0, (R)0, pos[2] - (R)0.0f};
+            R dist = SQRT(FN(dot3)(dz, dz));
+            R ftmp = ks * (1.0 - dist / l0[ci]);
+            R u[3];
+            FN(normalize3)(c2v, u);
+            for (int k = 0; k < 3; k++) F[idx[i] * 3 + k] += u[k] * ftmp;
+          }
+        }
+      }
+    }
+  }
+
+  /* RepellingForces, Cell3D_Kernel.cl:251-310 */
+  if (pairs) memset(pairs, 0, nvt * (size_t)ncells);
+  if (wnum)
+    for (size_t k = 0; k < nvt * (size_t)ncells; k++) wnum[k] = 0;
+  if ((mask & 8u) && Kre != 0.0f) {
+    const R fourpi = (R)4.0f * (R)3.14159265358979323846;
+    ORA_PARALLEL_FOR
+    for (int ci = 0; ci < ncells; ci++) {
+      const R* Ci = &com[ci * 3];
+      for (int v = 0; v < nv; v++) {
+        size_t gv = (size_t)ci * nv + v;
+        const R* p = &X[gv * 3];
+        for (int cj = 0; cj < ncells; cj++) {
+          if (cj == ci) continue;
+          R shift[3] = {0, 0, 0};
+          if (PBC)
+            for (int k = 0; k < 3; k++)
+              shift[k] = L * ROUND((Ci[k] - com[cj * 3 + k]) / L);
+          R tot = FN(omega_sum)(faces, nf, X, nv, cj, shift, p);
+          R wn = tot / fourpi;
+          if (wnum) wnum[gv * ncells + cj] = wn;
+          if (FABS(wn) < 1e-6f) continue;
+          if (pairs) pairs[gv * ncells + cj] = 1;
+          R dv[3] = {Ci[0] - p[0], Ci[1] - p[1], Ci[2] - p[2]}, dir[3];
+          FN(normalize3)(dv, dir);
+          R mag = FABS(wn) * 0.5f * Kre;
+          for (int k = 0; k < 3; k++) F[gv * 3 + k] += mag * dir[k];
+        }
+      }
+    }
+  }
+
+  /* AllVertAttraction, Cell3D_Kernel.cl:313-364 (not launched by the
+   * reference host; enabled here by mask bit 16). */
+  if ((mask & 16u) && Kat != 0.0f) {
+    for (int ci = 0; ci < ncells; ci++) {
+      R L0 = l0[ci];
+      if (L0 < 1e-12f) L0 = 1e-12f;
+      for (int vi = 0; vi < nv; vi++) {
+        size_t a = (size_t)ci * nv + vi;
+        for (int cj = 0; cj < ncells; cj++) {
+          if (cj == ci) continue;
+          for (int vj = 0; vj < nv; vj++) {
+            size_t b = (size_t)cj * nv + vj;
+            R d[3];
+            for (int k = 0; k < 3; k++) d[k] = X[b * 3 + k] - X[a * 3 + k];
+            if (PBC)
+              for (int k = 0; k < 3; k++) d[k] -= L * ROUND(d[k] / L);
+            R dist = SQRT(FN(dot3)(d, d));
+            if (dist < L0 * 2.0f && dist > 1e-12f) {
+              for (int k = 0; k < 3; k++) {
+                R t = Kat * 0.5f * (dist / L0 - 1.0f) * (d[k] / dist);
+                F[a * 3 + k] += -t;
+                F[b * 3 + k] += t;
+              }
+              if (adh_count) adh_count[a]++;
+            }
+          }
+        }
+      }
+    }
+  }
+
+  /* Bending (no reference counterpart): F = -Kb * L(L x) with L the
+   * unnormalised graph Laplacian of the mesh edge graph, so that on a cycle
+   * graph it is the reference 2D bending Kb(2s_i - s_{i-1} - s_{i+1}),
+   * Cell2D_kernel.cl:80-89.  adj[v*maxdeg + k] lists the neighbours of v in
+   * ascending order, -1 padded. */
+  if ((mask & 32u) && adj) {
+    R* s = (R*)malloc(sizeof(R) * nv * 3);
+    if (!s) return -1;
+    for (int ci = 0; ci < ncells; ci++) {
+      if (Kb[ci] == (R)0) continue;
+      for (int v = 0; v < nv; v++) {
+        R acc[3] = {0, 0, 0};
+        const R* xv = &X[((size_t)ci * nv + v) * 3];
+        for (int k = 0; k < maxdeg; k++) {
+          int u = adj[v * maxdeg + k];
+          if (u < 0) break;
+          const R* xu = &X[((size_t)ci * nv + u) * 3];
+          for (int c = 0; c < 3; c++) acc[c] += xu[c] - xv[c];
+        }
+        for (int c = 0; c < 3; c++) s[v * 3 + c] = acc[c];
+      }
+      for (int v = 0; v < nv; v++) {
+        R acc[3] = {0, 0, 0};
+        int deg = 0;
+        for (int k = 0; k < maxdeg; k++) {
+          int u = adj[v * maxdeg + k];
+          if (u < 0) break;
+          deg++;
+          for (int c = 0; c < 3; c++) acc[c] += s[u * 3 + c];
+        }
+        for (int c = 0; c < 3; c++)
+          F[((size_t)ci * nv + v) * 3 + c] += -Kb[ci] * (acc[c] - (R)deg * s[v * 3 + c]);
+      }
+    }
+    free(s);
+  }
+  free(l0);
+  free(com);
+  return 0;
+}
+
+/* Tissue3D::CLEulerUpdate's step loop, Tissue3D.cpp:372-445: clear, forces,
+ * EulerPosition (Cell3D_Kernel.cl:371-381).  Fout = forces of the last step. */
+int FN(ora3d_run)(int ncells, int nv, int nf, const int* faces, const int* adj,
+                  int maxdeg, R* X, const R* Kv, const R* Ka, const R* Ks,
+                  const R* Kb, const R* v0, const R* a0, R L, int PBC, R Kre,
+                  R Kat, unsigned mask, R ks_grad, int nsteps, R dt, R* Fout) {
+  size_t n3 = (size_t)ncells * nv * 3;
+  R* F = (R*)calloc(n3, sizeof(R));
+  if (!F) return -1;
+  for (int s = 0; s < nsteps; s++) {
+    int rc = FN(ora3d_forces)(ncells, nv, nf, faces, adj, maxdeg, X, Kv, Ka, Ks,
+                              Kb, v0, a0, L, PBC, Kre, Kat, mask, ks_grad, F,
+                              NULL, NULL, NULL, NULL);
+    if (rc) { free(F); return rc; }
+    for (size_t k = 0; k < n3; k++) X[k] = FMA(F[k], dt, X[k]);
+  }
+  if (Fout) memcpy(Fout, F, sizeof(R) * n3);
+  free(F);
+  return 0;
+}
+
+/* Per-cell volume (getVolume), exposed for tests. */
+void FN(ora3d_volumes)(int ncells, int nv, int nf, const int* faces,
+                       const R* X, R* out) {
+  for (int ci = 0; ci < ncells; ci++) out[ci] = FN(volume3)(faces, nf, X, nv, ci);
+}
+
+/* Winding number of point p about cell cj (no shift), exposed for tests. */
+R FN(ora3d_winding)(int nv, int nf, const int* faces, const R* X, int cj,
+                    const R* p) {
+  R shift[3] = {0, 0, 0};
+  return FN(omega_sum)(faces, nf, X, nv, cj, shift, p) /
+         ((R)4.0f * (R)3.14159265358979323846);
+}

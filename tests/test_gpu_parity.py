@@ -1,0 +1,262 @@
+"""Parity of the HIP timestep with the CPU oracle, through the C ABI.
+
+Bars (stated per test):
+  * forces: ||F_gpu - F_ref64|| / ||F_ref64|| <= 1e-5 and every vertex within
+    1e-4 * max|F_ref64|, from identical float inputs;
+  * pair sets (3D contact vertex-cell pairs, facing-down faces, adhesion
+    partners; 2D overlap and attraction partners): bit-exact against the
+    float restatement ref32, whose decisions use the same IEEE operation
+    order as the kernels;
+  * batching, graph replay and repeated runs: bit-identical to the plain path.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from opencl_dpm_amd import capi, clDPM
+from opencl_dpm_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FORCE_RTOL = 1e-5
+FORCE_VTOL = 1e-4
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def assert_forces(Fg, F64, what):
+    r = rel(Fg, F64)
+    vmax = float(np.abs(np.asarray(Fg, np.float64) - F64).max())
+    scale = float(np.abs(F64).max())
+    print(f"{what}: rel={r:.3e} vmax={vmax:.3e} scale={scale:.3e}", flush=True)
+    assert r <= FORCE_RTOL, (what, r)
+    assert vmax <= FORCE_VTOL * scale, (what, vmax, scale)
+
+
+def tissue3d(m, mask=capi.DPM3D_ALL, Kat=None, Kre=None):
+    return capi.Tissues3D(m.mesh.faces, [0, m.C], m.Kv, m.Ka, m.Ks, m.v0, m.a0, m.L,
+                          m.Kre if Kre is None else Kre, Kat=m.Kat if Kat is None else Kat,
+                          Kb=m.Kb, ks_grad=m.ks_grad, mask=mask, nv=m.mesh.nv)
+
+
+def ref3d(m, X, mask, prec, pairs=False):
+    return oracle.forces3d(X, m.mesh.faces, m.mesh.nv, m.Kv, m.Ka, m.Ks, m.v0, m.a0, m.L, m.Kre, Kat=m.Kat,
+                           Kb=m.Kb, mask=mask, ks_grad=m.ks_grad, prec=prec, pairs=pairs)
+
+
+CASES_3D = {
+    # name: (n_side, spacing, subdiv, seed)
+    "grid3x3": (3, 2.4, 2, 0),
+    "compressed3x3": (3, 1.7, 2, 1),
+    "compressed4x4": (4, 1.8, 2, 2),
+    "subdiv3_2x2": (2, 1.8, 3, 3),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES_3D))
+@pytest.mark.parametrize("mask", [capi.DPM3D_REFERENCE, capi.DPM3D_ALL])
+def test_3d_forces_and_pairs(case, mask):
+    n, sp, sd, seed = CASES_3D[case]
+    m = W.monolayer3d(n, sp, subdiv=sd, seed=seed)
+    t = tissue3d(m, mask)
+    t.set_positions(m.X)
+    g = t.probe()
+    r64 = ref3d(m, m.X, mask, 64)
+    r32 = ref3d(m, m.X, mask, 32, pairs=True)
+    assert_forces(g["forces"], r64["forces"], f"3d {case} mask={mask}")
+    np.testing.assert_array_equal(g["contact"], r32["contact"])
+    np.testing.assert_array_equal(g["down"], r32["down"])
+    np.testing.assert_array_equal(g["adhesion"], r32["adhesion"])
+    print(f"pairs: contact={int(g['contact'].sum())} down={int(g['down'].sum())} "
+          f"adhesion={int(g['adhesion'].sum())}", flush=True)
+    if case.startswith("compressed") and mask & capi.DPM3D_CONTACT:
+        assert g["contact"].sum() > 0
+
+
+def test_3d_substrate_gradient_forces():
+    m = W.monolayer_config("cfg5")
+    t = tissue3d(m)
+    t.set_positions(m.X)
+    assert_forces(t.probe()["forces"], ref3d(m, m.X, capi.DPM3D_ALL, 64)["forces"], "3d cfg5 ks_grad=9")
+
+
+def test_3d_trajectory_vs_ref64():
+    m = W.monolayer3d(3, 1.7, seed=4)
+    t = tissue3d(m)
+    t.set_positions(m.X)
+    nsteps, dt = 40, 0.002
+    t.euler(nsteps, dt)
+    Xg = t.positions()
+    X64, F64 = oracle.run3d(m.X, m.mesh.faces, m.mesh.nv, m.Kv, m.Ka, m.Ks, m.v0, m.a0, m.L, m.Kre, nsteps, dt,
+                            Kat=m.Kat, Kb=m.Kb, mask=capi.DPM3D_ALL, prec=64)
+    err = float(np.abs(Xg - X64).max())
+    disp = float(np.abs(X64 - m.X).max())
+    print(f"40-step positions: max|dx|={err:.3e} (max displacement {disp:.3e})", flush=True)
+    assert err <= 1e-4 * max(disp, 1e-3) + 4e-6 * float(np.abs(X64).max())
+    assert_forces(t.forces(), F64, "3d last-step forces")
+
+
+def test_3d_full_size_shape_terms_and_determinism():
+    m = W.monolayer_config("cfg2")
+    shape = capi.DPM3D_VOLUME | capi.DPM3D_SURFACE | capi.DPM3D_SUBSTRATE | capi.DPM3D_BENDING
+    t = tissue3d(m, shape)
+    t.set_positions(m.X)
+    assert_forces(t.probe()["forces"], ref3d(m, m.X, shape, 64)["forces"], "3d cfg2 shape terms")
+    full = tissue3d(m)
+    full.set_positions(m.X)
+    full.euler(100, m.dt)
+    a, fa = full.positions(), full.forces()
+    full.set_positions(m.X)
+    full.euler(100, m.dt)
+    np.testing.assert_array_equal(a, full.positions())
+    np.testing.assert_array_equal(fa, full.forces())
+    assert np.isfinite(a).all()
+
+
+def test_3d_graph_replay_equals_eager():
+    m = W.monolayer3d(4, 1.9, seed=5)
+    t = tissue3d(m)
+    t.set_positions(m.X)
+    t.euler(150, 0.002)          # 149 steps through two 64-step graphs + eager
+    a = t.positions()
+    t.set_positions(m.X)
+    for _ in range(150):
+        t.euler(1, 0.002)        # eager, one launch per call
+    np.testing.assert_array_equal(a, t.positions())
+
+
+def test_3d_batch_equals_individual():
+    ms = [W.monolayer3d(3, 1.8, seed=10), W.monolayer3d(2, 1.9, seed=11), W.monolayer3d(4, 2.1, seed=12)]
+    ms[1].Kre, ms[1].Kat = 2.0, 0.3
+    cb = np.concatenate([[0], np.cumsum([m.C for m in ms])])
+    cat = lambda f: np.concatenate([np.broadcast_to(np.asarray(f(m), np.float32), (m.C,)) for m in ms])
+    b = capi.Tissues3D(ms[0].mesh.faces, cb, cat(lambda m: m.Kv), cat(lambda m: m.Ka), cat(lambda m: m.Ks),
+                       cat(lambda m: m.v0), cat(lambda m: m.a0), [m.L for m in ms], [m.Kre for m in ms],
+                       Kat=[m.Kat for m in ms], Kb=cat(lambda m: m.Kb), mask=capi.DPM3D_ALL, nv=ms[0].mesh.nv)
+    b.set_positions(np.concatenate([m.X for m in ms]))
+    b.euler(30, 0.002)
+    Xb = b.positions()
+    off = 0
+    for m in ms:
+        t = tissue3d(m)
+        t.set_positions(m.X)
+        t.euler(30, 0.002)
+        n = m.C * m.mesh.nv
+        np.testing.assert_array_equal(Xb[off:off + n], t.positions())
+        off += n
+
+
+def test_3d_dropin_class_matches_c_abi():
+    m = W.monolayer3d(3, 1.8, seed=6, keep_cells=True)
+    for c in m.cells:
+        c.Kv, c.Ka, c.Ks, c.Kb = m.Kv, m.Ka, m.Ks, m.Kb
+    T = clDPM.Tissue3D(m.cells, 0.35)
+    T.Kre, T.Kat, T.VertexAdhesion, T.Verbose = m.Kre, m.Kat, True, False
+    L = T.L
+    T.CLEulerUpdate(20, 0.002)
+    Xd = np.concatenate([np.asarray(c.Verts, np.float32) for c in T.Cells])
+    Fd = np.concatenate([np.asarray(c.Forces, np.float32) for c in T.Cells])
+    t = capi.Tissues3D(m.mesh.faces, [0, m.C], m.Kv, m.Ka, m.Ks, m.v0, m.a0, L, m.Kre, Kat=m.Kat, Kb=m.Kb,
+                       mask=capi.DPM3D_ALL, nv=m.mesh.nv)
+    t.set_positions(m.X)
+    t.euler(20, 0.002)
+    np.testing.assert_array_equal(Xd, t.positions())
+    np.testing.assert_array_equal(Fd, t.forces())
+    vol, com = t.cell_stats()
+    host = np.array([c.GetVolume() for c in T.Cells])
+    np.testing.assert_allclose(vol, host, rtol=1e-5)
+
+
+# ----------------------------------------------------------------------- 2D
+
+def tissue2d_handle(p: W.Packing2D, mask=capi.DPM2D_ALL):
+    return capi.Tissues2D([0, p.C], p.nv, p.Ka, p.Kl, p.Kb, p.a0, p.l0, p.r0, p.L, Kre=p.Kre, Kat=p.Kat,
+                          mask=mask)
+
+
+def ref2d(p: W.Packing2D, X, prec, mask=31):
+    P, stride = W.Packing2D(p.nv, X, p.a0, p.l0, p.r0, p.Ka, p.Kl, p.Kb, p.L).padded()
+    F, ov, at = oracle.forces2d(P, stride, p.nv, p.Ka, p.Kl, p.Kb, p.a0, p.l0, p.r0, p.L, p.Kre, p.Kat,
+                                mask=mask, prec=prec)
+    keep = np.concatenate([np.arange(c * stride, c * stride + p.nv[c]) for c in range(p.C)])
+    return F[keep], ov[keep], at[keep]
+
+
+CASES_2D = {
+    # name: (ncells, nv, phi, Kb, Kat, seed, nv_range)
+    "cfg1": (16, 24, 0.6, 0.0, 0.0, 0, None),
+    "dense_attr": (16, 24, 0.92, 0.01, 0.5, 1, None),
+    "uneven": (40, 0, 0.88, 0.01, 0.3, 2, (16, 64)),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES_2D))
+def test_2d_forces_and_pairs(case):
+    nc, nv, phi, Kb, Kat, seed, rng = CASES_2D[case]
+    p = W.tissue2d(nc, nv, 1.4, phi, Kb=Kb, Kat=Kat, seed=seed, nv_range=rng)
+    t = tissue2d_handle(p)
+    t.set_positions(p.X)
+    g = t.probe()
+    F64, _, _ = ref2d(p, p.X, 64)
+    F32, ov32, at32 = ref2d(p, p.X, 32)
+    assert_forces(g["forces"], F64, f"2d {case}")
+    np.testing.assert_array_equal(g["overlap"], ov32)
+    np.testing.assert_array_equal(g["attraction"], at32)
+    print(f"pairs: overlap={int(ov32.sum())} attraction={int(at32.sum())}", flush=True)
+
+
+def test_2d_cfg1_1000_steps_vs_ref64():
+    p = W.tissue2d(16, 24, 1.4, 0.6, seed=0)
+    t = tissue2d_handle(p)
+    t.set_positions(p.X)
+    t.euler(1000, 0.005)
+    P, stride = p.padded()
+    X64, F64 = oracle.run2d(P, stride, p.nv, p.Ka, p.Kl, p.Kb, p.a0, p.l0, p.r0, p.L, p.Kre, p.Kat, 1000, 0.005,
+                            prec=64)
+    err = float(np.abs(t.positions() - X64).max())
+    disp = float(np.abs(X64 - p.X).max())
+    print(f"2d cfg1 1000 steps: max|dx|={err:.3e}, displacement {disp:.3e}", flush=True)
+    assert err <= 1e-4 * max(disp, 1e-3) + 4e-6 * float(np.abs(X64).max())
+    assert_forces(t.forces(), F64, "2d cfg1 last-step forces")
+
+
+def test_2d_batch_equals_individual():
+    ps = W.packings2d(6, cells=(8, 40), verts=(16, 40), seed=3, phi=0.85)
+    cb = np.concatenate([[0], np.cumsum([p.C for p in ps])])
+    cat = lambda f: np.concatenate([f(p) for p in ps])
+    b = capi.Tissues2D(cb, cat(lambda p: p.nv), cat(lambda p: p.Ka), cat(lambda p: p.Kl), cat(lambda p: p.Kb),
+                       cat(lambda p: p.a0), cat(lambda p: p.l0), cat(lambda p: p.r0), [p.L for p in ps])
+    b.set_positions(cat(lambda p: p.X))
+    b.euler(200, 0.005)
+    Xb = b.positions()
+    off = 0
+    for p in ps:
+        t = tissue2d_handle(p)
+        t.set_positions(p.X)
+        t.euler(200, 0.005)
+        np.testing.assert_array_equal(Xb[off:off + len(p.X)], t.positions())
+        off += len(p.X)
+
+
+def test_2d_dropin_class_matches_c_abi():
+    p = W.tissue2d(16, 24, 1.4, 0.6, Kat=0.2, seed=7)
+    cells = []
+    off = 0
+    for c in range(p.C):
+        cell = clDPM.Cell2D(0.0, 0.0, 1.0, int(p.nv[c]), float(p.r0[c]))
+        cell.Verts = p.X[off:off + p.nv[c]].tolist()
+        cell.Ka, cell.Kl, cell.Kb = 1.0, 1.0, 0.0
+        cells.append(cell)
+        off += p.nv[c]
+    T = clDPM.Tissue2D(cells, 0.7745966692414834)
+    T.Kat = 0.2
+    T.CLEulerUpdate(300, 0.005)
+    Xd = np.concatenate([np.asarray(c.Verts, np.float32) for c in T.Cells])
+    a0 = np.array([c.a0 for c in cells], np.float32)
+    l0 = np.array([c.l0 for c in cells], np.float32)
+    t = capi.Tissues2D([0, p.C], p.nv, 1.0, 1.0, 0.0, a0, l0, p.r0, T.L, Kre=1.0, Kat=0.2)
+    t.set_positions(p.X)
+    t.euler(300, 0.005)
+    np.testing.assert_array_equal(Xd, t.positions())
