@@ -274,17 +274,23 @@ __device__ __noinline__ void force_vertex(const Run2D& a, const Tissue2DCtx& T, 
   if (mask & M2_ATTRACTION) {
     const float l0 = a.l0[gc];
     const float kat = T.Kat / (float)n;
+    const float half_l = 0.49f * L;
+    const float lim = l0 * l0 * 1.00001f;
     int cnt = 0;
     for (int q = 0; q < ncl; q++) {
       const int cj = scan_all ? q : (int)l.cand[ci * DPM2D_MAXCAND + q];
       if (cj == ci) continue;
       for (int j = l.vb[cj]; j < l.vb[cj + 1]; j++) {
         float rx = l.x[j] - px, ry = l.y[j] - py;
-        if (T.pbc) {
+        // minimum image; a component below 0.49 L rounds to image 0, which
+        // leaves it unchanged, so the division is skipped there
+        if (T.pbc && !(fabsf(rx) < half_l && fabsf(ry) < half_l)) {
           rx -= L * roundf(rx / L);
           ry -= L * roundf(ry / L);
         }
-        const float dist = sqrtf(rx * rx + ry * ry);
+        const float d2 = rx * rx + ry * ry;
+        if (!(d2 < lim)) continue;  // sqrt(d2) < l0 needs d2 < l0^2 (1 + 1e-5)
+        const float dist = sqrtf(d2);
         if (dist < l0) {
           const float ftmp = kat * dist / l0;
           float ux, uy;

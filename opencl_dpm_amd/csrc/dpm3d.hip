@@ -2,11 +2,12 @@
 //
 // One workgroup owns one cell for a whole step (grid = cells of all tissues,
 // workgroups spread over the 8 XCDs so that neighbouring cells share an L2).
-// The cell's vertices live in LDS for the step.  Every force term of the
-// reference's per-face kernels is evaluated by *gathering* over the faces
-// incident to a vertex in a fixed order, so no float atomics are used and
-// a step is bit-reproducible run to run (the reference scatters with a
-// compare-and-swap float add, Cell3D_Kernel.cl:10-32, whose order varies).
+// The cell's vertices and the mesh's face table live in LDS for the step.
+// Every force term of the reference's per-face kernels is evaluated by
+// *gathering* over the faces incident to a vertex in a fixed order, so no
+// float atomics are used and a step is bit-reproducible run to run (the
+// reference scatters with a compare-and-swap float add,
+// Cell3D_Kernel.cl:10-32, whose order varies).
 //
 // Positions are double-buffered (read set -> write set) and the per-cell
 // statistics the next step needs (centre of mass, bounding radius, volume)
@@ -26,6 +27,7 @@
 using namespace dpm;
 
 #define DPM3D_MAXDEG 8
+#define DPM3D_MAXCLUS 64
 
 enum : unsigned {
   M_VOLUME = 1u,
@@ -68,10 +70,15 @@ struct Step3D {
   const float* tKre;
   const float* tKat;
   const float* tksg;
-  const int* faces;  // [nf*3]
-  const int* vf;     // [nv*MAXDEG] incident (face*3 + corner), ascending, -1 pad
-  const int* adj;    // [nv*MAXDEG] neighbour vertices, ascending, -1 pad
+  const int* faces;        // [nf*3]
+  const int* vf;           // [nv*MAXDEG] incident (face*3 + corner), ascending, -1 pad
+  const int* adj;          // [nv*MAXDEG] neighbour vertices, ascending, -1 pad
+  const int* clus_of;      // [nv] cluster of each vertex (cluster k is seeded at vertex k)
+  const int* clus_begin;   // [nclus+1]
+  const int* clus_members; // [nv] vertices grouped by cluster
+  int nclus;
   int nv, nf, ncells;
+  int ww;                  // waves that evaluate winding numbers (LDS tables)
   float dt;
   unsigned mask;
   int flags;
@@ -94,39 +101,77 @@ __device__ __forceinline__ float contact_margin(float Rj, float l0j) {
   return 2.0f * l0j + 0.1f * Rj;
 }
 
-// LDS layout shared by both kernels (sizes in elements):
-//   float own[3*nv] | float nbr[3*nv] | float bend[3*nv] | float wnum[nv]
-//   int clist[NT] | int plist[NT] | int cnt[32] | (8-aligned) double red[32]
-__host__ __device__ inline size_t lds3d_bytes(int nv, int nt) {
-  size_t b = sizeof(float) * 10u * (size_t)nv + sizeof(int) * (2u * (size_t)nt + 32u);
-  b = (b + 7u) & ~(size_t)7u;
-  return b + sizeof(double) * 32u;
+// LDS layout (byte offsets), shared by host (sizing) and device (carving).
+struct Lds3DLayout {
+  size_t own, nbr, bend, wnum, faces, members, clist, plist, cnt, crad, utab, fc, red, total;
+};
+__host__ __device__ inline Lds3DLayout lds3d_layout(int nv, int nf, int nt, int ww) {
+  Lds3DLayout L;
+  size_t o = 0;
+  L.own = o;     o += 12u * (size_t)nv;
+  L.nbr = o;     o += 12u * (size_t)nv;
+  L.bend = o;    o += 12u * (size_t)nv;
+  L.wnum = o;    o += 4u * (size_t)nv;
+  L.faces = o;   o += 6u * (size_t)nf;
+  L.members = o; o += 2u * (size_t)nv;
+  o = (o + 3u) & ~(size_t)3u;
+  L.clist = o;   o += 4u * (size_t)nt;
+  L.plist = o;   o += 4u * (size_t)nt;
+  L.cnt = o;     o += 4u * 32u;
+  L.crad = o;    o += 4u * (2u * DPM3D_MAXCLUS + 1u);  // radii (uint bits) + member offsets
+  L.utab = o;    o += 12u * (size_t)nv * (size_t)ww;
+  L.fc = o;      o += 36u * (size_t)nf;                // per (face, corner) shape forces
+  o = (o + 7u) & ~(size_t)7u;
+  L.red = o;     o += 8u * 32u;
+  L.total = o;
+  return L;
 }
 
 struct Lds3D {
-  float *sx, *sy, *sz, *nx, *ny, *nz, *sb, *wb;
-  int *clist, *plist, *cnt;
+  float *sx, *sy, *sz, *nx, *ny, *nz, *sb, *wb, *utab, *fcx, *fcy, *fcz;
+  unsigned short *sf, *smem_members;
+  int *clist, *plist, *cnt, *cbeg;
+  unsigned* crad;
   double* red;
 };
-__device__ __forceinline__ Lds3D carve3d(unsigned char* smem, int nv, int nt) {
+__device__ __forceinline__ Lds3D carve3d(unsigned char* smem, int nv, int nf, int nt, int ww) {
+  const Lds3DLayout L = lds3d_layout(nv, nf, nt, ww);
   Lds3D l;
-  float* f = reinterpret_cast<float*>(smem);
-  l.sx = f;
-  l.sy = f + nv;
-  l.sz = f + 2 * nv;
-  l.nx = f + 3 * nv;
-  l.ny = f + 4 * nv;
-  l.nz = f + 5 * nv;
-  l.sb = f + 6 * nv;
-  l.wb = f + 9 * nv;
-  int* ip = reinterpret_cast<int*>(f + 10 * nv);
-  l.clist = ip;
-  l.plist = ip + nt;
-  l.cnt = ip + 2 * nt;
-  size_t off = sizeof(float) * 10u * (size_t)nv + sizeof(int) * (2u * (size_t)nt + 32u);
-  off = (off + 7u) & ~(size_t)7u;
-  l.red = reinterpret_cast<double*>(smem + off);
+  float* o = reinterpret_cast<float*>(smem + L.own);
+  l.sx = o;
+  l.sy = o + nv;
+  l.sz = o + 2 * nv;
+  float* n = reinterpret_cast<float*>(smem + L.nbr);
+  l.nx = n;
+  l.ny = n + nv;
+  l.nz = n + 2 * nv;
+  l.sb = reinterpret_cast<float*>(smem + L.bend);
+  l.wb = reinterpret_cast<float*>(smem + L.wnum);
+  l.sf = reinterpret_cast<unsigned short*>(smem + L.faces);
+  l.smem_members = reinterpret_cast<unsigned short*>(smem + L.members);
+  l.clist = reinterpret_cast<int*>(smem + L.clist);
+  l.plist = reinterpret_cast<int*>(smem + L.plist);
+  l.cnt = reinterpret_cast<int*>(smem + L.cnt);
+  l.crad = reinterpret_cast<unsigned*>(smem + L.crad);
+  l.cbeg = reinterpret_cast<int*>(smem + L.crad) + DPM3D_MAXCLUS;
+  l.utab = reinterpret_cast<float*>(smem + L.utab);
+  l.fcx = reinterpret_cast<float*>(smem + L.fc);
+  l.fcy = l.fcx + 3 * nf;
+  l.fcz = l.fcy + 3 * nf;
+  l.red = reinterpret_cast<double*>(smem + L.red);
   return l;
+}
+
+// Orders this wave's LDS writes before its later LDS reads (the LDS unit
+// executes one wave's operations in order; this keeps the compiler from
+// reordering them and waits for outstanding LDS traffic).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ f3 lds_v(const float* x, const float* y, const float* z, int i) {
+  return mk(x[i], y[i], z[i]);
 }
 
 // Centre of mass (double sums), bounding radius and volume of the cell whose
@@ -135,8 +180,7 @@ __device__ __forceinline__ Lds3D carve3d(unsigned char* smem, int nv, int nt) {
 // terms in float, the sum in double — the reference sums absolute-coordinate
 // triple products in float (getVolume, Cell3D_Kernel.cl:46-64), which
 // cancels catastrophically far from the origin.
-__device__ void cell_stats(const Lds3D& l, const int* __restrict__ faces, int nv, int nf,
-                           float4* st_out, double* vol_out, int ci) {
+__device__ void cell_stats(const Lds3D& l, int nv, int nf, float4* st_out, double* vol_out, int ci) {
   const int tid = threadIdx.x, NT = blockDim.x;
   double px = 0.0, py = 0.0, pz = 0.0;
   for (int v = tid; v < nv; v += NT) {
@@ -150,16 +194,16 @@ __device__ void cell_stats(const Lds3D& l, const int* __restrict__ faces, int nv
   const f3 C = {(float)(px / nv), (float)(py / nv), (float)(pz / nv)};
   float r = 0.0f;
   for (int v = tid; v < nv; v += NT) {
-    f3 d = sub(mk(l.sx[v], l.sy[v], l.sz[v]), C);
+    const f3 d = sub(lds_v(l.sx, l.sy, l.sz, v), C);
     r = fmaxf(r, sqrtf(dot(d, d)));
   }
   r = block_max(r, reinterpret_cast<float*>(l.red));
   double vol = 0.0;
   for (int f = tid; f < nf; f += NT) {
-    const int i0 = faces[3 * f], i1 = faces[3 * f + 1], i2 = faces[3 * f + 2];
-    f3 P0 = sub(mk(l.sx[i0], l.sy[i0], l.sz[i0]), C);
-    f3 P1 = sub(mk(l.sx[i1], l.sy[i1], l.sz[i1]), C);
-    f3 P2 = sub(mk(l.sx[i2], l.sy[i2], l.sz[i2]), C);
+    const int i0 = l.sf[3 * f], i1 = l.sf[3 * f + 1], i2 = l.sf[3 * f + 2];
+    const f3 P0 = sub(lds_v(l.sx, l.sy, l.sz, i0), C);
+    const f3 P1 = sub(lds_v(l.sx, l.sy, l.sz, i1), C);
+    const f3 P2 = sub(lds_v(l.sx, l.sy, l.sz, i2), C);
     vol += (double)dot(cross(P0, P1), P2);
   }
   vol = block_sum(vol, l.red);
@@ -171,19 +215,82 @@ __device__ void cell_stats(const Lds3D& l, const int* __restrict__ faces, int nv
   }
 }
 
+__device__ __forceinline__ void stage_topology(const Step3D& a, const Lds3D& l) {
+  const int tid = threadIdx.x, NT = blockDim.x;
+  for (int i = tid; i < 3 * a.nf; i += NT) l.sf[i] = (unsigned short)a.faces[i];
+  for (int i = tid; i < a.nv; i += NT) l.smem_members[i] = (unsigned short)a.clus_members[i];
+  if (tid <= a.nclus) l.cbeg[tid] = a.clus_begin[tid];
+}
+
+// Substrate force on one corner of a facing-down face
+// (Cell3D_Kernel.cl:229-245): a z < 0 penalty and the spreading push away
+// from the centre of mass within 2 l0 of the plane.
+__device__ __forceinline__ f3 substrate_corner(f3 pos, f3 C, float Ks, float ksg, float L, float l0) {
+  f3 c = {0.0f, 0.0f, 0.0f};
+  float ks = Ks;
+  if (ksg != 0.0f) {
+    const float xw = pos.x - L * floorf(pos.x / L);
+    ks = Ks * (1.0f + ksg * xw / L);
+  }
+  if (pos.z < 0.0f) c.z += ks * fabsf(pos.z);
+  const float height = fabsf(pos.z);
+  if (height < l0 * 2.0f) {
+    const f3 c2v = mk(pos.x - C.x, pos.y - C.y, 0.0f - C.z);
+    const float dz = pos.z - 0.0f;
+    const float dist = sqrtf((0.0f * 0.0f + 0.0f * 0.0f) + dz * dz);
+    const float ftmp = (float)((double)ks * (1.0 - (double)(dist / l0)));
+    c = add(c, scale(normalize(c2v), ftmp));
+  }
+  return c;
+}
+
 extern "C" __global__ void __launch_bounds__(1024) k3d_stats(const Step3D a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nv = a.nv, NT = blockDim.x, tid = threadIdx.x;
   const int ci = xcd_remap(blockIdx.x, gridDim.x);
-  Lds3D l = carve3d(smem, nv, NT);
+  Lds3D l = carve3d(smem, nv, a.nf, NT, a.ww);
   const size_t base = (size_t)ci * nv;
+  stage_topology(a, l);
   for (int v = tid; v < nv; v += NT) {
     l.sx[v] = a.xin[base + v];
     l.sy[v] = a.yin[base + v];
     l.sz[v] = a.zin[base + v];
   }
   __syncthreads();
-  cell_stats(l, a.faces, nv, a.nf, a.st_out, a.vol_out, ci);
+  cell_stats(l, nv, a.nf, a.st_out, a.vol_out, ci);
+}
+
+// Winding number of point p about the staged cell (l.nx/ny/nz shifted by sh),
+// evaluated by one wave: lanes normalise the directions to the cell's
+// vertices once into this wave's LDS table (the same bits a per-face
+// normalisation gives, so every face's `denom < 1e-8` decision is the
+// reference order's), then split the faces.  The sum is reduced in double.
+__device__ __forceinline__ float winding_wave(const Lds3D& l, float* ux, float* uy, float* uz,
+                                              int nv, int nf, f3 sh, f3 p) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1);
+  for (int v = lane; v < nv; v += DPM_WAVE) {
+    // Verts + shift - point, in the reference's order (Cell3D_Kernel.cl:285-287)
+    const f3 A = mk((l.nx[v] + sh.x) - p.x, (l.ny[v] + sh.y) - p.y, (l.nz[v] + sh.z) - p.z);
+    const f3 u = normalize(A);
+    ux[v] = u.x;
+    uy[v] = u.y;
+    uz[v] = u.z;
+  }
+  wave_lds_sync();
+  float part = 0.0f;
+  for (int f = lane; f < nf; f += DPM_WAVE) {
+    const int ia = l.sf[3 * f], ib = l.sf[3 * f + 1], ic = l.sf[3 * f + 2];
+    const f3 u = mk(ux[ia], uy[ia], uz[ia]);
+    const f3 v = mk(ux[ib], uy[ib], uz[ib]);
+    const f3 w = mk(ux[ic], uy[ic], uz[ic]);
+    const float denom = ((1.0f + dot(u, v)) + dot(v, w)) + dot(w, u);
+    if (denom < 1e-8f) continue;
+    const float num = dot(u, cross(v, w));
+    part += 2.0f * atan2f(num, denom);
+  }
+  const double tot = wave_sum((double)part);
+  wave_lds_sync();  // this wave's table reads finish before it is rewritten
+  return (float)(tot / (double)(4.0f * 3.14159265358979323846f));
 }
 
 // One overdamped Euler step of every cell (or, without F_MOVE, one force
@@ -191,9 +298,9 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_stats(const Step3D a) {
 extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nv = a.nv, nf = a.nf, NT = blockDim.x, tid = threadIdx.x;
-  const int lane = tid & (DPM_WAVE - 1), wave = tid / DPM_WAVE, nw = NT / DPM_WAVE;
+  const int lane = tid & (DPM_WAVE - 1), wave = tid / DPM_WAVE;
   const int ci = xcd_remap(blockIdx.x, gridDim.x);
-  Lds3D l = carve3d(smem, nv, NT);
+  Lds3D l = carve3d(smem, nv, nf, NT, a.ww);
 
   const int t = a.cell_tissue[ci];
   const int cb = a.tissue_begin[t], ce = a.tissue_begin[t + 1];
@@ -207,7 +314,16 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
   const float v0i = a.v0[ci], a0i = a.a0[ci], l0i = a.l0[ci];
   const unsigned mask = a.mask;
   const size_t base = (size_t)ci * nv;
+  const bool own = tid < nv;
 
+  // per-vertex topology rows in registers
+  int vfr[DPM3D_MAXDEG], adjr[DPM3D_MAXDEG];
+#pragma unroll
+  for (int k = 0; k < DPM3D_MAXDEG; k++) {
+    vfr[k] = own ? a.vf[tid * DPM3D_MAXDEG + k] : -1;
+    adjr[k] = own ? a.adj[tid * DPM3D_MAXDEG + k] : -1;
+  }
+  stage_topology(a, l);
   for (int v = tid; v < nv; v += NT) {
     l.sx[v] = a.xin[base + v];
     l.sy[v] = a.yin[base + v];
@@ -215,71 +331,74 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
   }
   __syncthreads();
 
-  const bool own = tid < nv;
-  const f3 P = own ? mk(l.sx[tid], l.sy[tid], l.sz[tid]) : mk(0.0f, 0.0f, 0.0f);
+  const f3 P = own ? lds_v(l.sx, l.sy, l.sz, tid) : mk(0.0f, 0.0f, 0.0f);
   f3 F = {0.0f, 0.0f, 0.0f};
 
   // ---------------------------------------------------------- shape forces
-  if (own) {
+  // Per-face pass: each face's volume, edge-spring and substrate terms for
+  // its three corners are computed once into LDS; every vertex then gathers
+  // its incident (face, corner) entries in ascending face order.
+  if (mask & (M_VOLUME | M_SURFACE | M_SUBSTRATE)) {  // block-uniform
     // Volume strain of the current volume, in double (the residual V/v0 - 1
     // cancels when the cell is near its target).
     const float strain = (float)(a.vol_in[ci] / (double)v0i - 1.0);
-    const float prevol = -Kvi * strain;
+    const float kvol = -Kvi * strain / 6.0f;
+    const bool vol_on = (mask & M_VOLUME) && Kvi != 0.0f;
+    const bool surf_on = (mask & M_SURFACE) && !(Kai < 1e-8f);
+    const bool sub_on = (mask & M_SUBSTRATE) && !(Ksi < 1e-12f);
     const float sc = Kai * sqrtf(a0i) / l0i * 0.3f;
-    for (int k = 0; k < DPM3D_MAXDEG; k++) {
-      const int e = a.vf[tid * DPM3D_MAXDEG + k];
-      if (e < 0) break;
-      const int f = e / 3, corner = e - 3 * f;
-      const int i0 = a.faces[3 * f], i1 = a.faces[3 * f + 1], i2 = a.faces[3 * f + 2];
-      const f3 P0 = mk(l.sx[i0], l.sy[i0], l.sz[i0]);
-      const f3 P1 = mk(l.sx[i1], l.sy[i1], l.sz[i1]);
-      const f3 P2 = mk(l.sx[i2], l.sy[i2], l.sz[i2]);
-      if ((mask & M_VOLUME) && Kvi != 0.0f) {
+    const float inv_l0 = 1.0f / l0i;
+    for (int f = tid; f < nf; f += NT) {
+      const int i0 = l.sf[3 * f], i1 = l.sf[3 * f + 1], i2 = l.sf[3 * f + 2];
+      const f3 P0 = lds_v(l.sx, l.sy, l.sz, i0);
+      const f3 P1 = lds_v(l.sx, l.sy, l.sz, i1);
+      const f3 P2 = lds_v(l.sx, l.sy, l.sz, i2);
+      f3 c0 = {0.0f, 0.0f, 0.0f}, c1 = c0, c2 = c0;
+      if (vol_on) {  // Cell3D_Kernel.cl:95-111
         const f3 A = sub(P1, Ci), B = sub(P2, Ci), Cc = sub(P0, Ci);
-        const f3 g = corner == 0 ? cross(A, B) : (corner == 1 ? cross(B, Cc) : cross(Cc, A));
-        F.x += prevol * g.x / 6.0f;
-        F.y += prevol * g.y / 6.0f;
-        F.z += prevol * g.z / 6.0f;
+        c0 = scale(cross(A, B), kvol);
+        c1 = scale(cross(B, Cc), kvol);
+        c2 = scale(cross(Cc, A), kvol);
       }
-      if ((mask & M_SURFACE) && !(Kai < 1e-8f)) {
+      if (surf_on) {  // Cell3D_Kernel.cl:140-176
         const f3 lv0 = sub(P1, P0), lv1 = sub(P2, P1), lv2 = sub(P0, P2);
         const float len0 = sqrtf(dot(lv0, lv0)), len1 = sqrtf(dot(lv1, lv1)),
                     len2 = sqrtf(dot(lv2, lv2));
         if (!(len0 < 1e-12f || len1 < 1e-12f || len2 < 1e-12f)) {
-          const f3 u0 = mk(lv0.x / len0, lv0.y / len0, lv0.z / len0);
-          const f3 u1 = mk(lv1.x / len1, lv1.y / len1, lv1.z / len1);
-          const f3 u2 = mk(lv2.x / len2, lv2.y / len2, lv2.z / len2);
-          const float d0 = (len0 / l0i) - 1.0f, d1 = (len1 / l0i) - 1.0f, d2 = (len2 / l0i) - 1.0f;
-          f3 g;
-          if (corner == 0)
-            g = sub(scale(u0, d0), scale(u2, d2));
-          else if (corner == 1)
-            g = sub(scale(u1, d1), scale(u0, d0));
-          else
-            g = sub(scale(u2, d2), scale(u1, d1));
-          F = add(F, scale(g, sc));
+          const f3 e0 = scale(lv0, (len0 * inv_l0 - 1.0f) / len0);
+          const f3 e1 = scale(lv1, (len1 * inv_l0 - 1.0f) / len1);
+          const f3 e2 = scale(lv2, (len2 * inv_l0 - 1.0f) / len2);
+          c0 = add(c0, scale(sub(e0, e2), sc));
+          c1 = add(c1, scale(sub(e1, e0), sc));
+          c2 = add(c2, scale(sub(e2, e1), sc));
         }
       }
-      if ((mask & M_SUBSTRATE) && !(Ksi < 1e-12f)) {
+      if (sub_on) {  // Cell3D_Kernel.cl:206-246
         const f3 un = normalize(cross(sub(P1, P0), sub(P2, P0)));
         if (un.z < -0.1f) {
-          if (a.p_down && corner == 0) a.p_down[(size_t)ci * nf + f] = 1;
-          const f3 pos = P;  // this vertex is corner `corner` of face f
-          float ks = Ksi;
-          if (ksg != 0.0f) {
-            const float xw = pos.x - L * floorf(pos.x / L);
-            ks = Ksi * (1.0f + ksg * xw / L);
-          }
-          if (pos.z < 0.0f) F.z += ks * fabsf(pos.z);
-          const float height = fabsf(pos.z);
-          if (height < l0i * 2.0f) {
-            const f3 c2v = mk(pos.x - Ci.x, pos.y - Ci.y, 0.0f - Ci.z);
-            const float dz = pos.z - 0.0f;
-            const float dist = sqrtf((0.0f * 0.0f + 0.0f * 0.0f) + dz * dz);
-            const float ftmp = (float)((double)ks * (1.0 - (double)(dist / l0i)));
-            F = add(F, scale(normalize(c2v), ftmp));
-          }
+          if (a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
+          c0 = add(c0, substrate_corner(P0, Ci, Ksi, ksg, L, l0i));
+          c1 = add(c1, substrate_corner(P1, Ci, Ksi, ksg, L, l0i));
+          c2 = add(c2, substrate_corner(P2, Ci, Ksi, ksg, L, l0i));
         }
+      }
+      l.fcx[3 * f] = c0.x;
+      l.fcy[3 * f] = c0.y;
+      l.fcz[3 * f] = c0.z;
+      l.fcx[3 * f + 1] = c1.x;
+      l.fcy[3 * f + 1] = c1.y;
+      l.fcz[3 * f + 1] = c1.z;
+      l.fcx[3 * f + 2] = c2.x;
+      l.fcy[3 * f + 2] = c2.y;
+      l.fcz[3 * f + 2] = c2.z;
+    }
+    __syncthreads();
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < DPM3D_MAXDEG; k++) {
+        const int e = vfr[k];  // face*3 + corner
+        if (e < 0) break;
+        F = add(F, mk(l.fcx[e], l.fcy[e], l.fcz[e]));
       }
     }
   }
@@ -288,10 +407,11 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
   if ((mask & M_BENDING) && Kbi != 0.0f) {  // block-uniform
     if (own) {
       f3 s = {0.0f, 0.0f, 0.0f};
+#pragma unroll
       for (int k = 0; k < DPM3D_MAXDEG; k++) {
-        const int u = a.adj[tid * DPM3D_MAXDEG + k];
+        const int u = adjr[k];
         if (u < 0) break;
-        s = add(s, sub(mk(l.sx[u], l.sy[u], l.sz[u]), P));
+        s = add(s, sub(lds_v(l.sx, l.sy, l.sz, u), P));
       }
       l.sb[3 * tid] = s.x;
       l.sb[3 * tid + 1] = s.y;
@@ -301,8 +421,9 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
     if (own) {
       f3 acc = {0.0f, 0.0f, 0.0f};
       int deg = 0;
+#pragma unroll
       for (int k = 0; k < DPM3D_MAXDEG; k++) {
-        const int u = a.adj[tid * DPM3D_MAXDEG + k];
+        const int u = adjr[k];
         if (u < 0) break;
         deg++;
         acc = add(acc, mk(l.sb[3 * u], l.sb[3 * u + 1], l.sb[3 * u + 2]));
@@ -319,6 +440,9 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
   unsigned long long n_wind = 0, n_contact = 0, n_cand = 0;
   if (do_contact || do_adh) {  // block-uniform
     const float L0i = fmaxf(l0i, 1e-12f);
+    float* ux = l.utab + (size_t)wave * 3 * nv;
+    float* uy = ux + nv;
+    float* uz = uy + nv;
     for (int c0 = cb; c0 < ce; c0 += NT) {
       const int cj = c0 + tid;
       bool cand = false;
@@ -362,6 +486,14 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
           l.ny[v] = a.yin[bj + v];
           l.nz[v] = a.zin[bj + v];
         }
+        if (tid < DPM3D_MAXCLUS) l.crad[tid] = 0u;
+        __syncthreads();
+        if (do_adh && own) {
+          // radius of each vertex cluster about its seed vertex
+          const int k = a.clus_of[tid];
+          const f3 d = sub(lds_v(l.nx, l.ny, l.nz, tid), lds_v(l.nx, l.ny, l.nz, k));
+          atomicMax(&l.crad[k], __float_as_uint(sqrtf(dot(d, d))));
+        }
         __syncthreads();
         if (do_contact) {
           const f3 Cj = mk(stj.x + sh.x, stj.y + sh.y, stj.z + sh.z);
@@ -373,24 +505,12 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
           }
           const int np = block_compact(need, tid, l.plist, l.cnt);
           n_wind += np;
-          for (int pi = wave; pi < np; pi += nw) {
-            const int v = l.plist[pi];
-            const f3 p = mk(l.sx[v], l.sy[v], l.sz[v]);
-            float part = 0.0f;
-            for (int f = lane; f < nf; f += DPM_WAVE) {
-              const int ia = a.faces[3 * f], ib = a.faces[3 * f + 1], ic = a.faces[3 * f + 2];
-              // Verts + shift - point, in the reference's order.
-              const f3 A = mk((l.nx[ia] + sh.x) - p.x, (l.ny[ia] + sh.y) - p.y, (l.nz[ia] + sh.z) - p.z);
-              const f3 B = mk((l.nx[ib] + sh.x) - p.x, (l.ny[ib] + sh.y) - p.y, (l.nz[ib] + sh.z) - p.z);
-              const f3 C = mk((l.nx[ic] + sh.x) - p.x, (l.ny[ic] + sh.y) - p.y, (l.nz[ic] + sh.z) - p.z);
-              const f3 u = normalize(A), vv = normalize(B), w = normalize(C);
-              const float denom = ((1.0f + dot(u, vv)) + dot(vv, w)) + dot(w, u);
-              if (denom < 1e-8f) continue;
-              const float num = dot(u, cross(vv, w));
-              part += 2.0f * atan2f(num, denom);
+          if (wave < a.ww) {
+            for (int pi = wave; pi < np; pi += a.ww) {
+              const int v = l.plist[pi];
+              const float wv = winding_wave(l, ux, uy, uz, nv, nf, sh, lds_v(l.sx, l.sy, l.sz, v));
+              if (lane == 0) l.wb[v] = wv;
             }
-            const double tot = wave_sum((double)part);
-            if (lane == 0) l.wb[v] = (float)(tot / (double)(4.0f * 3.14159265358979323846f));
           }
           __syncthreads();
           if (need) {
@@ -408,36 +528,55 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
         }
         if (do_adh && own) {
           const float L0j = fmaxf(a.l0[cj2], 1e-12f);
+          const float l0m = fmaxf(L0i, L0j);
           const f3 Cj = mk(stj.x + sh.x, stj.y + sh.y, stj.z + sh.z);
           const f3 d = sub(P, Cj);
-          const float r2 = (stj.w + 2.0f * fmaxf(L0i, L0j)) * 1.0001f + 1e-6f;
-          const bool all_images = pbc && 2.0f * (Ri + stj.w + 2.0f * fmaxf(L0i, L0j)) >= 0.999f * L;
+          const float r2 = (stj.w + 2.0f * l0m) * 1.0001f + 1e-6f;
+          const bool all_images = pbc && 2.0f * (Ri + stj.w + 2.0f * l0m) >= 0.999f * L;
           if (all_images || dot(d, d) <= r2 * r2) {
             int cnt_own = 0;
-            for (int vj = 0; vj < nv; vj++) {
-              const f3 pj = mk(l.nx[vj], l.ny[vj], l.nz[vj]);
-              // this vertex's own loop: delta = pj - p
-              f3 dl = sub(pj, P);
-              if (pbc) {
-                const float rx = roundf(dl.x / L), ry = roundf(dl.y / L), rz = roundf(dl.z / L);
-                dl = sub(dl, mk(L * rx, L * ry, L * rz));
+            const float half_l = 0.49f * L;
+            const float lim_own = (L0i * 2.0f) * (L0i * 2.0f) * 1.00001f;
+            const float lim_rev = (L0j * 2.0f) * (L0j * 2.0f) * 1.00001f;
+            for (int k = 0; k < a.nclus; k++) {
+              if (!all_images) {
+                // the cluster's vertices lie within crad[k] of its seed
+                const f3 s = mk(l.nx[k] + sh.x, l.ny[k] + sh.y, l.nz[k] + sh.z);
+                const f3 ds = sub(P, s);
+                const float rk = (__uint_as_float(l.crad[k]) + 2.0f * l0m) * 1.0001f + 1e-6f;
+                if (dot(ds, ds) > rk * rk) continue;
               }
-              const float ds = sqrtf(dot(dl, dl));
-              if (ds < L0i * 2.0f && ds > 1e-12f) {
-                const float k = Kat * 0.5f * (ds / L0i - 1.0f);
-                F = sub(F, mk(k * (dl.x / ds), k * (dl.y / ds), k * (dl.z / ds)));
-                cnt_own++;
-              }
-              // the partner's loop applies +t to this vertex: delta = p - pj
-              f3 dr = sub(P, pj);
-              if (pbc) {
-                const float rx = roundf(dr.x / L), ry = roundf(dr.y / L), rz = roundf(dr.z / L);
-                dr = sub(dr, mk(L * rx, L * ry, L * rz));
-              }
-              const float dsr = sqrtf(dot(dr, dr));
-              if (dsr < L0j * 2.0f && dsr > 1e-12f) {
-                const float k = Kat * 0.5f * (dsr / L0j - 1.0f);
-                F = add(F, mk(k * (dr.x / dsr), k * (dr.y / dsr), k * (dr.z / dsr)));
+              for (int m = l.cbeg[k]; m < l.cbeg[k + 1]; m++) {
+                const int vj = l.smem_members[m];
+                const f3 pj = lds_v(l.nx, l.ny, l.nz, vj);
+                // this vertex's own loop: delta = pj - p, minimum image per
+                // component (Cell3D_Kernel.cl:341-346); a component below
+                // 0.49 L rounds to image 0, which leaves it unchanged.
+                f3 dl = sub(pj, P);
+                f3 dr = sub(P, pj);  // the partner's loop: delta = p - pj
+                if (pbc && !(fabsf(dl.x) < half_l && fabsf(dl.y) < half_l && fabsf(dl.z) < half_l)) {
+                  dl = sub(dl, mk(L * roundf(dl.x / L), L * roundf(dl.y / L), L * roundf(dl.z / L)));
+                  dr = sub(dr, mk(L * roundf(dr.x / L), L * roundf(dr.y / L), L * roundf(dr.z / L)));
+                }
+                // sqrt(d2) < c is only possible for d2 < c^2 (1 + 1e-5):
+                // the exact comparisons run where they can succeed.
+                const float d2 = dot(dl, dl);
+                if (d2 < lim_own) {
+                  const float ds = sqrtf(d2);
+                  if (ds < L0i * 2.0f && ds > 1e-12f) {
+                    const float kk = Kat * 0.5f * (ds / L0i - 1.0f);
+                    F = sub(F, mk(kk * (dl.x / ds), kk * (dl.y / ds), kk * (dl.z / ds)));
+                    cnt_own++;
+                  }
+                }
+                const float d2r = dot(dr, dr);
+                if (d2r < lim_rev) {
+                  const float dsr = sqrtf(d2r);
+                  if (dsr < L0j * 2.0f && dsr > 1e-12f) {
+                    const float kk = Kat * 0.5f * (dsr / L0j - 1.0f);
+                    F = add(F, mk(kk * (dr.x / dsr), kk * (dr.y / dsr), kk * (dr.z / dsr)));
+                  }
+                }
               }
             }
             if (a.p_adh) a.p_adh[base + tid] += cnt_own;
@@ -469,12 +608,10 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
       l.sz[tid] = Pn.z;
     }
     __syncthreads();
-    cell_stats(l, a.faces, nv, nf, a.st_out, a.vol_out, ci);
+    cell_stats(l, nv, nf, a.st_out, a.vol_out, ci);
   }
   if (a.counters) {
-    // one atomic per workgroup per counter (each thread's copy is identical
-    // for the block-level counts; per-vertex counts are summed first).
-    double c = block_sum((double)n_contact, l.red);
+    const double c = block_sum((double)n_contact, l.red);
     if (tid == 0) {
       if (blockIdx.x == 0) atomicAdd(&a.counters[0], 1ull);
       atomicAdd(&a.counters[1], n_wind);

@@ -84,6 +84,8 @@ struct dpm_tissue {
   float *tL = nullptr, *tKre = nullptr, *tKat = nullptr, *tksg = nullptr;
   int* tpbc = nullptr;
   int *faces = nullptr, *vf = nullptr, *adj = nullptr;
+  int *clus_of = nullptr, *clus_begin = nullptr, *clus_members = nullptr;
+  int nclus = 0, ww = 1;
   unsigned mask = 0;
   int cur = 0;
   bool stats_valid = false;
@@ -110,9 +112,9 @@ struct dpm_tissue {
     return DPM_OK;
   }
   ~dpm_tissue() {
-    if (graph) hipGraphExecDestroy(graph);
+    if (graph) (void)hipGraphExecDestroy(graph);
     for (void* p : allocs) hipFree(p);
-    if (stream) hipStreamDestroy(stream);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
@@ -143,7 +145,7 @@ static int init_common(dpm_tissue* h) {
 
 extern "C" void dpm_destroy(dpm_tissue* h) {
   if (!h) return;
-  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   delete h;
 }
 
@@ -236,6 +238,38 @@ extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) {
       nb[v].push_back(a);
       nb[a].push_back(v);
     }
+  // Vertex clusters for adhesion culling: graph-distance Voronoi regions of
+  // seed vertices 0..K-1 (in an icosphere, the 42 vertices of the once
+  // subdivided icosahedron come first), multi-source breadth-first, first
+  // visit wins (deterministic).
+  const int K = nv >= 162 ? 42 : std::min(12, nv);
+  std::vector<int> clus_of(nv, -1), bfs;
+  for (int k = 0; k < K; k++) {
+    clus_of[k] = k;
+    bfs.push_back(k);
+  }
+  std::vector<std::vector<int>> nbs(nv);
+  for (int f = 0; f < nf; f++)
+    for (int k = 0; k < 3; k++) {
+      const int a = d->faces[3 * f + k], b = d->faces[3 * f + (k + 1) % 3];
+      nbs[a].push_back(b);
+      nbs[b].push_back(a);
+    }
+  for (auto& v : nbs) std::sort(v.begin(), v.end());
+  for (size_t q = 0; q < bfs.size(); q++)
+    for (int u : nbs[bfs[q]])
+      if (clus_of[u] < 0) {
+        clus_of[u] = clus_of[bfs[q]];
+        bfs.push_back(u);
+      }
+  for (int v = 0; v < nv; v++)
+    if (clus_of[v] < 0) clus_of[v] = 0;  // disconnected vertex: any cluster is conservative
+  std::vector<int> clus_begin(K + 1, 0), clus_members;
+  for (int k = 0; k < K; k++) {
+    for (int v = 0; v < nv; v++)
+      if (clus_of[v] == k) clus_members.push_back(v);
+    clus_begin[k + 1] = (int)clus_members.size();
+  }
   std::vector<int> vf((size_t)nv * DPM3D_MAXDEG, -1), adj((size_t)nv * DPM3D_MAXDEG, -1);
   for (int v = 0; v < nv; v++) {
     std::sort(nb[v].begin(), nb[v].end());
@@ -256,7 +290,10 @@ extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) {
   h->nf = nf;
   h->ncmax = ncmax;
   h->NT = std::max(256, (nv + 63) / 64 * 64);
-  h->lds = lds3d_bytes(nv, h->NT);
+  // Waves that evaluate winding numbers, each with an nv-entry direction
+  // table in LDS (at most ~48 KB of tables).
+  h->ww = std::max(1, std::min(h->NT / 64, 49152 / (12 * nv)));
+  h->lds = lds3d_layout(nv, nf, h->NT, h->ww).total;
   if ((int)h->lds > max_lds_per_block(h->device)) {
     delete h;
     return fail(DPM_ERANGE, "cell mesh does not fit the LDS of one workgroup");
@@ -291,6 +328,10 @@ extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) {
   rc |= A(&h->faces, (size_t)nf * 3);
   rc |= A(&h->vf, vf.size());
   rc |= A(&h->adj, adj.size());
+  rc |= A(&h->clus_of, (size_t)nv);
+  rc |= A(&h->clus_begin, (size_t)K + 1);
+  rc |= A(&h->clus_members, (size_t)nv);
+  h->nclus = K;
   if (rc) { delete h; return DPM_ENOMEM; }
   h->tissue_begin_h.assign(d->cell_begin, d->cell_begin + T + 1);
   h->faces_h.assign(d->faces, d->faces + 3 * nf);
@@ -304,6 +345,9 @@ extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) {
   rc |= upload(h->faces, d->faces, (size_t)nf * 3, s);
   rc |= upload(h->vf, vf.data(), vf.size(), s);
   rc |= upload(h->adj, adj.data(), adj.size(), s);
+  rc |= upload(h->clus_of, clus_of.data(), (size_t)nv, s);
+  rc |= upload(h->clus_begin, clus_begin.data(), (size_t)K + 1, s);
+  rc |= upload(h->clus_members, clus_members.data(), (size_t)nv, s);
   HIPCHECK(hipMemsetAsync(h->fx, 0, NVT * sizeof(float), s));
   HIPCHECK(hipMemsetAsync(h->fy, 0, NVT * sizeof(float), s));
   HIPCHECK(hipMemsetAsync(h->fz, 0, NVT * sizeof(float), s));
@@ -316,8 +360,8 @@ extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) {
   }
   if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(DPM_EDEVICE, "stream sync failed");
   if (rc) { delete h; return rc; }
-  hipFuncSetAttribute((const void*)k3d_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
-  hipFuncSetAttribute((const void*)k3d_stats, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
+  (void)hipFuncSetAttribute((const void*)k3d_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
+  (void)hipFuncSetAttribute((const void*)k3d_stats, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
   *out = h;
   return DPM_OK;
 }
@@ -370,7 +414,7 @@ extern "C" int dpm3d_set_tissue_params(dpm_tissue* h, const float* L, const int*
   h->mask = mask & DPM3D_ALL;
   HIPCHECK(hipStreamSynchronize(s));
   if (h->graph) {
-    hipGraphExecDestroy(h->graph);
+    (void)hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
   }
   return DPM_OK;
@@ -456,6 +500,11 @@ static Step3D step_args(dpm_tissue* h, int in, float dt, int flags) {
   a.faces = h->faces;
   a.vf = h->vf;
   a.adj = h->adj;
+  a.clus_of = h->clus_of;
+  a.clus_begin = h->clus_begin;
+  a.clus_members = h->clus_members;
+  a.nclus = h->nclus;
+  a.ww = h->ww;
   a.nv = h->nv;
   a.nf = h->nf;
   a.ncells = h->C;
@@ -488,7 +537,7 @@ static int launch_step3d(dpm_tissue* h, int in, float dt, int flags) {
 // A graph of kGraphSteps steps starting from buffer 0 (ends on buffer 0).
 static int build_graph3d(dpm_tissue* h, float dt) {
   if (h->graph) {
-    hipGraphExecDestroy(h->graph);
+    (void)hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
   }
   hipGraph_t g = nullptr;
@@ -497,12 +546,12 @@ static int build_graph3d(dpm_tissue* h, float dt) {
   for (int s = 0; s < kGraphSteps && !rc; s++) rc = launch_step3d(h, s & 1, dt, F_MOVE);
   hipError_t e = hipStreamEndCapture(h->stream, &g);
   if (rc) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   if (e != hipSuccess) return fail(DPM_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
     h->graph = nullptr;
     return fail(DPM_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
@@ -576,10 +625,10 @@ extern "C" int dpm3d_probe(dpm_tissue* h, float* forces, unsigned char* contact,
   if (down) rc |= dev_alloc(&dd, (size_t)h->C * h->nf);
   if (adhesion) rc |= dev_alloc(&da, NVT);
   if (!rc) {
-    if (dc) hipMemsetAsync(dc, 0, npair, s);
-    if (dw) hipMemsetAsync(dw, 0, npair * sizeof(float), s);
-    if (dd) hipMemsetAsync(dd, 0, (size_t)h->C * h->nf, s);
-    if (da) hipMemsetAsync(da, 0, NVT * sizeof(int), s);
+    if (dc) (void)hipMemsetAsync(dc, 0, npair, s);
+    if (dw) (void)hipMemsetAsync(dw, 0, npair * sizeof(float), s);
+    if (dd) (void)hipMemsetAsync(dd, 0, (size_t)h->C * h->nf, s);
+    if (da) (void)hipMemsetAsync(da, 0, NVT * sizeof(int), s);
     Step3D a = step_args(h, h->cur, 0.0f, F_WRITE_FORCES);
     a.counters = nullptr;
     a.p_contact = dc;
@@ -598,10 +647,10 @@ extern "C" int dpm3d_probe(dpm_tissue* h, float* forces, unsigned char* contact,
   if (!rc && da)
     rc = hipMemcpyAsync(adhesion, da, NVT * sizeof(int), hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
   if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(DPM_EDEVICE, "stream sync failed");
-  if (dc) hipFree(dc);
-  if (dw) hipFree(dw);
-  if (dd) hipFree(dd);
-  if (da) hipFree(da);
+  if (dc) (void)hipFree(dc);
+  if (dw) (void)hipFree(dw);
+  if (dd) (void)hipFree(dd);
+  if (da) (void)hipFree(da);
   if (rc) return rc;
   h->forces_valid = true;
   if (forces) TRY(dpm3d_get_forces(h, forces, NVT));
@@ -672,12 +721,12 @@ extern "C" int dpm2d_create(const dpm2d_desc* d, dpm_tissue** out) {
   hipStream_t s = h->stream;
   rc |= upload(h->tcb, d->cell_begin, T + 1, s);
   rc |= upload(h->cvb, cvb.data(), C + 1, s);
-  hipMemsetAsync(h->fx2, 0, h->V * sizeof(float), s);
-  hipMemsetAsync(h->fy2, 0, h->V * sizeof(float), s);
+  (void)hipMemsetAsync(h->fx2, 0, h->V * sizeof(float), s);
+  (void)hipMemsetAsync(h->fy2, 0, h->V * sizeof(float), s);
   if (!rc) rc = dpm2d_set_cell_params(h, d->Ka, d->Kl, d->Kb, d->a0, d->l0, d->r0);
   if (!rc) rc = dpm2d_set_tissue_params(h, d->L, d->pbc, d->Kre, d->Kat, d->mask);
   if (rc) { delete h; return rc; }
-  hipFuncSetAttribute((const void*)k2d_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds2);
+  (void)hipFuncSetAttribute((const void*)k2d_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds2);
   *out = h;
   return DPM_OK;
 }
@@ -790,8 +839,8 @@ extern "C" int dpm2d_probe(dpm_tissue* h, float* forces, int* overlap, int* attr
   int rc = dev_alloc(&dov, h->V);
   rc |= dev_alloc(&dat, h->V);
   if (!rc) {
-    hipMemsetAsync(dov, 0, h->V * sizeof(int), s);
-    hipMemsetAsync(dat, 0, h->V * sizeof(int), s);
+    (void)hipMemsetAsync(dov, 0, h->V * sizeof(int), s);
+    (void)hipMemsetAsync(dat, 0, h->V * sizeof(int), s);
     Run2D a = run2d_args(h, 0, 0.0f);
     a.counters = nullptr;
     a.p_overlap = dov;
@@ -805,8 +854,8 @@ extern "C" int dpm2d_probe(dpm_tissue* h, float* forces, int* overlap, int* attr
   if (!rc && attraction)
     rc = hipMemcpyAsync(attraction, dat, h->V * sizeof(int), hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
   if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(DPM_EDEVICE, "stream sync failed");
-  if (dov) hipFree(dov);
-  if (dat) hipFree(dat);
+  if (dov) (void)hipFree(dov);
+  if (dat) (void)hipFree(dat);
   if (rc) return rc;
   if (forces) TRY(dpm2d_get_forces(h, forces, h->V));
   return DPM_OK;
