@@ -101,6 +101,19 @@ def cpu_baseline(m, steps):
                       f"OpenMP over cells: {sec:.2f} s"}
 
 
+def measured_traffic_per_vertex_step():
+    """HBM bytes per vertex-step of k3d_step from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "k3d_step_monolayer_256x162.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_vertex_step:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +161,8 @@ def main():
         f"launch {b_launch:.3f} ms)")
     achieved = B_ALG * nvb / (b_launch / 1e3) / 1e9
     windings = float(b_cnt[1]) / max(1, int(b_cnt[0]))
+    tpv = measured_traffic_per_vertex_step()
+    traffic = None if tpv is None else tpv * nvb
 
     out = {
         "metric": METRIC,
@@ -174,7 +189,7 @@ def main():
         "hbm_gbps_algorithmic": B_ALG * rate / 1e9,
         "winding_evals_per_step": windings,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": "k3d_step", "launch_ms": b_launch, "bytes_per_vertex_step": B_ALG},
     }
     if not args.no_cpu:

@@ -282,21 +282,27 @@ __device__ __noinline__ void force_vertex(const Run2D& a, const Tissue2DCtx& T, 
       if (cj == ci) continue;
       for (int j = l.vb[cj]; j < l.vb[cj + 1]; j++) {
         float rx = l.x[j] - px, ry = l.y[j] - py;
+        float nx = 0.0f, ny = 0.0f;
         // minimum image; a component below 0.49 L rounds to image 0, which
         // leaves it unchanged, so the division is skipped there
         if (T.pbc && !(fabsf(rx) < half_l && fabsf(ry) < half_l)) {
-          rx -= L * roundf(rx / L);
-          ry -= L * roundf(ry / L);
+          nx = roundf(rx / L);
+          ny = roundf(ry / L);
+          rx -= L * nx;
+          ry -= L * ny;
         }
         const float d2 = rx * rx + ry * ry;
         if (!(d2 < lim)) continue;  // sqrt(d2) < l0 needs d2 < l0^2 (1 + 1e-5)
         const float dist = sqrtf(d2);
         if (dist < l0) {
-          const float ftmp = kat * dist / l0;
-          float ux, uy;
-          normalize2(rx, ry, ux, uy);
-          fx += ftmp * ux;
-          fy += ftmp * uy;
+          // the reference's float decision; the force (kat d/l0 along r)
+          // in double on the same image, as a short image vector keeps few
+          // digits in float
+          const double ddx = ((double)l.x[j] - (double)px) - (double)L * (double)nx;
+          const double ddy = ((double)l.y[j] - (double)py) - (double)L * (double)ny;
+          const double k = (double)kat / (double)l0;  // ftmp/d = kat/l0
+          fx += (float)(k * ddx);
+          fy += (float)(k * ddy);
           cnt++;
         }
       }
@@ -405,8 +411,9 @@ extern "C" __global__ void __launch_bounds__(1024) k2d_run(const Run2D a) {
   if (a.counters) {
     const double c = block_sum((double)n_over_total, l.red);
     if (tid == 0) {
-      if (blockIdx.x == 0) atomicAdd(&a.counters[0], (unsigned long long)nevals);
-      atomicAdd(&a.counters[2], (unsigned long long)c);
+      unsigned long long* slot = a.counters + 4 * (blockIdx.x % DPM_COUNTER_SLOTS);
+      if (blockIdx.x == 0) atomicAdd(&slot[0], (unsigned long long)nevals);
+      if (c != 0.0) atomicAdd(&slot[2], (unsigned long long)c);
     }
   }
 }

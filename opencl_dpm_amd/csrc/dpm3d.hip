@@ -54,8 +54,10 @@ struct Step3D {
   float* fz;
   const float4* st_in;  // per cell: COM xyz, bounding radius
   const double* vol_in;
+  const float* em_in;   // per cell: longest edge
   float4* st_out;
   double* vol_out;
+  float* em_out;
   const float* Kv;
   const float* Ka;
   const float* Ks;
@@ -92,13 +94,17 @@ struct Step3D {
 };
 
 // Culling reach added to a cell's bounding radius for the winding-number
-// test.  Outside radius R_j + m_j of the (shifted) centre of mass every face
-// of cell j is at least m_j away, so with m_j >= 2 l0 no face can subtend the
-// more-than-hemisphere solid angle the reference skips (denom < 1e-8), the
-// exact winding number is 0 and its float evaluation stays orders of
-// magnitude below the 1e-6 pair threshold.
-__device__ __forceinline__ float contact_margin(float Rj, float l0j) {
-  return 2.0f * l0j + 0.1f * Rj;
+// test.  Every face of cell j lies inside its bounding sphere R_j, so a
+// point outside R_j + m_j is at least m_j from every face.  A triangle whose
+// edges are all <= e fits in a disc of radius e/sqrt(3); seen from distance
+// >= e it subtends at most 2*pi*(1 - sqrt(3/4)) ~ 0.84 sr, far from the
+// more-than-hemisphere (> pi) solid angle the reference skips
+// (denom < 1e-8).  With m_j = e_max(j) no face is skipped, the exact winding
+// number of such a point is 0, and its float evaluation (|Omega_f| small,
+// about 1e-7 relative rounding each) stays orders of magnitude below the
+// 1e-6 pair threshold — the pair is absent in the reference arithmetic too.
+__device__ __forceinline__ float contact_margin(float Rj, float emax_j) {
+  return emax_j * 1.001f + 1e-6f * Rj;
 }
 
 // LDS layout (byte offsets), shared by host (sizing) and device (carving).
@@ -112,14 +118,15 @@ __host__ __device__ inline Lds3DLayout lds3d_layout(int nv, int nf, int nt, int 
   L.nbr = o;     o += 12u * (size_t)nv;
   L.bend = o;    o += 12u * (size_t)nv;
   L.wnum = o;    o += 4u * (size_t)nv;
-  L.faces = o;   o += 6u * (size_t)nf;
+  L.faces = o;   o += 8u * (size_t)nf;                 // (i0 | i1 << 16, i2) per face
   L.members = o; o += 2u * (size_t)nv;
   o = (o + 3u) & ~(size_t)3u;
   L.clist = o;   o += 4u * (size_t)nt;
   L.plist = o;   o += 4u * (size_t)nt;
   L.cnt = o;     o += 4u * 32u;
-  L.crad = o;    o += 4u * (2u * DPM3D_MAXCLUS + 1u);  // radii (uint bits) + member offsets
-  L.utab = o;    o += 12u * (size_t)nv * (size_t)ww;
+  L.crad = o;    o += 4u * (3u * DPM3D_MAXCLUS + 1u);  // radii (uint bits), member offsets, active list
+  o = (o + 15u) & ~(size_t)15u;
+  L.utab = o;    o += 16u * (size_t)nv * (size_t)ww;
   L.fc = o;      o += 36u * (size_t)nf;                // per (face, corner) shape forces
   o = (o + 7u) & ~(size_t)7u;
   L.red = o;     o += 8u * 32u;
@@ -128,9 +135,11 @@ __host__ __device__ inline Lds3DLayout lds3d_layout(int nv, int nf, int nt, int 
 }
 
 struct Lds3D {
-  float *sx, *sy, *sz, *nx, *ny, *nz, *sb, *wb, *utab, *fcx, *fcy, *fcz;
-  unsigned short *sf, *smem_members;
-  int *clist, *plist, *cnt, *cbeg;
+  float *sx, *sy, *sz, *nx, *ny, *nz, *sb, *wb, *fcx, *fcy, *fcz;
+  float4* utab;
+  uint2* sf;
+  unsigned short* smem_members;
+  int *clist, *plist, *cnt, *cbeg, *cact;
   unsigned* crad;
   double* red;
 };
@@ -147,14 +156,15 @@ __device__ __forceinline__ Lds3D carve3d(unsigned char* smem, int nv, int nf, in
   l.nz = n + 2 * nv;
   l.sb = reinterpret_cast<float*>(smem + L.bend);
   l.wb = reinterpret_cast<float*>(smem + L.wnum);
-  l.sf = reinterpret_cast<unsigned short*>(smem + L.faces);
+  l.sf = reinterpret_cast<uint2*>(smem + L.faces);
   l.smem_members = reinterpret_cast<unsigned short*>(smem + L.members);
   l.clist = reinterpret_cast<int*>(smem + L.clist);
   l.plist = reinterpret_cast<int*>(smem + L.plist);
   l.cnt = reinterpret_cast<int*>(smem + L.cnt);
   l.crad = reinterpret_cast<unsigned*>(smem + L.crad);
   l.cbeg = reinterpret_cast<int*>(smem + L.crad) + DPM3D_MAXCLUS;
-  l.utab = reinterpret_cast<float*>(smem + L.utab);
+  l.cact = l.cbeg + DPM3D_MAXCLUS + 1;
+  l.utab = reinterpret_cast<float4*>(smem + L.utab);
   l.fcx = reinterpret_cast<float*>(smem + L.fc);
   l.fcy = l.fcx + 3 * nf;
   l.fcz = l.fcy + 3 * nf;
@@ -174,13 +184,21 @@ __device__ __forceinline__ f3 lds_v(const float* x, const float* y, const float*
   return mk(x[i], y[i], z[i]);
 }
 
+__device__ __forceinline__ void face_of(const Lds3D& l, int f, int& i0, int& i1, int& i2) {
+  const uint2 w = l.sf[f];
+  i0 = (int)(w.x & 0xffffu);
+  i1 = (int)(w.x >> 16);
+  i2 = (int)w.y;
+}
+
 // Centre of mass (double sums), bounding radius and volume of the cell whose
 // vertices are in l.sx/sy/sz.  The volume is summed over faces relative to
 // the centre of mass (translation-invariant for a closed surface), per-face
 // terms in float, the sum in double — the reference sums absolute-coordinate
 // triple products in float (getVolume, Cell3D_Kernel.cl:46-64), which
 // cancels catastrophically far from the origin.
-__device__ void cell_stats(const Lds3D& l, int nv, int nf, float4* st_out, double* vol_out, int ci) {
+__device__ void cell_stats(const Lds3D& l, int nv, int nf, float4* st_out, double* vol_out,
+                           float* em_out, int ci) {
   const int tid = threadIdx.x, NT = blockDim.x;
   double px = 0.0, py = 0.0, pz = 0.0;
   for (int v = tid; v < nv; v += NT) {
@@ -199,27 +217,47 @@ __device__ void cell_stats(const Lds3D& l, int nv, int nf, float4* st_out, doubl
   }
   r = block_max(r, reinterpret_cast<float*>(l.red));
   double vol = 0.0;
+  float e2 = 0.0f;
   for (int f = tid; f < nf; f += NT) {
-    const int i0 = l.sf[3 * f], i1 = l.sf[3 * f + 1], i2 = l.sf[3 * f + 2];
+    int i0, i1, i2;
+    face_of(l, f, i0, i1, i2);
     const f3 P0 = sub(lds_v(l.sx, l.sy, l.sz, i0), C);
     const f3 P1 = sub(lds_v(l.sx, l.sy, l.sz, i1), C);
     const f3 P2 = sub(lds_v(l.sx, l.sy, l.sz, i2), C);
     vol += (double)dot(cross(P0, P1), P2);
+    const f3 e0 = sub(P1, P0), e1 = sub(P2, P1), ee = sub(P0, P2);
+    e2 = fmaxf(e2, fmaxf(dot(e0, e0), fmaxf(dot(e1, e1), dot(ee, ee))));
   }
   vol = block_sum(vol, l.red);
+  e2 = block_max(e2, reinterpret_cast<float*>(l.red));
   if (tid == 0) {
     // Slack of a few ulp so that the culling tests are never tighter than the
-    // largest float distance a vertex can have from C.
+    // largest float distance a vertex can have from C, or the longest edge.
     st_out[ci] = make_float4(C.x, C.y, C.z, r * 1.0000005f + 1e-30f);
     vol_out[ci] = fabs(vol / 6.0);
+    em_out[ci] = sqrtf(e2) * 1.000001f;
   }
 }
 
 __device__ __forceinline__ void stage_topology(const Step3D& a, const Lds3D& l) {
   const int tid = threadIdx.x, NT = blockDim.x;
-  for (int i = tid; i < 3 * a.nf; i += NT) l.sf[i] = (unsigned short)a.faces[i];
+  for (int f = tid; f < a.nf; f += NT)
+    l.sf[f] = make_uint2((unsigned)a.faces[3 * f] | ((unsigned)a.faces[3 * f + 1] << 16),
+                         (unsigned)a.faces[3 * f + 2]);
   for (int i = tid; i < a.nv; i += NT) l.smem_members[i] = (unsigned short)a.clus_members[i];
   if (tid <= a.nclus) l.cbeg[tid] = a.clus_begin[tid];
+}
+
+// Kat/2 (d/L0 - 1) delta/d with delta = (b - a) - L n, the adhesion term of
+// Cell3D_Kernel.cl:354-357, evaluated in double.
+__device__ __forceinline__ f3 adhesion_term(f3 b, f3 a, f3 n, float L, float Kat, float L0) {
+  const double Ld = (double)L;
+  const double dx = ((double)b.x - (double)a.x) - Ld * (double)n.x;
+  const double dy = ((double)b.y - (double)a.y) - Ld * (double)n.y;
+  const double dz = ((double)b.z - (double)a.z) - Ld * (double)n.z;
+  const double d = sqrt(dx * dx + dy * dy + dz * dz);
+  const double k = (double)Kat * 0.5 * (d / (double)L0 - 1.0) / d;
+  return mk((float)(k * dx), (float)(k * dy), (float)(k * dz));
 }
 
 // Substrate force on one corner of a facing-down face
@@ -257,7 +295,7 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_stats(const Step3D a) {
     l.sz[v] = a.zin[base + v];
   }
   __syncthreads();
-  cell_stats(l, nv, a.nf, a.st_out, a.vol_out, ci);
+  cell_stats(l, nv, a.nf, a.st_out, a.vol_out, a.em_out, ci);
 }
 
 // Winding number of point p about the staged cell (l.nx/ny/nz shifted by sh),
@@ -265,24 +303,24 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_stats(const Step3D a) {
 // vertices once into this wave's LDS table (the same bits a per-face
 // normalisation gives, so every face's `denom < 1e-8` decision is the
 // reference order's), then split the faces.  The sum is reduced in double.
-__device__ __forceinline__ float winding_wave(const Lds3D& l, float* ux, float* uy, float* uz,
+__device__ __forceinline__ float winding_wave(const Lds3D& l, float4* ut,
                                               int nv, int nf, f3 sh, f3 p) {
   const int lane = threadIdx.x & (DPM_WAVE - 1);
   for (int v = lane; v < nv; v += DPM_WAVE) {
     // Verts + shift - point, in the reference's order (Cell3D_Kernel.cl:285-287)
     const f3 A = mk((l.nx[v] + sh.x) - p.x, (l.ny[v] + sh.y) - p.y, (l.nz[v] + sh.z) - p.z);
     const f3 u = normalize(A);
-    ux[v] = u.x;
-    uy[v] = u.y;
-    uz[v] = u.z;
+    ut[v] = make_float4(u.x, u.y, u.z, 0.0f);
   }
   wave_lds_sync();
   float part = 0.0f;
   for (int f = lane; f < nf; f += DPM_WAVE) {
-    const int ia = l.sf[3 * f], ib = l.sf[3 * f + 1], ic = l.sf[3 * f + 2];
-    const f3 u = mk(ux[ia], uy[ia], uz[ia]);
-    const f3 v = mk(ux[ib], uy[ib], uz[ib]);
-    const f3 w = mk(ux[ic], uy[ic], uz[ic]);
+    int ia, ib, ic;
+    face_of(l, f, ia, ib, ic);
+    const float4 ua = ut[ia], ub = ut[ib], uc = ut[ic];
+    const f3 u = mk(ua.x, ua.y, ua.z);
+    const f3 v = mk(ub.x, ub.y, ub.z);
+    const f3 w = mk(uc.x, uc.y, uc.z);
     const float denom = ((1.0f + dot(u, v)) + dot(v, w)) + dot(w, u);
     if (denom < 1e-8f) continue;
     const float num = dot(u, cross(v, w));
@@ -349,7 +387,8 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
     const float sc = Kai * sqrtf(a0i) / l0i * 0.3f;
     const float inv_l0 = 1.0f / l0i;
     for (int f = tid; f < nf; f += NT) {
-      const int i0 = l.sf[3 * f], i1 = l.sf[3 * f + 1], i2 = l.sf[3 * f + 2];
+      int i0, i1, i2;
+      face_of(l, f, i0, i1, i2);
       const f3 P0 = lds_v(l.sx, l.sy, l.sz, i0);
       const f3 P1 = lds_v(l.sx, l.sy, l.sz, i1);
       const f3 P2 = lds_v(l.sx, l.sy, l.sz, i2);
@@ -440,9 +479,7 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
   unsigned long long n_wind = 0, n_contact = 0, n_cand = 0;
   if (do_contact || do_adh) {  // block-uniform
     const float L0i = fmaxf(l0i, 1e-12f);
-    float* ux = l.utab + (size_t)wave * 3 * nv;
-    float* uy = ux + nv;
-    float* uz = uy + nv;
+    float4* ut = l.utab + (size_t)wave * nv;
     for (int c0 = cb; c0 < ce; c0 += NT) {
       const int cj = c0 + tid;
       bool cand = false;
@@ -457,7 +494,7 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
         const f3 d = mk(stj.x + sh.x - Ci.x, stj.y + sh.y - Ci.y, stj.z + sh.z - Ci.z);
         const float dist = sqrtf(dot(d, d));
         float reach = 0.0f;
-        if (do_contact) reach = Ri + stj.w + contact_margin(stj.w, a.l0[cj]);
+        if (do_contact) reach = Ri + stj.w + contact_margin(stj.w, a.em_in[cj]);
         if (do_adh) {
           const float l0m = fmaxf(L0i, fmaxf(a.l0[cj], 1e-12f));
           const float r2 = Ri + stj.w + 2.0f * l0m;
@@ -497,7 +534,7 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
         __syncthreads();
         if (do_contact) {
           const f3 Cj = mk(stj.x + sh.x, stj.y + sh.y, stj.z + sh.z);
-          const float rj = (stj.w + contact_margin(stj.w, a.l0[cj2])) * 1.0001f + 1e-6f;
+          const float rj = (stj.w + contact_margin(stj.w, a.em_in[cj2])) * 1.0001f + 1e-6f;
           bool need = false;
           if (own) {
             const f3 d = sub(P, Cj);
@@ -508,7 +545,7 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
           if (wave < a.ww) {
             for (int pi = wave; pi < np; pi += a.ww) {
               const int v = l.plist[pi];
-              const float wv = winding_wave(l, ux, uy, uz, nv, nf, sh, lds_v(l.sx, l.sy, l.sz, v));
+              const float wv = winding_wave(l, ut, nv, nf, sh, lds_v(l.sx, l.sy, l.sz, v));
               if (lane == 0) l.wb[v] = wv;
             }
           }
@@ -526,19 +563,32 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
             }
           }
         }
-        if (do_adh && own) {
-          const float L0j = fmaxf(a.l0[cj2], 1e-12f);
-          const float l0m = fmaxf(L0i, L0j);
+        int nact = 0;
+        const float L0j = fmaxf(a.l0[cj2], 1e-12f);
+        const float l0m = fmaxf(L0i, L0j);
+        const bool all_images = pbc && 2.0f * (Ri + stj.w + 2.0f * l0m) >= 0.999f * L;
+        if (do_adh) {  // block-uniform
+          // clusters of cell j that can reach this cell at all
+          bool act = false;
+          if (tid < a.nclus) {
+            const f3 s = mk(l.nx[tid] + sh.x, l.ny[tid] + sh.y, l.nz[tid] + sh.z);
+            const f3 ds = sub(s, Ci);
+            const float rk = (Ri + __uint_as_float(l.crad[tid]) + 2.0f * l0m) * 1.0001f + 1e-6f;
+            act = all_images || dot(ds, ds) <= rk * rk;
+          }
+          nact = block_compact(act, tid, l.cact, l.cnt);
+        }
+        if (do_adh && own && nact > 0) {
           const f3 Cj = mk(stj.x + sh.x, stj.y + sh.y, stj.z + sh.z);
           const f3 d = sub(P, Cj);
           const float r2 = (stj.w + 2.0f * l0m) * 1.0001f + 1e-6f;
-          const bool all_images = pbc && 2.0f * (Ri + stj.w + 2.0f * l0m) >= 0.999f * L;
           if (all_images || dot(d, d) <= r2 * r2) {
             int cnt_own = 0;
             const float half_l = 0.49f * L;
             const float lim_own = (L0i * 2.0f) * (L0i * 2.0f) * 1.00001f;
             const float lim_rev = (L0j * 2.0f) * (L0j * 2.0f) * 1.00001f;
-            for (int k = 0; k < a.nclus; k++) {
+            for (int q2 = 0; q2 < nact; q2++) {
+              const int k = l.cact[q2];
               if (!all_images) {
                 // the cluster's vertices lie within crad[k] of its seed
                 const f3 s = mk(l.nx[k] + sh.x, l.ny[k] + sh.y, l.nz[k] + sh.z);
@@ -554,28 +604,31 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
                 // 0.49 L rounds to image 0, which leaves it unchanged.
                 f3 dl = sub(pj, P);
                 f3 dr = sub(P, pj);  // the partner's loop: delta = p - pj
+                f3 nl = {0.0f, 0.0f, 0.0f}, nr = nl;  // image indices
                 if (pbc && !(fabsf(dl.x) < half_l && fabsf(dl.y) < half_l && fabsf(dl.z) < half_l)) {
-                  dl = sub(dl, mk(L * roundf(dl.x / L), L * roundf(dl.y / L), L * roundf(dl.z / L)));
-                  dr = sub(dr, mk(L * roundf(dr.x / L), L * roundf(dr.y / L), L * roundf(dr.z / L)));
+                  nl = mk(roundf(dl.x / L), roundf(dl.y / L), roundf(dl.z / L));
+                  nr = mk(roundf(dr.x / L), roundf(dr.y / L), roundf(dr.z / L));
+                  dl = sub(dl, mk(L * nl.x, L * nl.y, L * nl.z));
+                  dr = sub(dr, mk(L * nr.x, L * nr.y, L * nr.z));
                 }
                 // sqrt(d2) < c is only possible for d2 < c^2 (1 + 1e-5):
-                // the exact comparisons run where they can succeed.
+                // the exact comparisons run where they can succeed.  The
+                // decisions are the reference's float ones; the force of an
+                // accepted pair is evaluated in double on the same image,
+                // since across the periodic boundary the float difference
+                // of two near-L numbers keeps few digits of a short vector.
                 const float d2 = dot(dl, dl);
                 if (d2 < lim_own) {
                   const float ds = sqrtf(d2);
                   if (ds < L0i * 2.0f && ds > 1e-12f) {
-                    const float kk = Kat * 0.5f * (ds / L0i - 1.0f);
-                    F = sub(F, mk(kk * (dl.x / ds), kk * (dl.y / ds), kk * (dl.z / ds)));
+                    F = sub(F, adhesion_term(pj, P, nl, L, Kat, L0i));
                     cnt_own++;
                   }
                 }
                 const float d2r = dot(dr, dr);
                 if (d2r < lim_rev) {
                   const float dsr = sqrtf(d2r);
-                  if (dsr < L0j * 2.0f && dsr > 1e-12f) {
-                    const float kk = Kat * 0.5f * (dsr / L0j - 1.0f);
-                    F = add(F, mk(kk * (dr.x / dsr), kk * (dr.y / dsr), kk * (dr.z / dsr)));
-                  }
+                  if (dsr < L0j * 2.0f && dsr > 1e-12f) F = add(F, adhesion_term(P, pj, nr, L, Kat, L0j));
                 }
               }
             }
@@ -608,15 +661,18 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step(const Step3D a) {
       l.sz[tid] = Pn.z;
     }
     __syncthreads();
-    cell_stats(l, nv, nf, a.st_out, a.vol_out, ci);
+    cell_stats(l, nv, nf, a.st_out, a.vol_out, a.em_out, ci);
   }
   if (a.counters) {
     const double c = block_sum((double)n_contact, l.red);
     if (tid == 0) {
-      if (blockIdx.x == 0) atomicAdd(&a.counters[0], 1ull);
-      atomicAdd(&a.counters[1], n_wind);
-      atomicAdd(&a.counters[2], (unsigned long long)c);
-      atomicAdd(&a.counters[3], n_cand);
+      // spread over DPM_COUNTER_SLOTS slots so that workgroups do not
+      // serialise on one address; the host sums the slots
+      unsigned long long* slot = a.counters + 4 * (blockIdx.x % DPM_COUNTER_SLOTS);
+      if (blockIdx.x == 0) atomicAdd(&slot[0], 1ull);
+      if (n_wind) atomicAdd(&slot[1], n_wind);
+      if (c != 0.0) atomicAdd(&slot[2], (unsigned long long)c);
+      if (n_cand) atomicAdd(&slot[3], n_cand);
     }
   }
 }

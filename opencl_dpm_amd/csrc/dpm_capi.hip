@@ -69,7 +69,7 @@ struct dpm_tissue {
   int dim = 0;
   int device = 0;
   hipStream_t stream = nullptr;
-  unsigned long long* counters = nullptr;  // device [4]
+  unsigned long long* counters = nullptr;  // device [4 * DPM_COUNTER_SLOTS]
   std::vector<void*> allocs;
 
   // ------------------------------------------------------------------ 3D
@@ -78,6 +78,7 @@ struct dpm_tissue {
   float *x[2] = {}, *y[2] = {}, *z[2] = {}, *fx = nullptr, *fy = nullptr, *fz = nullptr;
   float4* st[2] = {};
   double* vol[2] = {};
+  float* em[2] = {};  // per-cell longest edge
   float *Kv = nullptr, *Ka = nullptr, *Ks = nullptr, *Kb = nullptr, *v0 = nullptr, *a0 = nullptr,
         *l0 = nullptr;
   int *cell_tissue = nullptr, *tissue_begin = nullptr;
@@ -137,9 +138,9 @@ static int init_common(dpm_tissue* h) {
     return fail(DPM_EDEVICE, "no HIP device is visible");
   HIPCHECK(hipGetDevice(&h->device));
   HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-  TRY(h->alloc(&h->counters, 4));
+  TRY(h->alloc(&h->counters, 4 * DPM_COUNTER_SLOTS));
   TRY(h->alloc(&h->flag, 1));
-  HIPCHECK(hipMemsetAsync(h->counters, 0, 4 * sizeof(unsigned long long), h->stream));
+  HIPCHECK(hipMemsetAsync(h->counters, 0, 4 * DPM_COUNTER_SLOTS * sizeof(unsigned long long), h->stream));
   return DPM_OK;
 }
 
@@ -159,10 +160,16 @@ extern "C" int dpm_synchronize(dpm_tissue* h) {
 
 extern "C" int dpm_counters(dpm_tissue* h, unsigned long long* out4, int reset) {
   if (!h || !out4) return fail(DPM_EINVAL, "null argument");
-  HIPCHECK(hipMemcpyAsync(out4, h->counters, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                          h->stream));
+  std::vector<unsigned long long> slots(4 * DPM_COUNTER_SLOTS);
+  HIPCHECK(hipMemcpyAsync(slots.data(), h->counters, slots.size() * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, h->stream));
   HIPCHECK(hipStreamSynchronize(h->stream));
-  if (reset) HIPCHECK(hipMemsetAsync(h->counters, 0, 4 * sizeof(unsigned long long), h->stream));
+  for (int k = 0; k < 4; k++) {
+    out4[k] = 0;
+    for (int s = 0; s < DPM_COUNTER_SLOTS; s++) out4[k] += slots[4 * s + k];
+  }
+  if (reset)
+    HIPCHECK(hipMemsetAsync(h->counters, 0, slots.size() * sizeof(unsigned long long), h->stream));
   return DPM_OK;
 }
 
@@ -292,7 +299,7 @@ extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) {
   h->NT = std::max(256, (nv + 63) / 64 * 64);
   // Waves that evaluate winding numbers, each with an nv-entry direction
   // table in LDS (at most ~48 KB of tables).
-  h->ww = std::max(1, std::min(h->NT / 64, 49152 / (12 * nv)));
+  h->ww = std::max(1, std::min(h->NT / 64, 49152 / (16 * nv)));
   h->lds = lds3d_layout(nv, nf, h->NT, h->ww).total;
   if ((int)h->lds > max_lds_per_block(h->device)) {
     delete h;
@@ -307,6 +314,7 @@ extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) {
     rc |= A(&h->z[b], NVT);
     rc |= A(&h->st[b], (size_t)C);
     rc |= A(&h->vol[b], (size_t)C);
+    rc |= A(&h->em[b], (size_t)C);
   }
   rc |= A(&h->fx, NVT);
   rc |= A(&h->fy, NVT);
@@ -481,8 +489,10 @@ static Step3D step_args(dpm_tissue* h, int in, float dt, int flags) {
   a.fz = h->fz;
   a.st_in = h->st[in];
   a.vol_in = h->vol[in];
+  a.em_in = h->em[in];
   a.st_out = h->st[1 - in];
   a.vol_out = h->vol[1 - in];
+  a.em_out = h->em[1 - in];
   a.Kv = h->Kv;
   a.Ka = h->Ka;
   a.Ks = h->Ks;
@@ -521,6 +531,7 @@ static int ensure_stats(dpm_tissue* h) {
   Step3D a = step_args(h, h->cur, 0.0f, 0);
   a.st_out = h->st[h->cur];
   a.vol_out = h->vol[h->cur];
+  a.em_out = h->em[h->cur];
   hipLaunchKernelGGL(k3d_stats, dim3(h->C), dim3(h->NT), h->lds, h->stream, a);
   HIPCHECK(hipGetLastError());
   h->stats_valid = true;

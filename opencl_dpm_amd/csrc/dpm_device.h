@@ -12,6 +12,8 @@
 #include <stdint.h>
 
 #define DPM_WAVE 64
+// Diagnostic counters are kept in this many slots of 4 (summed on the host).
+#define DPM_COUNTER_SLOTS 256
 
 namespace dpm {
 

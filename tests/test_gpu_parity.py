@@ -75,6 +75,25 @@ def test_3d_forces_and_pairs(case, mask):
         assert g["contact"].sum() > 0
 
 
+@pytest.mark.parametrize("spacing,seed", [(1.7, 21), (1.6, 22)])
+def test_3d_pairs_after_deformation(spacing, seed):
+    # cells squeezed against each other for 300 steps: deformed meshes with
+    # stretched faces, then every pair set vs ref32 at those positions
+    m = W.monolayer3d(3, spacing, seed=seed)
+    t = tissue3d(m)
+    t.set_positions(m.X)
+    t.euler(300, 0.002)
+    X = t.positions()
+    g = t.probe()
+    r32 = ref3d(m, X, capi.DPM3D_ALL, 32, pairs=True)
+    assert_forces(g["forces"], ref3d(m, X, capi.DPM3D_ALL, 64)["forces"], f"3d deformed spacing={spacing}")
+    np.testing.assert_array_equal(g["contact"], r32["contact"])
+    np.testing.assert_array_equal(g["down"], r32["down"])
+    np.testing.assert_array_equal(g["adhesion"], r32["adhesion"])
+    print(f"deformed pairs: contact={int(g['contact'].sum())}", flush=True)
+    assert g["contact"].sum() > 0
+
+
 def test_3d_substrate_gradient_forces():
     m = W.monolayer_config("cfg5")
     t = tissue3d(m)
