@@ -102,8 +102,9 @@ int dpm3d_set_positions(dpm_tissue* h, const float* xyz, size_t n);
 int dpm3d_get_positions(dpm_tissue* h, float* xyz, size_t n);
 /* Forces of the last step taken (Tissue3D.cpp:425-434, 498-499). */
 int dpm3d_get_forces(dpm_tissue* h, float* xyz, size_t n);
-/* Per-cell parameters after creation (any pointer may be NULL = unchanged).
- * The reference re-reads them from the cells on every call. */
+/* Per-cell and per-tissue parameters after creation (any pointer may be
+ * NULL = unchanged; the mask is always replaced).  The reference re-reads
+ * them from the cells on every call. */
 int dpm3d_set_cell_params(dpm_tissue* h, const float* Kv, const float* Ka,
                           const float* Ks, const float* Kb, const float* v0,
                           const float* a0);
@@ -125,6 +126,27 @@ int dpm3d_cell_stats(dpm_tissue* h, double* volume, float* com);
  * vertex gv on its own side (Cell3D_Kernel.cl:353).  Any output may be NULL. */
 int dpm3d_probe(dpm_tissue* h, float* forces, unsigned char* contact,
                 float* winding, unsigned char* down, int* adhesion);
+
+/* Full double precision.  dpm3d_create_f64 makes a handle whose positions,
+ * forces and per-cell statistics are double on the device and whose step
+ * evaluates every formula in double (the float thresholds of the reference
+ * kernels promoted); the reference's kernels are float only
+ * (Cell3D_Kernel.cl), so this is the FP64 comparison run of a precision
+ * study, not a reference entry point.  All dpm3d_* and dpm_* calls accept
+ * either kind of handle; the _f64 accessors move doubles without rounding
+ * (on a float handle they convert exactly from float). */
+int dpm3d_create_f64(const dpm3d_desc* desc, dpm_tissue** out);
+int dpm3d_set_positions_f64(dpm_tissue* h, const double* xyz, size_t n);
+int dpm3d_get_positions_f64(dpm_tissue* h, double* xyz, size_t n);
+int dpm3d_get_forces_f64(dpm_tissue* h, double* xyz, size_t n);
+/* Elastic energy of each cell at the current positions, in double:
+ *   Kv v0/2 (V/v0 - 1)^2 + sum over face edges sc l0/2 (l/l0 - 1)^2
+ *   + Kb/2 sum_v |sum_{u~v} (x_u - x_v)|^2,   sc = 0.3 Ka sqrt(a0)/l0,
+ * restricted to the terms the mask enables; minus its gradient is the
+ * volume + surface + bending force of the step.  The reference has no
+ * energy (the substrate, contact and adhesion forces are not conservative);
+ * it is the drift diagnostic of the precision study.  energy[ncells]. */
+int dpm3d_energy(dpm_tissue* h, double* energy);
 
 /* ------------------------------------------------------------------ 2D ---
  * A batch of tissues of polygon cells with per-cell vertex counts

@@ -30,6 +30,8 @@ EXPORTS = [
     "dpm_last_error", "dpm_version", "dpm_device_count",
     "dpm3d_create", "dpm3d_set_positions", "dpm3d_get_positions", "dpm3d_get_forces",
     "dpm3d_set_cell_params", "dpm3d_set_tissue_params", "dpm3d_cell_stats", "dpm3d_probe",
+    "dpm3d_create_f64", "dpm3d_set_positions_f64", "dpm3d_get_positions_f64",
+    "dpm3d_get_forces_f64", "dpm3d_energy",
     "dpm2d_create", "dpm2d_set_positions", "dpm2d_get_positions", "dpm2d_get_forces",
     "dpm2d_set_cell_params", "dpm2d_set_tissue_params", "dpm2d_probe",
     "dpm_euler", "dpm_euler_async", "dpm_synchronize", "dpm_stream", "dpm_counters",
@@ -57,6 +59,7 @@ def load() -> ctypes.CDLL:
     lib.dpm_stream.restype = vp
     lib.dpm_stream.argtypes = [vp]
     for name in ["dpm3d_set_positions", "dpm3d_get_positions", "dpm3d_get_forces",
+                 "dpm3d_set_positions_f64", "dpm3d_get_positions_f64", "dpm3d_get_forces_f64",
                  "dpm2d_set_positions", "dpm2d_get_positions", "dpm2d_get_forces"]:
         getattr(lib, name).argtypes = [vp, vp, sz]
     lib.dpm_euler.argtypes = [vp, i, fp]
@@ -66,6 +69,8 @@ def load() -> ctypes.CDLL:
     lib.dpm_destroy.argtypes = [vp]
     lib.dpm_destroy.restype = None
     lib.dpm3d_create.argtypes = [vp, vp]
+    lib.dpm3d_create_f64.argtypes = [vp, vp]
+    lib.dpm3d_energy.argtypes = [vp, vp]
     lib.dpm2d_create.argtypes = [vp, vp]
     lib.dpm3d_set_cell_params.argtypes = [vp] + [vp] * 6
     lib.dpm3d_set_tissue_params.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_uint]
@@ -163,11 +168,18 @@ class _Handle:
 
 
 class Tissues3D(_Handle):
-    """A batch of 3D tissues sharing one mesh (include/dpm.h, dpm3d_*)."""
+    """A batch of 3D tissues sharing one mesh (include/dpm.h, dpm3d_*).
+
+    precision=32 is the production step (float positions and forces, double
+    per-cell sums); precision=64 the all-double step (dpm3d_create_f64)."""
 
     def __init__(self, faces, cell_begin: Sequence[int], Kv, Ka, Ks, v0, a0, L, Kre, Kat=0.0,
-                 pbc=1, Kb=None, ks_grad=None, mask: int = DPM3D_REFERENCE, nv: int = None):
+                 pbc=1, Kb=None, ks_grad=None, mask: int = DPM3D_REFERENCE, nv: int = None,
+                 precision: int = 32):
         super().__init__()
+        if precision not in (32, 64):
+            raise ValueError("precision must be 32 or 64")
+        self.precision = precision
         self._faces = _i32(faces).reshape(-1, 3)
         self.nf = self._faces.shape[0]
         self.nv = int(nv if nv is not None else self._faces.max() + 1)
@@ -187,24 +199,41 @@ class Tissues3D(_Handle):
         d.Kv, d.Ka, d.Ks, d.Kb, d.v0, d.a0 = [_p(a) for a in self._cell]
         d.L, d.pbc, d.Kre, d.Kat, d.ks_grad = [_p(a) for a in self._tis]
         d.mask = mask
-        _check(load().dpm3d_create(ctypes.byref(d), ctypes.byref(self._h)))
+        create = load().dpm3d_create_f64 if precision == 64 else load().dpm3d_create
+        _check(create(ctypes.byref(d), ctypes.byref(self._h)))
 
     @property
     def nverts(self) -> int:
         return self.C * self.nv
 
     def set_positions(self, xyz) -> None:
-        a = _f32(xyz).reshape(-1, 3)
-        _check(load().dpm3d_set_positions(self._h, _p(a), a.shape[0]))
+        """float32 input goes through dpm3d_set_positions, anything else as
+        float64 through dpm3d_set_positions_f64 (exact on a float64 handle)."""
+        if np.asarray(xyz).dtype == np.float32:
+            a = _f32(xyz).reshape(-1, 3)
+            _check(load().dpm3d_set_positions(self._h, _p(a), a.shape[0]))
+        else:
+            a = np.ascontiguousarray(np.asarray(xyz, np.float64)).reshape(-1, 3)
+            _check(load().dpm3d_set_positions_f64(self._h, _p(a), a.shape[0]))
 
-    def positions(self) -> np.ndarray:
-        out = np.empty((self.nverts, 3), np.float32)
-        _check(load().dpm3d_get_positions(self._h, _p(out), self.nverts))
+    def _get(self, f32, f64, dtype):
+        dtype = np.dtype(dtype or (np.float64 if self.precision == 64 else np.float32))
+        out = np.empty((self.nverts, 3), dtype)
+        fn = f64 if dtype == np.float64 else f32
+        _check(fn(self._h, _p(out), self.nverts))
         return out
 
-    def forces(self) -> np.ndarray:
-        out = np.empty((self.nverts, 3), np.float32)
-        _check(load().dpm3d_get_forces(self._h, _p(out), self.nverts))
+    def positions(self, dtype=None) -> np.ndarray:
+        """Positions in the handle's precision (or `dtype`: float32/float64)."""
+        return self._get(load().dpm3d_get_positions, load().dpm3d_get_positions_f64, dtype)
+
+    def forces(self, dtype=None) -> np.ndarray:
+        return self._get(load().dpm3d_get_forces, load().dpm3d_get_forces_f64, dtype)
+
+    def energy(self) -> np.ndarray:
+        """Per-cell elastic energy (volume + edge springs + bending, per mask)."""
+        out = np.empty(self.C, np.float64)
+        _check(load().dpm3d_energy(self._h, _p(out)))
         return out
 
     def set_tissue_params(self, L=None, pbc=None, Kre=None, Kat=None, ks_grad=None, mask=None):

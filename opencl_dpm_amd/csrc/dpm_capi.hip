@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "dpm2d.hip"
@@ -74,13 +75,15 @@ struct dpm_tissue {
 
   // ------------------------------------------------------------------ 3D
   int T = 0, C = 0, nv = 0, nf = 0, NT = 0, ncmax = 0;
+  int rs = 4;  // bytes per real of positions, forces and cell statistics: 4 or 8
   size_t lds = 0;
-  float *x[2] = {}, *y[2] = {}, *z[2] = {}, *fx = nullptr, *fy = nullptr, *fz = nullptr;
-  float4* st[2] = {};
+  // arrays of the real type (float or double, per rs)
+  void *x[2] = {}, *y[2] = {}, *z[2] = {}, *fx = nullptr, *fy = nullptr, *fz = nullptr;
+  void* st[2] = {};  // CellStat: centre of mass, bounding radius
   double* vol[2] = {};
-  float* em[2] = {};  // per-cell longest edge
-  float *Kv = nullptr, *Ka = nullptr, *Ks = nullptr, *Kb = nullptr, *v0 = nullptr, *a0 = nullptr,
-        *l0 = nullptr;
+  void* em[2] = {};  // per-cell longest edge
+  double* energy = nullptr;
+  float *Kv = nullptr, *Ka = nullptr, *Ks = nullptr, *Kb = nullptr, *v0 = nullptr, *a0 = nullptr;
   int *cell_tissue = nullptr, *tissue_begin = nullptr;
   float *tL = nullptr, *tKre = nullptr, *tKat = nullptr, *tksg = nullptr;
   int* tpbc = nullptr;
@@ -112,9 +115,15 @@ struct dpm_tissue {
     allocs.push_back(*p);
     return DPM_OK;
   }
+  int alloc_real(void** p, size_t n) {  // n reals of rs bytes
+    unsigned char* q = nullptr;
+    TRY(alloc(&q, n * (size_t)rs));
+    *p = q;
+    return DPM_OK;
+  }
   ~dpm_tissue() {
     if (graph) (void)hipGraphExecDestroy(graph);
-    for (void* p : allocs) hipFree(p);
+    for (void* p : allocs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -173,7 +182,8 @@ extern "C" int dpm_counters(dpm_tissue* h, unsigned long long* out4, int reset) 
   return DPM_OK;
 }
 
-__global__ void k_nonfinite(const float* a, const float* b, const float* c, size_t n, int* flag) {
+template <typename R>
+__global__ void k_nonfinite(const R* a, const R* b, const R* c, size_t n, int* flag) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     const bool bad = !isfinite(a[i]) || !isfinite(b[i]) || (c && !isfinite(c[i]));
@@ -182,10 +192,11 @@ __global__ void k_nonfinite(const float* a, const float* b, const float* c, size
 }
 
 // Positions of all vertices finite?  (Tissue3D.cpp:481-488)
-static int check_finite(dpm_tissue* h, const float* a, const float* b, const float* c, size_t n) {
+template <typename R>
+static int check_finite(dpm_tissue* h, const R* a, const R* b, const R* c, size_t n) {
   HIPCHECK(hipMemsetAsync(h->flag, 0, sizeof(int), h->stream));
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 2048);
-  k_nonfinite<<<blocks, 256, 0, h->stream>>>(a, b, c, n, h->flag);
+  k_nonfinite<R><<<blocks, 256, 0, h->stream>>>(a, b, c, n, h->flag);
   HIPCHECK(hipGetLastError());
   int f = 0;
   HIPCHECK(hipMemcpyAsync(&f, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -196,13 +207,14 @@ static int check_finite(dpm_tissue* h, const float* a, const float* b, const flo
 
 // ================================================================== 3D ====
 
-static bool all_finite(const float* p, size_t n) {
+template <typename R>
+static bool all_finite(const R* p, size_t n) {
   for (size_t i = 0; i < n; i++)
     if (!std::isfinite(p[i])) return false;
   return true;
 }
 
-extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) {
+static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
   if (!d || !out) return fail(DPM_EINVAL, "null argument");
   *out = nullptr;
   if (d->ntissues <= 0 || !d->cell_begin) return fail(DPM_EINVAL, "NCELLS must be positive");
@@ -289,6 +301,7 @@ extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) {
 
   dpm_tissue* h = new dpm_tissue();
   h->dim = 3;
+  h->rs = rs;
   int rc = init_common(h);
   if (rc) { delete h; return rc; }
   h->T = T;
@@ -299,8 +312,8 @@ extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) {
   h->NT = std::max(256, (nv + 63) / 64 * 64);
   // Waves that evaluate winding numbers, each with an nv-entry direction
   // table in LDS (at most ~48 KB of tables).
-  h->ww = std::max(1, std::min(h->NT / 64, 49152 / (16 * nv)));
-  h->lds = lds3d_layout(nv, nf, h->NT, h->ww).total;
+  h->ww = std::max(1, std::min(h->NT / 64, 49152 / (4 * rs * nv)));
+  h->lds = lds3d_layout(nv, nf, h->NT, h->ww, rs).total;
   if ((int)h->lds > max_lds_per_block(h->device)) {
     delete h;
     return fail(DPM_ERANGE, "cell mesh does not fit the LDS of one workgroup");
@@ -308,24 +321,25 @@ extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) {
   const size_t NVT = (size_t)C * nv;
   auto A = [&](auto** p, size_t n) { return h->alloc(p, n); };
   rc = 0;
+  auto AR = [&](void** p, size_t n) { return h->alloc_real(p, n); };
   for (int b = 0; b < 2 && !rc; b++) {
-    rc |= A(&h->x[b], NVT);
-    rc |= A(&h->y[b], NVT);
-    rc |= A(&h->z[b], NVT);
-    rc |= A(&h->st[b], (size_t)C);
+    rc |= AR(&h->x[b], NVT);
+    rc |= AR(&h->y[b], NVT);
+    rc |= AR(&h->z[b], NVT);
+    rc |= AR(&h->st[b], 4 * (size_t)C);
     rc |= A(&h->vol[b], (size_t)C);
-    rc |= A(&h->em[b], (size_t)C);
+    rc |= AR(&h->em[b], (size_t)C);
   }
-  rc |= A(&h->fx, NVT);
-  rc |= A(&h->fy, NVT);
-  rc |= A(&h->fz, NVT);
+  rc |= AR(&h->fx, NVT);
+  rc |= AR(&h->fy, NVT);
+  rc |= AR(&h->fz, NVT);
+  rc |= A(&h->energy, (size_t)C);
   rc |= A(&h->Kv, C);
   rc |= A(&h->Ka, C);
   rc |= A(&h->Ks, C);
   rc |= A(&h->Kb, C);
   rc |= A(&h->v0, C);
   rc |= A(&h->a0, C);
-  rc |= A(&h->l0, C);
   rc |= A(&h->cell_tissue, C);
   rc |= A(&h->tissue_begin, T + 1);
   rc |= A(&h->tL, T);
@@ -356,23 +370,31 @@ extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) {
   rc |= upload(h->clus_of, clus_of.data(), (size_t)nv, s);
   rc |= upload(h->clus_begin, clus_begin.data(), (size_t)K + 1, s);
   rc |= upload(h->clus_members, clus_members.data(), (size_t)nv, s);
-  HIPCHECK(hipMemsetAsync(h->fx, 0, NVT * sizeof(float), s));
-  HIPCHECK(hipMemsetAsync(h->fy, 0, NVT * sizeof(float), s));
-  HIPCHECK(hipMemsetAsync(h->fz, 0, NVT * sizeof(float), s));
+  HIPCHECK(hipMemsetAsync(h->fx, 0, NVT * rs, s));
+  HIPCHECK(hipMemsetAsync(h->fy, 0, NVT * rs, s));
+  HIPCHECK(hipMemsetAsync(h->fz, 0, NVT * rs, s));
   if (rc) { delete h; return rc; }
   rc = dpm3d_set_cell_params(h, d->Kv, d->Ka, d->Ks, d->Kb, d->v0, d->a0);
-  if (!rc) rc = dpm3d_set_tissue_params(h, d->L, d->pbc, d->Kre, d->Kat, d->ks_grad, d->mask);
+  std::vector<float> no_grad(T, 0.0f);
+  if (!rc)
+    rc = dpm3d_set_tissue_params(h, d->L, d->pbc, d->Kre, d->Kat, d->ks_grad ? d->ks_grad : no_grad.data(),
+                                 d->mask);
   if (!d->Kb && !rc) {
     std::vector<float> z(C, 0.0f);
     rc = upload(h->Kb, z.data(), C, s);
   }
   if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(DPM_EDEVICE, "stream sync failed");
   if (rc) { delete h; return rc; }
-  (void)hipFuncSetAttribute((const void*)k3d_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
-  (void)hipFuncSetAttribute((const void*)k3d_stats, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
+  const void* kern[2][3] = {{(const void*)k3d_step_f32, (const void*)k3d_stats_f32, (const void*)k3d_energy_f32},
+                            {(const void*)k3d_step_f64, (const void*)k3d_stats_f64, (const void*)k3d_energy_f64}};
+  for (const void* k : kern[rs == 8 ? 1 : 0])
+    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
   *out = h;
   return DPM_OK;
 }
+
+extern "C" int dpm3d_create(const dpm3d_desc* d, dpm_tissue** out) { return create3d(d, out, 4); }
+extern "C" int dpm3d_create_f64(const dpm3d_desc* d, dpm_tissue** out) { return create3d(d, out, 8); }
 
 extern "C" int dpm3d_set_cell_params(dpm_tissue* h, const float* Kv, const float* Ka,
                                      const float* Ks, const float* Kb, const float* v0,
@@ -391,14 +413,7 @@ extern "C" int dpm3d_set_cell_params(dpm_tissue* h, const float* Kv, const float
   if (Ks) TRY(upload(h->Ks, Ks, C, s));
   if (Kb) TRY(upload(h->Kb, Kb, C, s));
   if (v0) TRY(upload(h->v0, v0, C, s));
-  if (a0) {
-    TRY(upload(h->a0, a0, C, s));
-    // l0 = sqrt(4 a0)/sqrt(3), Tissue3D.cpp:177, in float.
-    std::vector<float> l0(C);
-    for (int c = 0; c < C; c++) l0[c] = sqrtf(4.0f * a0[c]) / sqrtf(3.0f);
-    TRY(upload(h->l0, l0.data(), C, s));
-    HIPCHECK(hipStreamSynchronize(s));
-  }
+  if (a0) TRY(upload(h->a0, a0, C, s));
   HIPCHECK(hipStreamSynchronize(s));
   return DPM_OK;
 }
@@ -417,8 +432,7 @@ extern "C" int dpm3d_set_tissue_params(dpm_tissue* h, const float* L, const int*
   if (pbc) TRY(upload(h->tpbc, pbc, T, s));
   if (Kre) TRY(upload(h->tKre, Kre, T, s));
   if (Kat) TRY(upload(h->tKat, Kat, T, s));
-  std::vector<float> zero(T, 0.0f);
-  TRY(upload(h->tksg, ks_grad ? ks_grad : zero.data(), T, s));
+  if (ks_grad) TRY(upload(h->tksg, ks_grad, T, s));
   h->mask = mask & DPM3D_ALL;
   HIPCHECK(hipStreamSynchronize(s));
   if (h->graph) {
@@ -428,78 +442,108 @@ extern "C" int dpm3d_set_tissue_params(dpm_tissue* h, const float* L, const int*
   return DPM_OK;
 }
 
-extern "C" int dpm3d_set_positions(dpm_tissue* h, const float* xyz, size_t n) {
+// Host AoS (reference order) <-> device SoA of the handle's real type.
+template <typename In>
+static int set_positions3(dpm_tissue* h, const In* xyz, size_t n) {
   if (!h || h->dim != 3 || !xyz) return fail(DPM_EINVAL, "bad arguments");
   const size_t NVT = (size_t)h->C * h->nv;
   if (n != NVT) return fail(DPM_EINVAL, "position count does not match the tissue");
   if (!all_finite(xyz, 3 * n)) return fail(DPM_ENONFINITE, "Non-finite vertex coordinates");
-  std::vector<float> a(n), b(n), c(n);
-  for (size_t i = 0; i < n; i++) {
-    a[i] = xyz[3 * i];
-    b[i] = xyz[3 * i + 1];
-    c[i] = xyz[3 * i + 2];
-  }
   const int k = h->cur;
-  TRY(upload(h->x[k], a.data(), n, h->stream));
-  TRY(upload(h->y[k], b.data(), n, h->stream));
-  TRY(upload(h->z[k], c.data(), n, h->stream));
-  HIPCHECK(hipStreamSynchronize(h->stream));
+  auto put = [&](auto tag) -> int {
+    using R = decltype(tag);
+    std::vector<R> a(n), b(n), c(n);
+    for (size_t i = 0; i < n; i++) {
+      a[i] = (R)xyz[3 * i];
+      b[i] = (R)xyz[3 * i + 1];
+      c[i] = (R)xyz[3 * i + 2];
+    }
+    TRY(upload((R*)h->x[k], a.data(), n, h->stream));
+    TRY(upload((R*)h->y[k], b.data(), n, h->stream));
+    TRY(upload((R*)h->z[k], c.data(), n, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    return DPM_OK;
+  };
+  TRY(h->rs == 8 ? put(0.0) : put(0.0f));
   h->stats_valid = false;
   return DPM_OK;
 }
 
-static int download3(dpm_tissue* h, const float* a, const float* b, const float* c, float* xyz,
-                     size_t n) {
-  std::vector<float> A(n), B(n), Cc(n);
-  HIPCHECK(hipMemcpyAsync(A.data(), a, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-  HIPCHECK(hipMemcpyAsync(B.data(), b, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-  HIPCHECK(hipMemcpyAsync(Cc.data(), c, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-  HIPCHECK(hipStreamSynchronize(h->stream));
-  for (size_t i = 0; i < n; i++) {
-    xyz[3 * i] = A[i];
-    xyz[3 * i + 1] = B[i];
-    xyz[3 * i + 2] = Cc[i];
-  }
-  return DPM_OK;
+template <typename Out>
+static int download3(dpm_tissue* h, const void* a, const void* b, const void* c, Out* xyz, size_t n) {
+  auto get = [&](auto tag) -> int {
+    using R = decltype(tag);
+    std::vector<R> A(n), B(n), Cc(n);
+    HIPCHECK(hipMemcpyAsync(A.data(), a, n * sizeof(R), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipMemcpyAsync(B.data(), b, n * sizeof(R), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipMemcpyAsync(Cc.data(), c, n * sizeof(R), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < n; i++) {
+      xyz[3 * i] = (Out)A[i];
+      xyz[3 * i + 1] = (Out)B[i];
+      xyz[3 * i + 2] = (Out)Cc[i];
+    }
+    return DPM_OK;
+  };
+  return h->rs == 8 ? get(0.0) : get(0.0f);
 }
 
-extern "C" int dpm3d_get_positions(dpm_tissue* h, float* xyz, size_t n) {
+template <typename Out>
+static int get_positions3(dpm_tissue* h, Out* xyz, size_t n) {
   if (!h || h->dim != 3 || !xyz) return fail(DPM_EINVAL, "bad arguments");
   if (n != (size_t)h->C * h->nv) return fail(DPM_EINVAL, "position count does not match the tissue");
   const int k = h->cur;
   return download3(h, h->x[k], h->y[k], h->z[k], xyz, n);
 }
 
-extern "C" int dpm3d_get_forces(dpm_tissue* h, float* xyz, size_t n) {
+template <typename Out>
+static int get_forces3(dpm_tissue* h, Out* xyz, size_t n) {
   if (!h || h->dim != 3 || !xyz) return fail(DPM_EINVAL, "bad arguments");
   if (n != (size_t)h->C * h->nv) return fail(DPM_EINVAL, "force count does not match the tissue");
   return download3(h, h->fx, h->fy, h->fz, xyz, n);
 }
 
-static Step3D step_args(dpm_tissue* h, int in, float dt, int flags) {
-  Step3D a{};
-  a.xin = h->x[in];
-  a.yin = h->y[in];
-  a.zin = h->z[in];
-  a.xout = h->x[1 - in];
-  a.yout = h->y[1 - in];
-  a.zout = h->z[1 - in];
-  a.fx = h->fx;
-  a.fy = h->fy;
-  a.fz = h->fz;
-  a.st_in = h->st[in];
+extern "C" int dpm3d_set_positions(dpm_tissue* h, const float* xyz, size_t n) {
+  return set_positions3(h, xyz, n);
+}
+extern "C" int dpm3d_get_positions(dpm_tissue* h, float* xyz, size_t n) {
+  return get_positions3(h, xyz, n);
+}
+extern "C" int dpm3d_get_forces(dpm_tissue* h, float* xyz, size_t n) { return get_forces3(h, xyz, n); }
+extern "C" int dpm3d_set_positions_f64(dpm_tissue* h, const double* xyz, size_t n) {
+  return set_positions3(h, xyz, n);
+}
+extern "C" int dpm3d_get_positions_f64(dpm_tissue* h, double* xyz, size_t n) {
+  return get_positions3(h, xyz, n);
+}
+extern "C" int dpm3d_get_forces_f64(dpm_tissue* h, double* xyz, size_t n) {
+  return get_forces3(h, xyz, n);
+}
+
+template <typename R>
+static Step3D<R> step_args(dpm_tissue* h, int in, float dt, int flags) {
+  Step3D<R> a{};
+  a.xin = (const R*)h->x[in];
+  a.yin = (const R*)h->y[in];
+  a.zin = (const R*)h->z[in];
+  a.xout = (R*)h->x[1 - in];
+  a.yout = (R*)h->y[1 - in];
+  a.zout = (R*)h->z[1 - in];
+  a.fx = (R*)h->fx;
+  a.fy = (R*)h->fy;
+  a.fz = (R*)h->fz;
+  a.st_in = (const CellStat<R>*)h->st[in];
   a.vol_in = h->vol[in];
-  a.em_in = h->em[in];
-  a.st_out = h->st[1 - in];
+  a.em_in = (const R*)h->em[in];
+  a.st_out = (CellStat<R>*)h->st[1 - in];
   a.vol_out = h->vol[1 - in];
-  a.em_out = h->em[1 - in];
+  a.em_out = (R*)h->em[1 - in];
   a.Kv = h->Kv;
   a.Ka = h->Ka;
   a.Ks = h->Ks;
   a.Kb = h->Kb;
   a.v0 = h->v0;
   a.a0 = h->a0;
-  a.l0 = h->l0;
   a.cell_tissue = h->cell_tissue;
   a.tissue_begin = h->tissue_begin;
   a.tL = h->tL;
@@ -518,31 +562,53 @@ static Step3D step_args(dpm_tissue* h, int in, float dt, int flags) {
   a.nv = h->nv;
   a.nf = h->nf;
   a.ncells = h->C;
-  a.dt = dt;
+  a.dt = (R)dt;
   a.mask = h->mask;
   a.flags = flags;
   a.counters = h->counters;
   a.ncmax = h->ncmax;
+  a.energy = h->energy;
   return a;
+}
+
+enum Kind3D { K_STEP, K_STATS, K_ENERGY };
+
+// Launches one kernel of the 3D family over all cells, in the handle's
+// precision; `edit` may adjust the arguments before the launch.
+template <typename Edit>
+static int launch3d(dpm_tissue* h, Kind3D kind, int in, float dt, int flags, Edit edit) {
+  auto go = [&](auto tag) -> int {
+    using R = decltype(tag);
+    Step3D<R> a = step_args<R>(h, in, dt, flags);
+    edit(a);
+    void (*k)(Step3D<R>) = nullptr;
+    if constexpr (sizeof(R) == 8)
+      k = kind == K_STEP ? k3d_step_f64 : kind == K_STATS ? k3d_stats_f64 : k3d_energy_f64;
+    else
+      k = kind == K_STEP ? k3d_step_f32 : kind == K_STATS ? k3d_stats_f32 : k3d_energy_f32;
+    hipLaunchKernelGGL(k, dim3(h->C), dim3(h->NT), h->lds, h->stream, a);
+    HIPCHECK(hipGetLastError());
+    return DPM_OK;
+  };
+  return h->rs == 8 ? go(0.0) : go(0.0f);
 }
 
 static int ensure_stats(dpm_tissue* h) {
   if (h->stats_valid) return DPM_OK;
-  Step3D a = step_args(h, h->cur, 0.0f, 0);
-  a.st_out = h->st[h->cur];
-  a.vol_out = h->vol[h->cur];
-  a.em_out = h->em[h->cur];
-  hipLaunchKernelGGL(k3d_stats, dim3(h->C), dim3(h->NT), h->lds, h->stream, a);
-  HIPCHECK(hipGetLastError());
+  const int c = h->cur;
+  TRY(launch3d(h, K_STATS, c, 0.0f, 0, [&](auto& a) {
+    using S = std::remove_reference_t<decltype(*a.st_out)>;
+    using R = std::remove_reference_t<decltype(*a.em_out)>;
+    a.st_out = (S*)h->st[c];
+    a.vol_out = h->vol[c];
+    a.em_out = (R*)h->em[c];
+  }));
   h->stats_valid = true;
   return DPM_OK;
 }
 
 static int launch_step3d(dpm_tissue* h, int in, float dt, int flags) {
-  Step3D a = step_args(h, in, dt, flags);
-  hipLaunchKernelGGL(k3d_step, dim3(h->C), dim3(h->NT), h->lds, h->stream, a);
-  HIPCHECK(hipGetLastError());
-  return DPM_OK;
+  return launch3d(h, K_STEP, in, dt, flags, [](auto&) {});
 }
 
 // A graph of kGraphSteps steps starting from buffer 0 (ends on buffer 0).
@@ -606,17 +672,24 @@ extern "C" int dpm3d_cell_stats(dpm_tissue* h, double* volume, float* com) {
   if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
   TRY(ensure_stats(h));
   const int C = h->C;
-  std::vector<float4> st(C);
-  HIPCHECK(hipMemcpyAsync(st.data(), h->st[h->cur], C * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+  std::vector<unsigned char> st((size_t)C * 4 * h->rs);
+  HIPCHECK(hipMemcpyAsync(st.data(), h->st[h->cur], st.size(), hipMemcpyDeviceToHost, h->stream));
   if (volume)
     HIPCHECK(hipMemcpyAsync(volume, h->vol[h->cur], C * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHECK(hipStreamSynchronize(h->stream));
   if (com)
-    for (int c = 0; c < C; c++) {
-      com[3 * c] = st[c].x;
-      com[3 * c + 1] = st[c].y;
-      com[3 * c + 2] = st[c].z;
-    }
+    for (int c = 0; c < C; c++)
+      for (int k = 0; k < 3; k++)
+        com[3 * c + k] = h->rs == 8 ? (float)((const double*)st.data())[4 * c + k]
+                                    : ((const float*)st.data())[4 * c + k];
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_energy(dpm_tissue* h, double* energy) {
+  if (!h || h->dim != 3 || !energy) return fail(DPM_EINVAL, "bad arguments");
+  TRY(launch3d(h, K_ENERGY, h->cur, 0.0f, 0, [](auto&) {}));
+  HIPCHECK(hipMemcpyAsync(energy, h->energy, h->C * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
   return DPM_OK;
 }
 
@@ -640,15 +713,13 @@ extern "C" int dpm3d_probe(dpm_tissue* h, float* forces, unsigned char* contact,
     if (dw) (void)hipMemsetAsync(dw, 0, npair * sizeof(float), s);
     if (dd) (void)hipMemsetAsync(dd, 0, (size_t)h->C * h->nf, s);
     if (da) (void)hipMemsetAsync(da, 0, NVT * sizeof(int), s);
-    Step3D a = step_args(h, h->cur, 0.0f, F_WRITE_FORCES);
-    a.counters = nullptr;
-    a.p_contact = dc;
-    a.p_winding = dw;
-    a.p_down = dd;
-    a.p_adh = da;
-    hipLaunchKernelGGL(k3d_step, dim3(h->C), dim3(h->NT), h->lds, s, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) rc = fail(DPM_EDEVICE, std::string("k3d_step: ") + hipGetErrorString(e));
+    rc = launch3d(h, K_STEP, h->cur, 0.0f, F_WRITE_FORCES, [&](auto& a) {
+      a.counters = nullptr;
+      a.p_contact = dc;
+      a.p_winding = dw;
+      a.p_down = dd;
+      a.p_adh = da;
+    });
   }
   if (!rc && dc) rc = hipMemcpyAsync(contact, dc, npair, hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
   if (!rc && dw)
@@ -884,7 +955,10 @@ extern "C" int dpm_euler(dpm_tissue* h, int nsteps, float dt) {
   TRY(dpm_euler_async(h, nsteps, dt));
   if (h->dim == 3) {
     const size_t n = (size_t)h->C * h->nv;
-    return check_finite(h, h->x[h->cur], h->y[h->cur], h->z[h->cur], n);
+    const int c = h->cur;
+    if (h->rs == 8)
+      return check_finite(h, (const double*)h->x[c], (const double*)h->y[c], (const double*)h->z[c], n);
+    return check_finite(h, (const float*)h->x[c], (const float*)h->y[c], (const float*)h->z[c], n);
   }
   // The reference's 2D path performs no result check (Tissue2D.cpp:231-240).
   HIPCHECK(hipStreamSynchronize(h->stream));

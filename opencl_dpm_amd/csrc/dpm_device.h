@@ -45,6 +45,35 @@ __device__ __forceinline__ void normalize2(float x, float y, float& ux, float& u
   uy = y * inv;
 }
 
+// ------------------------------------------- the same helpers for float/double
+// (the 3D step is instantiated in both; double evaluates the same formulas
+// with the float thresholds promoted, as the FP64 restatement does)
+template <typename R>
+struct V3 {
+  R x, y, z;
+};
+template <typename R>
+__device__ __forceinline__ V3<R> v3(R x, R y, R z) { return {x, y, z}; }
+template <typename R>
+__device__ __forceinline__ V3<R> vsub(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename R>
+__device__ __forceinline__ V3<R> vadd(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename R>
+__device__ __forceinline__ V3<R> vscale(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <typename R>
+__device__ __forceinline__ R vdot(V3<R> a, V3<R> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+template <typename R>
+__device__ __forceinline__ V3<R> vcross(V3<R> a, V3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <typename R>
+__device__ __forceinline__ V3<R> vnormalize(V3<R> a) {
+  const R d = vdot(a, a);
+  if (d == R(0)) return {R(0), R(0), R(0)};
+  const R inv = R(1) / sqrt(d);
+  return {a.x * inv, a.y * inv, a.z * inv};
+}
+
 // ---------------------------------------------------------------- reductions
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, DPM_WAVE);
@@ -73,19 +102,20 @@ __device__ __forceinline__ double block_sum(double v, double* scratch) {
   __syncthreads();
   return r;
 }
-__device__ __forceinline__ float block_max(float v, float* scratch) {
+template <typename R>
+__device__ __forceinline__ R block_max(R v, R* scratch) {
   const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
   const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
-  v = wave_max(v);
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, DPM_WAVE));
   if (lane == 0) scratch[wave] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
-    float s = scratch[0];
-    for (int w = 1; w < nw; w++) s = fmaxf(s, scratch[w]);
+    R s = scratch[0];
+    for (int w = 1; w < nw; w++) s = fmax(s, scratch[w]);
     scratch[16] = s;
   }
   __syncthreads();
-  float r = scratch[16];
+  R r = scratch[16];
   __syncthreads();
   return r;
 }

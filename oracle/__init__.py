@@ -146,6 +146,22 @@ def volumes(X, faces, nv, prec=64):
     return out
 
 
+def energy3d(X, faces, nv, Kv, Ka, v0, a0, Kb=0.0, mask=35, prec=64):
+    """Per-cell elastic energy (volume, edge springs, bending; mask bits 1, 2, 32)."""
+    dt_, ct, sfx = _t(prec)
+    X = np.ascontiguousarray(np.asarray(X, dt_).reshape(-1, 3))
+    nc = X.shape[0] // nv
+    faces = np.ascontiguousarray(np.asarray(faces, np.int32).reshape(-1, 3))
+    adj = adjacency(faces, nv)
+    arr = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dt_), (nc,)))
+    ps = [arr(a) for a in (Kv, Ka, Kb, v0, a0)]
+    out = np.zeros(nc, dt_)
+    getattr(_lib("liboracle.so"), "ora3d_energy" + sfx)(
+        ctypes.c_int(nc), ctypes.c_int(nv), ctypes.c_int(faces.shape[0]), _p(faces), _p(adj),
+        ctypes.c_int(8), _p(X), *[_p(a) for a in ps], ctypes.c_uint(mask), _p(out))
+    return out
+
+
 # ------------------------------------------------------ the reference's own host code
 
 REF_LIB = os.path.join(_HERE, "_ref", "libdpmref.so")

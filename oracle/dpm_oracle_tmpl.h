@@ -589,3 +589,51 @@ R FN(ora3d_winding)(int nv, int nf, const int* faces, const R* X, int cj,
   return FN(omega_sum)(faces, nf, X, nv, cj, shift, p) /
          ((R)4.0f * (R)3.14159265358979323846);
 }
+
+/* Elastic energy of each cell, no reference counterpart (the reference
+ * computes forces only).  The energy whose negative gradient is the volume
+ * force (Cell3D_Kernel.cl:66-112: F = -Kv (V/v0 - 1) dV/dx), the edge
+ * springs of the surface force (:114-177: each face edge pulls with
+ * sc (l/l0 - 1), sc = 0.3 Ka sqrt(a0)/l0) and the graph-Laplacian bending
+ * (F = -Kb L L x):
+ *   U = Kv v0/2 (V/v0 - 1)^2 + sum_faces sum_edges sc l0/2 (l/l0 - 1)^2
+ *       + Kb/2 sum_v |(L x)_v|^2,
+ * restricted to mask bits 1, 2, 32.  Pinned by central differences of U
+ * against ora3d_forces in the tests. */
+void FN(ora3d_energy)(int ncells, int nv, int nf, const int* faces,
+                      const int* adj, int maxdeg, const R* X, const R* Kv,
+                      const R* Ka, const R* Kb, const R* v0, const R* a0,
+                      unsigned mask, R* E) {
+  for (int ci = 0; ci < ncells; ci++) {
+    const R l0 = SQRT((R)4.0f * a0[ci]) / SQRT((R)3.0f);
+    R e = 0;
+    if ((mask & 1u) && Kv[ci] != 0.0f) {
+      const R s = FN(volume3)(faces, nf, X, nv, ci) / v0[ci] - (R)1;
+      e += (R)0.5 * Kv[ci] * v0[ci] * s * s;
+    }
+    if ((mask & 2u) && !(Ka[ci] < 1e-8f)) {
+      const R sc = Ka[ci] * SQRT(a0[ci]) / l0 * 0.3f;
+      for (int f = 0; f < nf; f++)
+        for (int k = 0; k < 3; k++) {
+          const R* P = &X[((size_t)ci * nv + faces[f * 3 + k]) * 3];
+          const R* Q = &X[((size_t)ci * nv + faces[f * 3 + (k + 1) % 3]) * 3];
+          const R d[3] = {Q[0] - P[0], Q[1] - P[1], Q[2] - P[2]};
+          const R s = SQRT(FN(dot3)(d, d)) / l0 - (R)1;
+          e += (R)0.5 * sc * l0 * s * s;
+        }
+    }
+    if ((mask & 32u) && Kb[ci] != 0.0f) {
+      for (int v = 0; v < nv; v++) {
+        R s[3] = {0, 0, 0};
+        for (int k = 0; k < maxdeg; k++) {
+          const int u = adj[v * maxdeg + k];
+          if (u < 0) break;
+          for (int c = 0; c < 3; c++)
+            s[c] += X[((size_t)ci * nv + u) * 3 + c] - X[((size_t)ci * nv + v) * 3 + c];
+        }
+        e += (R)0.5 * Kb[ci] * FN(dot3)(s, s);
+      }
+    }
+    E[ci] = e;
+  }
+}

@@ -249,6 +249,24 @@ def test_bending_is_gradient_and_invariant():
 
 
 @pytest.mark.parametrize("mask", [1, 2, 32, 35])
+def test_energy_is_potential_of_shape_forces(mask):
+    # the oracle's energy (used for the drift study) against the oracle's
+    # forces, themselves pinned above against independent energies
+    mesh, X1 = ico(1, 1.1, (0.3, 0.2, 1.5), seed=7, noise=0.04)
+    _, X2 = ico(1, 0.9, (3.0, -1.0, 0.7), seed=8, noise=0.04)
+    X = np.concatenate([X1, X2])
+    p = dict(Kv=[1.3, 0.8], Ka=[0.7, 0.4], v0=[3.0, 2.5], a0=[0.05, 0.07], Kb=[0.3, 0.1])
+    E = lambda Y: oracle.energy3d(Y, mesh.faces, mesh.nv, p["Kv"], p["Ka"], p["v0"], p["a0"],
+                                  Kb=p["Kb"], mask=mask).sum()
+    F = oracle.forces3d(X, mesh.faces, mesh.nv, p["Kv"], p["Ka"], 0.5, p["v0"], p["a0"], 1000.0, 0.0,
+                        Kb=p["Kb"], mask=mask, prec=64)["forces"]
+    np.testing.assert_allclose(F, -fd_grad(E, X), rtol=1e-6, atol=1e-8)
+    # zero for the disabled terms, translation invariant
+    assert oracle.energy3d(X, mesh.faces, mesh.nv, 1, 1, 3, 0.05, Kb=1, mask=0).max() == 0.0
+    np.testing.assert_allclose(E(X + [4.0, -2.0, 1.0]), E(X), rtol=1e-12)
+
+
+@pytest.mark.parametrize("mask", [1, 2, 32, 35])
 def test_3d_internal_forces_sum_to_zero(mask):
     mesh, X = ico(2, 1.0, (1.0, 2.0, 3.0), seed=5, noise=0.02)
     F = F3(X, mesh, mask)["forces"]
