@@ -43,7 +43,18 @@ typedef struct dpm_tissue dpm_tissue;
 #define DPM2D_BENDING 4u
 #define DPM2D_ATTRACTION 8u
 #define DPM2D_REPULSION 16u
-#define DPM2D_ALL 31u
+#define DPM2D_ALL 31u       /* exactly the terms Tissue2D::CLEulerUpdate runs */
+/* Two 2D terms with no reference counterpart that make the force law
+ * conservative (the reference's area force has a non-gradient y sign and
+ * its crossing repulsion is not a gradient), for FIRE and jamming:
+ * AREA_GRAD = -grad of Ka sqrt(a0)/2 (|A|/a0 - 1)^2 (equals the reference's
+ * x component on counter-clockwise cells); VREPULSION = soft disks on the
+ * vertices of different cells, U = Kre/2 (1 - d/sigma)^2 for
+ * d < sigma = (l0_i + l0_j)/2, minimum image. */
+#define DPM2D_AREA_GRAD 32u
+#define DPM2D_VREPULSION 64u
+#define DPM2D_CONSERVATIVE 102u /* AREA_GRAD | PERIMETER | BENDING | VREPULSION */
+#define DPM2D_ALL_TERMS 127u
 
 #define DPM3D_VOLUME 1u     /* VolumeForceUpdate, Cell3D_Kernel.cl:66-112 */
 #define DPM3D_SURFACE 2u    /* SurfaceAreaForceUpdate, :114-177 */
@@ -169,7 +180,7 @@ typedef struct {
   const int* pbc;
   const float* Kre;
   const float* Kat;
-  unsigned mask;          /* DPM2D_* terms */
+  unsigned mask;          /* DPM2D_* terms (DPM2D_ALL = the reference's) */
 } dpm2d_desc;
 
 int dpm2d_create(const dpm2d_desc* desc, dpm_tissue** out);
@@ -184,8 +195,92 @@ int dpm2d_set_tissue_params(dpm_tissue* h, const float* L, const int* pbc,
                             const float* Kre, const float* Kat, unsigned mask);
 /* Diagnostic as dpm3d_probe: forces at the current positions; overlap[v] = 1
  * where the repulsion pair exists (Cell2D_kernel.cl:199-205); attraction[v]
- * = number of attraction partners (Cell2D_kernel.cl:261). */
+ * = number of attraction partners (Cell2D_kernel.cl:261).  On a double
+ * tissue only the forces are available (overlap, attraction must be NULL). */
 int dpm2d_probe(dpm_tissue* h, float* forces, int* overlap, int* attraction);
+
+/* Double-precision 2D tissue (positions, forces, velocities, box in double;
+ * the same formulas), as dpm3d_create_f64.  Its Euler steps, FIRE and
+ * jamming all run in double. */
+int dpm2d_create_f64(const dpm2d_desc* desc, dpm_tissue** out);
+int dpm2d_set_positions_f64(dpm_tissue* h, const double* xy, size_t n);
+int dpm2d_get_positions_f64(dpm_tissue* h, double* xy, size_t n);
+int dpm2d_get_forces_f64(dpm_tissue* h, double* xy, size_t n);
+/* Box length of every tissue (jamming changes it). */
+int dpm2d_get_box(dpm_tissue* h, double* L);
+
+/* FIRE minimisation (Bitzek, Koskinen, Gaehler, Moseler & Gumbsch, PRL 97,
+ * 170201 (2006)) of every tissue with the tissue's force terms (use
+ * DPM2D_CONSERVATIVE: FIRE needs forces that are minus an energy gradient),
+ * semi-implicit Euler MD with unit masses, power P = F.v and the norms |F|,
+ * |v| reduced in double per tissue; converged when max_v |F_v| < ftol
+ * (ftol = 0 runs exactly max_iter iterations).  No
+ * reference counterpart (the reference has only the Euler step).  One
+ * launch for the whole batch, one workgroup per tissue. */
+typedef struct {
+  double dt0, dt_max, ftol, f_inc, f_dec, alpha0, f_alpha;
+  int max_iter, n_min;
+} dpm_fire_params;
+/* dt0 0.01, dt_max 0.1, ftol 1e-10, f_inc 1.1, f_dec 0.5, alpha0 0.1,
+ * f_alpha 0.99, max_iter 100000, n_min 5. */
+void dpm_fire_defaults(dpm_fire_params* p);
+/* Per-tissue results of dpm2d_fire / dpm2d_jam / dpm2d_energy. */
+typedef struct {
+  double iterations;   /* Euler steps or FIRE iterations (all stages) */
+  double converged;    /* 1 if the final max |F_v| < ftol */
+  double fmax;         /* final max_v |F_v| */
+  double energy;       /* final conservative energy (area, perimeter,
+                          bending, vertex repulsion) */
+  double e_int;        /* its vertex-repulsion part */
+  double phi;          /* final packing fraction sum(a0)/L^2 */
+  double phi_lo;       /* jamming: largest unjammed phi found */
+  double phi_hi;       /* jamming: smallest jammed phi found (0 = none) */
+  double stages;       /* jamming: minimisations run */
+  double L;            /* final box length */
+  double unconverged;  /* minimisations that stopped at max_iter */
+  double status;       /* jamming: 0 bracket within phi_tol, 1 phi0 was
+                          already jammed, 2 in progress, 3 stopped at
+                          max_stages */
+  double phi_target;   /* jamming: phi of the next minimisation */
+} dpm_relax_stats;
+int dpm2d_fire(dpm_tissue* h, const dpm_fire_params* p, dpm_relax_stats* stats);
+/* Energy, e_int, fmax, phi, L of the current state (forces updated). */
+int dpm2d_energy(dpm_tissue* h, dpm_relax_stats* stats);
+
+/* Jamming by compression and bisection on phi = sum(a0)/L^2 (O'Hern,
+ * Silbert, Liu & Nagel, PRE 68, 011306 (2003)): cells are moved rigidly so
+ * that their centres scale with the box; at each phi a FIRE minimisation
+ * that stops early ("cleared") once the vertex-repulsion energy per cell is
+ * at most u_jam; the state is jammed iff it was not cleared (FIRE reached
+ * ftol or max_iter with that energy above u_jam).  phi grows by dphi from phi0 until the first jammed state, then
+ * bisects between the largest unjammed and smallest jammed phi (restarting
+ * from the saved unjammed state) until phi_hi - phi_lo < phi_tol or
+ * max_stages minimisations.  Ends on the largest unjammed state.  The whole
+ * protocol of every tissue is one launch, captured in a hipGraph that later
+ * calls with the same parameters replay.  Needs DPM2D_VREPULSION. */
+typedef struct {
+  double phi0, dphi, phi_tol, u_jam;
+  int max_stages;
+  dpm_fire_params fire;
+} dpm_jam_params;
+/* phi0 0.35, dphi 0.02, phi_tol 1e-3, u_jam 1e-16, max_stages 40, FIRE
+ * defaults with max_iter 20000. */
+void dpm_jam_defaults(dpm_jam_params* p);
+int dpm2d_jam(dpm_tissue* h, const dpm_jam_params* p, dpm_relax_stats* stats);
+/* The same protocol over several launches: start != 0 begins it, each call
+ * runs at most `stages` more minimisations per tissue (status 2 while one
+ * is unfinished; the state stays on the device).  Any split gives results
+ * bit-identical to dpm2d_jam; callers use it to report progress or bound a
+ * launch's duration. */
+int dpm2d_jam_step(dpm_tissue* h, const dpm_jam_params* p, int start, int stages,
+                   dpm_relax_stats* stats);
+/* Workgroup size of the FIRE / jamming / double-Euler kernel (64..1024, a
+ * multiple of 64; 0 = default), and the autotuner that picks it by timing
+ * `iters` FIRE iterations of the batch at each size (positions restored);
+ * times_ms[5] (may be NULL) receives the timings at 64..1024. */
+int dpm2d_set_relax_threads(dpm_tissue* h, int threads);
+int dpm2d_relax_threads(dpm_tissue* h);
+int dpm2d_autotune(dpm_tissue* h, int iters, int* best, float* times_ms);
 
 /* ------------------------------------------------------------- common ---
  * Overdamped forward Euler, x += F*dt, nsteps times (the step loops of

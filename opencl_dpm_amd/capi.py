@@ -21,6 +21,9 @@ DPM_OK, DPM_EINVAL, DPM_ENONFINITE, DPM_EDEVICE, DPM_ENOMEM, DPM_ERANGE = range(
 
 DPM2D_AREA, DPM2D_PERIMETER, DPM2D_BENDING, DPM2D_ATTRACTION, DPM2D_REPULSION = 1, 2, 4, 8, 16
 DPM2D_ALL = 31
+DPM2D_AREA_GRAD, DPM2D_VREPULSION = 32, 64
+DPM2D_CONSERVATIVE = 102
+DPM2D_ALL_TERMS = 127
 DPM3D_VOLUME, DPM3D_SURFACE, DPM3D_SUBSTRATE, DPM3D_CONTACT = 1, 2, 4, 8
 DPM3D_ADHESION, DPM3D_BENDING = 16, 32
 DPM3D_REFERENCE, DPM3D_ALL = 15, 63
@@ -34,6 +37,9 @@ EXPORTS = [
     "dpm3d_get_forces_f64", "dpm3d_energy",
     "dpm2d_create", "dpm2d_set_positions", "dpm2d_get_positions", "dpm2d_get_forces",
     "dpm2d_set_cell_params", "dpm2d_set_tissue_params", "dpm2d_probe",
+    "dpm2d_create_f64", "dpm2d_set_positions_f64", "dpm2d_get_positions_f64", "dpm2d_get_forces_f64",
+    "dpm2d_get_box", "dpm_fire_defaults", "dpm_jam_defaults", "dpm2d_fire", "dpm2d_energy", "dpm2d_jam",
+    "dpm2d_set_relax_threads", "dpm2d_relax_threads", "dpm2d_autotune", "dpm2d_jam_step",
     "dpm_euler", "dpm_euler_async", "dpm_synchronize", "dpm_stream", "dpm_counters",
     "dpm_destroy",
 ]
@@ -60,7 +66,8 @@ def load() -> ctypes.CDLL:
     lib.dpm_stream.argtypes = [vp]
     for name in ["dpm3d_set_positions", "dpm3d_get_positions", "dpm3d_get_forces",
                  "dpm3d_set_positions_f64", "dpm3d_get_positions_f64", "dpm3d_get_forces_f64",
-                 "dpm2d_set_positions", "dpm2d_get_positions", "dpm2d_get_forces"]:
+                 "dpm2d_set_positions", "dpm2d_get_positions", "dpm2d_get_forces",
+                 "dpm2d_set_positions_f64", "dpm2d_get_positions_f64", "dpm2d_get_forces_f64"]:
         getattr(lib, name).argtypes = [vp, vp, sz]
     lib.dpm_euler.argtypes = [vp, i, fp]
     lib.dpm_euler_async.argtypes = [vp, i, fp]
@@ -72,6 +79,19 @@ def load() -> ctypes.CDLL:
     lib.dpm3d_create_f64.argtypes = [vp, vp]
     lib.dpm3d_energy.argtypes = [vp, vp]
     lib.dpm2d_create.argtypes = [vp, vp]
+    lib.dpm2d_create_f64.argtypes = [vp, vp]
+    lib.dpm2d_get_box.argtypes = [vp, vp]
+    lib.dpm_fire_defaults.argtypes = [vp]
+    lib.dpm_fire_defaults.restype = None
+    lib.dpm_jam_defaults.argtypes = [vp]
+    lib.dpm_jam_defaults.restype = None
+    lib.dpm2d_fire.argtypes = [vp, vp, vp]
+    lib.dpm2d_energy.argtypes = [vp, vp]
+    lib.dpm2d_jam.argtypes = [vp, vp, vp]
+    lib.dpm2d_jam_step.argtypes = [vp, vp, i, i, vp]
+    lib.dpm2d_set_relax_threads.argtypes = [vp, i]
+    lib.dpm2d_relax_threads.argtypes = [vp]
+    lib.dpm2d_autotune.argtypes = [vp, i, vp, vp]
     lib.dpm3d_set_cell_params.argtypes = [vp] + [vp] * 6
     lib.dpm3d_set_tissue_params.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_uint]
     lib.dpm2d_set_cell_params.argtypes = [vp] + [vp] * 6
@@ -120,6 +140,38 @@ class _Desc2D(ctypes.Structure):
         ("L", ctypes.c_void_p), ("pbc", ctypes.c_void_p), ("Kre", ctypes.c_void_p),
         ("Kat", ctypes.c_void_p), ("mask", ctypes.c_uint),
     ]
+
+
+class FireParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("dt0", "dt_max", "ftol", "f_inc", "f_dec", "alpha0", "f_alpha")] + \
+               [("max_iter", ctypes.c_int), ("n_min", ctypes.c_int)]
+
+
+class JamParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("phi0", "dphi", "phi_tol", "u_jam")] + \
+               [("max_stages", ctypes.c_int), ("fire", FireParams)]
+
+
+RELAX_STATS = ("iterations", "converged", "fmax", "energy", "e_int", "phi", "phi_lo", "phi_hi", "stages", "L",
+               "unconverged", "status", "phi_target")
+
+
+def fire_params(**kw) -> FireParams:
+    p = FireParams()
+    load().dpm_fire_defaults(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def jam_params(fire=None, **kw) -> JamParams:
+    p = JamParams()
+    load().dpm_jam_defaults(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    for k, v in (fire or {}).items():
+        setattr(p.fire, k, v)
+    return p
 
 
 def _f32(a) -> np.ndarray:
@@ -256,24 +308,34 @@ class Tissues3D(_Handle):
         _check(load().dpm3d_cell_stats(self._h, _p(vol), _p(com)))
         return vol, com
 
-    def probe(self, winding: bool = False):
-        """Forces at the current positions plus the recorded pair sets."""
+    def eval_forces(self, dtype=None) -> np.ndarray:
+        """Forces at the current positions, positions unchanged (dpm3d_probe
+        with no pair tables), in the handle's precision or `dtype`."""
+        _check(load().dpm3d_probe(self._h, None, None, None, None, None))
+        return self.forces(dtype)
+
+    def probe(self, winding: bool = False, contact: bool = True):
+        """Forces at the current positions (handle precision) plus the
+        recorded pair sets; contact=False skips the vertex x cell table."""
         ncmax = int(np.max(np.diff(self.cell_begin)))
-        f = np.empty((self.nverts, 3), np.float32)
-        contact = np.zeros((self.nverts, ncmax), np.uint8)
+        ct = np.zeros((self.nverts, ncmax), np.uint8) if contact else None
         w = np.zeros((self.nverts, ncmax), np.float32) if winding else None
         down = np.zeros((self.C, self.nf), np.uint8)
         adh = np.zeros(self.nverts, np.int32)
-        _check(load().dpm3d_probe(self._h, _p(f), _p(contact), _p(w), _p(down), _p(adh)))
-        return {"forces": f, "contact": contact, "winding": w, "down": down, "adhesion": adh}
+        _check(load().dpm3d_probe(self._h, None, _p(ct), _p(w), _p(down), _p(adh)))
+        return {"forces": self.forces(), "contact": ct, "winding": w, "down": down, "adhesion": adh}
 
 
 class Tissues2D(_Handle):
-    """A batch of 2D tissues (include/dpm.h, dpm2d_*); vertices packed."""
+    """A batch of 2D tissues (include/dpm.h, dpm2d_*); vertices packed.
+    precision=64 makes an all-double tissue (dpm2d_create_f64)."""
 
     def __init__(self, cell_begin, nv, Ka, Kl, Kb, a0, l0, r0, L, Kre=1.0, Kat=0.0, pbc=1,
-                 mask: int = DPM2D_ALL):
+                 mask: int = DPM2D_ALL, precision: int = 32):
         super().__init__()
+        if precision not in (32, 64):
+            raise ValueError("precision must be 32 or 64")
+        self.precision = precision
         self.cell_begin = _i32(cell_begin)
         self.T = len(self.cell_begin) - 1
         self.nv = _i32(nv)
@@ -291,21 +353,84 @@ class Tissues2D(_Handle):
         d.Ka, d.Kl, d.Kb, d.a0, d.l0, d.r0 = [_p(a) for a in self._cell]
         d.L, d.pbc, d.Kre, d.Kat = [_p(a) for a in self._tis]
         d.mask = mask
-        _check(load().dpm2d_create(ctypes.byref(d), ctypes.byref(self._h)))
+        create = load().dpm2d_create_f64 if precision == 64 else load().dpm2d_create
+        _check(create(ctypes.byref(d), ctypes.byref(self._h)))
 
     def set_positions(self, xy) -> None:
-        a = _f32(xy).reshape(-1, 2)
-        _check(load().dpm2d_set_positions(self._h, _p(a), a.shape[0]))
+        if np.asarray(xy).dtype == np.float32:
+            a = _f32(xy).reshape(-1, 2)
+            _check(load().dpm2d_set_positions(self._h, _p(a), a.shape[0]))
+        else:
+            a = np.ascontiguousarray(np.asarray(xy, np.float64)).reshape(-1, 2)
+            _check(load().dpm2d_set_positions_f64(self._h, _p(a), a.shape[0]))
 
-    def positions(self) -> np.ndarray:
-        out = np.empty((self.V, 2), np.float32)
-        _check(load().dpm2d_get_positions(self._h, _p(out), self.V))
+    def _get(self, f32, f64, dtype):
+        dtype = np.dtype(dtype or (np.float64 if self.precision == 64 else np.float32))
+        out = np.empty((self.V, 2), dtype)
+        _check((f64 if dtype == np.float64 else f32)(self._h, _p(out), self.V))
         return out
 
-    def forces(self) -> np.ndarray:
-        out = np.empty((self.V, 2), np.float32)
-        _check(load().dpm2d_get_forces(self._h, _p(out), self.V))
+    def positions(self, dtype=None) -> np.ndarray:
+        return self._get(load().dpm2d_get_positions, load().dpm2d_get_positions_f64, dtype)
+
+    def forces(self, dtype=None) -> np.ndarray:
+        return self._get(load().dpm2d_get_forces, load().dpm2d_get_forces_f64, dtype)
+
+    def box(self) -> np.ndarray:
+        out = np.empty(self.T, np.float64)
+        _check(load().dpm2d_get_box(self._h, _p(out)))
         return out
+
+    def set_tissue_params(self, L=None, pbc=None, Kre=None, Kat=None, mask=DPM2D_ALL):
+        bc = lambda v: None if v is None else _f32(np.broadcast_to(np.asarray(v, np.float32), (self.T,)))
+        arrs = [bc(L), None if pbc is None else _i32(np.broadcast_to(np.asarray(pbc), (self.T,))),
+                bc(Kre), bc(Kat)]
+        self._keep = arrs
+        _check(load().dpm2d_set_tissue_params(self._h, *[_p(a) for a in arrs], ctypes.c_uint(mask)))
+
+    @staticmethod
+    def _stats(buf):
+        a = buf.reshape(-1, len(RELAX_STATS))
+        return {k: a[:, i].copy() for i, k in enumerate(RELAX_STATS)}
+
+    def fire(self, **kw):
+        """FIRE minimisation of every tissue (dpm2d_fire); per-tissue stats."""
+        p = fire_params(**kw)
+        buf = np.zeros(self.T * len(RELAX_STATS))
+        _check(load().dpm2d_fire(self._h, ctypes.byref(p), _p(buf)))
+        return self._stats(buf)
+
+    def energy(self):
+        buf = np.zeros(self.T * len(RELAX_STATS))
+        _check(load().dpm2d_energy(self._h, _p(buf)))
+        return self._stats(buf)
+
+    def jam(self, fire=None, **kw):
+        """Jamming protocol of every tissue in one (graph-replayed) launch."""
+        p = jam_params(fire=fire, **kw)
+        buf = np.zeros(self.T * len(RELAX_STATS))
+        _check(load().dpm2d_jam(self._h, ctypes.byref(p), _p(buf)))
+        return self._stats(buf)
+
+    def jam_step(self, start: bool, stages: int, fire=None, **kw):
+        """At most `stages` more minimisations of a jamming protocol
+        (dpm2d_jam_step); status 2 while unfinished."""
+        p = jam_params(fire=fire, **kw)
+        buf = np.zeros(self.T * len(RELAX_STATS))
+        _check(load().dpm2d_jam_step(self._h, ctypes.byref(p), int(bool(start)), int(stages), _p(buf)))
+        return self._stats(buf)
+
+    def set_relax_threads(self, n: int) -> None:
+        _check(load().dpm2d_set_relax_threads(self._h, int(n)))
+
+    def relax_threads(self) -> int:
+        return int(load().dpm2d_relax_threads(self._h))
+
+    def autotune(self, iters: int = 50):
+        best = ctypes.c_int(0)
+        times = np.zeros(5, np.float32)
+        _check(load().dpm2d_autotune(self._h, int(iters), ctypes.byref(best), _p(times)))
+        return best.value, dict(zip((64, 128, 256, 512, 1024), times.tolist()))
 
     def probe(self):
         f = np.empty((self.V, 2), np.float32)

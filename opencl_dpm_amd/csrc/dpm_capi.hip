@@ -102,12 +102,20 @@ struct dpm_tissue {
   int* flag = nullptr;  // device non-finite flag
 
   // ------------------------------------------------------------------ 2D
-  int V = 0, NT2 = 0;
+  int V = 0, NT2 = 0, NTr = 0;  // Euler threads, relax threads per tissue
   size_t lds2 = 0;
-  float *x2 = nullptr, *y2 = nullptr, *fx2 = nullptr, *fy2 = nullptr;
+  // arrays of the real type (rs bytes)
+  void *x2 = nullptr, *y2 = nullptr, *fx2 = nullptr, *fy2 = nullptr;
+  void *vx2 = nullptr, *vy2 = nullptr, *sx2 = nullptr, *sy2 = nullptr;  // FIRE / jamming
+  void* tLr = nullptr;  // per-tissue box length
+  double* res2 = nullptr;     // [T * RS_COUNT]
+  double* sum_a0 = nullptr;   // [T]
   int *tcb = nullptr, *cvb = nullptr;
   float *Kl = nullptr, *l0_2 = nullptr, *r0 = nullptr;
   std::vector<int> cvb_h;
+  std::vector<float> a0_h;
+  hipGraphExec_t jam_graph = nullptr;
+  std::vector<unsigned char> jam_key;
 
   template <class T_>
   int alloc(T_** p, size_t n) {
@@ -123,6 +131,7 @@ struct dpm_tissue {
   }
   ~dpm_tissue() {
     if (graph) (void)hipGraphExecDestroy(graph);
+    if (jam_graph) (void)hipGraphExecDestroy(jam_graph);
     for (void* p : allocs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -741,7 +750,9 @@ extern "C" int dpm3d_probe(dpm_tissue* h, float* forces, unsigned char* contact,
 
 // ================================================================== 2D ====
 
-extern "C" int dpm2d_create(const dpm2d_desc* d, dpm_tissue** out) {
+static_assert(sizeof(dpm_relax_stats) == RS_COUNT * sizeof(double), "dpm_relax_stats layout");
+
+static int create2d(const dpm2d_desc* d, dpm_tissue** out, int rs) {
   if (!d || !out) return fail(DPM_EINVAL, "null argument");
   *out = nullptr;
   const int T = d->ntissues;
@@ -765,28 +776,34 @@ extern "C" int dpm2d_create(const dpm2d_desc* d, dpm_tissue** out) {
   }
   dpm_tissue* h = new dpm_tissue();
   h->dim = 2;
+  h->rs = rs;
   int rc = init_common(h);
   if (rc) { delete h; return rc; }
   h->T = T;
   h->C = C;
   h->V = cvb[C];
   h->NT2 = std::min(1024, std::max(64, (maxv + 63) / 64 * 64));
-  if (maxv > DPM2D_VPT * h->NT2) {
+  h->NTr = std::min(1024, std::max(64, (maxv / 4 + 63) / 64 * 64));
+  if (rs == 4 && maxv > DPM2D_VPT * h->NT2) {
     delete h;
     return fail(DPM_ERANGE, "tissue has more than 16384 vertices");
   }
-  h->lds2 = lds2d_bytes(maxv, maxc);
+  h->lds2 = lds2d_bytes(maxv, maxc, rs);
   if ((int)h->lds2 > max_lds_per_block(h->device)) {
     delete h;
     return fail(DPM_ERANGE, "2D tissue does not fit the LDS of one workgroup");
   }
   h->cvb_h = cvb;
   h->tissue_begin_h.assign(d->cell_begin, d->cell_begin + T + 1);
+  auto AR = [&](void** p, size_t n) { return h->alloc_real(p, n); };
   rc = 0;
-  rc |= h->alloc(&h->x2, h->V);
-  rc |= h->alloc(&h->y2, h->V);
-  rc |= h->alloc(&h->fx2, h->V);
-  rc |= h->alloc(&h->fy2, h->V);
+  rc |= AR(&h->x2, h->V);
+  rc |= AR(&h->y2, h->V);
+  rc |= AR(&h->fx2, h->V);
+  rc |= AR(&h->fy2, h->V);
+  rc |= AR(&h->tLr, T);
+  rc |= h->alloc(&h->res2, (size_t)T * RS_COUNT);
+  rc |= h->alloc(&h->sum_a0, T);
   rc |= h->alloc(&h->tcb, T + 1);
   rc |= h->alloc(&h->cvb, C + 1);
   rc |= h->alloc(&h->Ka, C);
@@ -795,7 +812,6 @@ extern "C" int dpm2d_create(const dpm2d_desc* d, dpm_tissue** out) {
   rc |= h->alloc(&h->a0, C);
   rc |= h->alloc(&h->l0_2, C);
   rc |= h->alloc(&h->r0, C);
-  rc |= h->alloc(&h->tL, T);
   rc |= h->alloc(&h->tpbc, T);
   rc |= h->alloc(&h->tKre, T);
   rc |= h->alloc(&h->tKat, T);
@@ -803,15 +819,21 @@ extern "C" int dpm2d_create(const dpm2d_desc* d, dpm_tissue** out) {
   hipStream_t s = h->stream;
   rc |= upload(h->tcb, d->cell_begin, T + 1, s);
   rc |= upload(h->cvb, cvb.data(), C + 1, s);
-  (void)hipMemsetAsync(h->fx2, 0, h->V * sizeof(float), s);
-  (void)hipMemsetAsync(h->fy2, 0, h->V * sizeof(float), s);
+  (void)hipMemsetAsync(h->fx2, 0, (size_t)h->V * rs, s);
+  (void)hipMemsetAsync(h->fy2, 0, (size_t)h->V * rs, s);
+  (void)hipMemsetAsync(h->res2, 0, sizeof(double) * T * RS_COUNT, s);
   if (!rc) rc = dpm2d_set_cell_params(h, d->Ka, d->Kl, d->Kb, d->a0, d->l0, d->r0);
   if (!rc) rc = dpm2d_set_tissue_params(h, d->L, d->pbc, d->Kre, d->Kat, d->mask);
   if (rc) { delete h; return rc; }
   (void)hipFuncSetAttribute((const void*)k2d_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds2);
+  (void)hipFuncSetAttribute(rs == 8 ? (const void*)k2d_relax_f64 : (const void*)k2d_relax_f32,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds2);
   *out = h;
   return DPM_OK;
 }
+
+extern "C" int dpm2d_create(const dpm2d_desc* d, dpm_tissue** out) { return create2d(d, out, 4); }
+extern "C" int dpm2d_create_f64(const dpm2d_desc* d, dpm_tissue** out) { return create2d(d, out, 8); }
 
 extern "C" int dpm2d_set_cell_params(dpm_tissue* h, const float* Ka, const float* Kl,
                                      const float* Kb, const float* a0, const float* l0,
@@ -822,10 +844,32 @@ extern "C" int dpm2d_set_cell_params(dpm_tissue* h, const float* Ka, const float
   if (Ka) TRY(upload(h->Ka, Ka, C, s));
   if (Kl) TRY(upload(h->Kl, Kl, C, s));
   if (Kb) TRY(upload(h->Kb, Kb, C, s));
-  if (a0) TRY(upload(h->a0, a0, C, s));
+  if (a0) {
+    TRY(upload(h->a0, a0, C, s));
+    h->a0_h.assign(a0, a0 + C);
+    std::vector<double> sa(h->T, 0.0);  // sum of rest areas per tissue
+    for (int t = 0; t < h->T; t++)
+      for (int c = h->tissue_begin_h[t]; c < h->tissue_begin_h[t + 1]; c++) sa[t] += a0[c];
+    TRY(upload(h->sum_a0, sa.data(), h->T, s));
+    HIPCHECK(hipStreamSynchronize(s));
+  }
   if (l0) TRY(upload(h->l0_2, l0, C, s));
   if (r0) TRY(upload(h->r0, r0, C, s));
   HIPCHECK(hipStreamSynchronize(s));
+  return DPM_OK;
+}
+
+template <typename In>
+static int upload_real(dpm_tissue* h, void* dst, const In* src, size_t n) {
+  if (h->rs == 8) {
+    std::vector<double> v(src, src + n);
+    TRY(upload((double*)dst, v.data(), n, h->stream));
+  } else {
+    std::vector<float> v(n);
+    for (size_t i = 0; i < n; i++) v[i] = (float)src[i];
+    TRY(upload((float*)dst, v.data(), n, h->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(h->stream));
   return DPM_OK;
 }
 
@@ -834,88 +878,380 @@ extern "C" int dpm2d_set_tissue_params(dpm_tissue* h, const float* L, const int*
   if (!h || h->dim != 2) return fail(DPM_EINVAL, "not a 2D tissue");
   const int T = h->T;
   hipStream_t s = h->stream;
-  if (L) TRY(upload(h->tL, L, T, s));
+  if (L) {
+    for (int t = 0; t < T; t++)
+      if (!(L[t] > 0) || !std::isfinite(L[t])) return fail(DPM_EINVAL, "box length must be positive");
+    TRY(upload_real(h, h->tLr, L, T));
+  }
   if (pbc) TRY(upload(h->tpbc, pbc, T, s));
   if (Kre) TRY(upload(h->tKre, Kre, T, s));
   if (Kat) TRY(upload(h->tKat, Kat, T, s));
-  h->mask = mask & DPM2D_ALL;
+  h->mask = mask & DPM2D_ALL_TERMS;
   HIPCHECK(hipStreamSynchronize(s));
   return DPM_OK;
 }
 
-extern "C" int dpm2d_set_positions(dpm_tissue* h, const float* xy, size_t n) {
+template <typename In>
+static int set_positions2(dpm_tissue* h, const In* xy, size_t n) {
   if (!h || h->dim != 2 || !xy) return fail(DPM_EINVAL, "bad arguments");
   if (n != (size_t)h->V) return fail(DPM_EINVAL, "position count does not match the tissue");
-  std::vector<float> a(n), b(n);
+  std::vector<In> a(n), b(n);
   for (size_t i = 0; i < n; i++) {
     a[i] = xy[2 * i];
     b[i] = xy[2 * i + 1];
   }
-  TRY(upload(h->x2, a.data(), n, h->stream));
-  TRY(upload(h->y2, b.data(), n, h->stream));
-  HIPCHECK(hipStreamSynchronize(h->stream));
+  TRY(upload_real(h, h->x2, a.data(), n));
+  TRY(upload_real(h, h->y2, b.data(), n));
   return DPM_OK;
 }
 
-static int download2(dpm_tissue* h, const float* a, const float* b, float* xy, size_t n) {
-  std::vector<float> A(n), B(n);
-  HIPCHECK(hipMemcpyAsync(A.data(), a, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-  HIPCHECK(hipMemcpyAsync(B.data(), b, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-  HIPCHECK(hipStreamSynchronize(h->stream));
-  for (size_t i = 0; i < n; i++) {
-    xy[2 * i] = A[i];
-    xy[2 * i + 1] = B[i];
+template <typename Out>
+static int download2(dpm_tissue* h, const void* a, const void* b, Out* xy, size_t n) {
+  auto get = [&](auto tag) -> int {
+    using R = decltype(tag);
+    std::vector<R> A(n), B(n);
+    HIPCHECK(hipMemcpyAsync(A.data(), a, n * sizeof(R), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipMemcpyAsync(B.data(), b, n * sizeof(R), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < n; i++) {
+      xy[2 * i] = (Out)A[i];
+      xy[2 * i + 1] = (Out)B[i];
+    }
+    return DPM_OK;
+  };
+  return h->rs == 8 ? get(0.0) : get(0.0f);
+}
+
+template <typename Out>
+static int get2(dpm_tissue* h, bool forces, Out* xy, size_t n) {
+  if (!h || h->dim != 2 || !xy) return fail(DPM_EINVAL, "bad arguments");
+  if (n != (size_t)h->V) return fail(DPM_EINVAL, "count does not match the tissue");
+  return forces ? download2(h, h->fx2, h->fy2, xy, n) : download2(h, h->x2, h->y2, xy, n);
+}
+
+extern "C" int dpm2d_set_positions(dpm_tissue* h, const float* xy, size_t n) { return set_positions2(h, xy, n); }
+extern "C" int dpm2d_get_positions(dpm_tissue* h, float* xy, size_t n) { return get2(h, false, xy, n); }
+extern "C" int dpm2d_get_forces(dpm_tissue* h, float* xy, size_t n) { return get2(h, true, xy, n); }
+extern "C" int dpm2d_set_positions_f64(dpm_tissue* h, const double* xy, size_t n) {
+  return set_positions2(h, xy, n);
+}
+extern "C" int dpm2d_get_positions_f64(dpm_tissue* h, double* xy, size_t n) { return get2(h, false, xy, n); }
+extern "C" int dpm2d_get_forces_f64(dpm_tissue* h, double* xy, size_t n) { return get2(h, true, xy, n); }
+
+extern "C" int dpm2d_get_box(dpm_tissue* h, double* L) {
+  if (!h || h->dim != 2 || !L) return fail(DPM_EINVAL, "bad arguments");
+  std::vector<double> out(h->T);
+  if (h->rs == 8) {
+    HIPCHECK(hipMemcpyAsync(out.data(), h->tLr, h->T * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+  } else {
+    std::vector<float> f(h->T);
+    HIPCHECK(hipMemcpyAsync(f.data(), h->tLr, h->T * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    for (int t = 0; t < h->T; t++) out[t] = f[t];
   }
+  std::copy(out.begin(), out.end(), L);
   return DPM_OK;
 }
 
-extern "C" int dpm2d_get_positions(dpm_tissue* h, float* xy, size_t n) {
-  if (!h || h->dim != 2 || !xy) return fail(DPM_EINVAL, "bad arguments");
-  if (n != (size_t)h->V) return fail(DPM_EINVAL, "position count does not match the tissue");
-  return download2(h, h->x2, h->y2, xy, n);
-}
-
-extern "C" int dpm2d_get_forces(dpm_tissue* h, float* xy, size_t n) {
-  if (!h || h->dim != 2 || !xy) return fail(DPM_EINVAL, "bad arguments");
-  if (n != (size_t)h->V) return fail(DPM_EINVAL, "force count does not match the tissue");
-  return download2(h, h->fx2, h->fy2, xy, n);
+template <typename R>
+static Geom2D<R> geom2d(dpm_tissue* h) {
+  Geom2D<R> g{};
+  g.tissue_cell_begin = h->tcb;
+  g.cell_vert_begin = h->cvb;
+  g.Ka = h->Ka;
+  g.Kl = h->Kl;
+  g.Kb = h->Kb;
+  g.a0 = h->a0;
+  g.l0 = h->l0_2;
+  g.r0 = h->r0;
+  g.tL = (const R*)h->tLr;
+  g.tpbc = h->tpbc;
+  g.tKre = h->tKre;
+  g.tKat = h->tKat;
+  g.mask = h->mask;
+  return g;
 }
 
 static Run2D run2d_args(dpm_tissue* h, int nsteps, float dt) {
   Run2D a{};
-  a.x = h->x2;
-  a.y = h->y2;
-  a.fx = h->fx2;
-  a.fy = h->fy2;
-  a.tissue_cell_begin = h->tcb;
-  a.cell_vert_begin = h->cvb;
-  a.Ka = h->Ka;
-  a.Kl = h->Kl;
-  a.Kb = h->Kb;
-  a.a0 = h->a0;
-  a.l0 = h->l0_2;
-  a.r0 = h->r0;
-  a.tL = h->tL;
-  a.tpbc = h->tpbc;
-  a.tKre = h->tKre;
-  a.tKat = h->tKat;
-  a.mask = h->mask;
+  a.g = geom2d<float>(h);
+  a.x = (float*)h->x2;
+  a.y = (float*)h->y2;
+  a.fx = (float*)h->fx2;
+  a.fy = (float*)h->fy2;
   a.nsteps = nsteps;
   a.dt = dt;
   a.counters = h->counters;
   return a;
 }
 
+static FireParams fire_params(const dpm_fire_params* p) {
+  FireParams f{};
+  f.dt0 = p->dt0;
+  f.dt_max = p->dt_max;
+  f.ftol = p->ftol;
+  f.f_inc = p->f_inc;
+  f.f_dec = p->f_dec;
+  f.alpha0 = p->alpha0;
+  f.f_alpha = p->f_alpha;
+  f.max_iter = p->max_iter;
+  f.n_min = p->n_min;
+  return f;
+}
+
+extern "C" void dpm_fire_defaults(dpm_fire_params* p) {
+  if (!p) return;
+  p->dt0 = 0.01;
+  p->dt_max = 0.1;
+  p->ftol = 1e-10;
+  p->f_inc = 1.1;
+  p->f_dec = 0.5;
+  p->alpha0 = 0.1;
+  p->f_alpha = 0.99;
+  p->max_iter = 100000;
+  p->n_min = 5;
+}
+
+extern "C" void dpm_jam_defaults(dpm_jam_params* p) {
+  if (!p) return;
+  p->phi0 = 0.35;
+  p->dphi = 0.02;
+  p->phi_tol = 1e-3;
+  p->u_jam = 1e-16;
+  p->max_stages = 40;
+  dpm_fire_defaults(&p->fire);
+  p->fire.max_iter = 20000;
+}
+
+static int check_fire(const dpm_fire_params* p) {
+  if (!p) return fail(DPM_EINVAL, "null FIRE parameters");
+  if (!(p->dt0 > 0) || !(p->dt_max >= p->dt0) || !(p->ftol >= 0) || !(p->f_inc >= 1) || !(p->f_dec > 0) ||
+      !(p->f_dec < 1) || !(p->alpha0 >= 0) || !(p->alpha0 < 1) || !(p->f_alpha > 0) || !(p->f_alpha <= 1) ||
+      p->max_iter < 0 || p->n_min < 0)
+    return fail(DPM_EINVAL, "invalid FIRE parameters");
+  return DPM_OK;
+}
+
+template <typename Edit>
+static int launch_relax(dpm_tissue* h, int mode, int threads, Edit edit) {
+  auto go = [&](auto tag) -> int {
+    using R = decltype(tag);
+    Relax2D<R> a{};
+    a.g = geom2d<R>(h);
+    a.x = (R*)h->x2;
+    a.y = (R*)h->y2;
+    a.vx = (R*)h->vx2;
+    a.vy = (R*)h->vy2;
+    a.fx = (R*)h->fx2;
+    a.fy = (R*)h->fy2;
+    a.sx = (R*)h->sx2;
+    a.sy = (R*)h->sy2;
+    a.tLout = (R*)h->tLr;
+    a.res = h->res2;
+    a.sum_a0 = h->sum_a0;
+    a.mode = mode;
+    dpm_fire_params fd;
+    dpm_fire_defaults(&fd);
+    a.fire = fire_params(&fd);
+    edit(a);
+    void (*k)(Relax2D<R>) = nullptr;
+    if constexpr (sizeof(R) == 8) k = k2d_relax_f64; else k = k2d_relax_f32;
+    hipLaunchKernelGGL(k, dim3(h->T), dim3(threads > 0 ? threads : h->NTr), h->lds2, h->stream, a);
+    HIPCHECK(hipGetLastError());
+    return DPM_OK;
+  };
+  return h->rs == 8 ? go(0.0) : go(0.0f);
+}
+
+static int ensure_relax_buffers(dpm_tissue* h) {
+  if (h->vx2) return DPM_OK;
+  int rc = h->alloc_real(&h->vx2, h->V);
+  rc |= h->alloc_real(&h->vy2, h->V);
+  rc |= h->alloc_real(&h->sx2, h->V);
+  rc |= h->alloc_real(&h->sy2, h->V);
+  return rc ? DPM_ENOMEM : DPM_OK;
+}
+
+static int read_stats(dpm_tissue* h, dpm_relax_stats* st) {
+  if (st)
+    HIPCHECK(hipMemcpyAsync(st, h->res2, sizeof(double) * h->T * RS_COUNT, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return DPM_OK;
+}
+
 static int euler2d(dpm_tissue* h, int nsteps, float dt) {
   if (nsteps <= 0) return fail(DPM_EINVAL, "nsteps must be positive");
+  if (h->rs == 8) {
+    TRY(ensure_relax_buffers(h));
+    return launch_relax(h, RELAX_EULER, 0, [&](auto& a) {
+      a.nsteps = nsteps;
+      a.dt = dt;
+    });
+  }
   Run2D a = run2d_args(h, nsteps, dt);
   hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds2, h->stream, a);
   HIPCHECK(hipGetLastError());
   return DPM_OK;
 }
 
+extern "C" int dpm2d_set_relax_threads(dpm_tissue* h, int threads) {
+  if (!h || h->dim != 2) return fail(DPM_EINVAL, "not a 2D tissue");
+  if (threads == 0) threads = std::min(1024, std::max(64, ((h->V / std::max(1, h->T)) / 4 + 63) / 64 * 64));
+  if (threads < 64 || threads > 1024 || threads % 64) return fail(DPM_EINVAL, "threads must be 64..1024, a multiple of 64");
+  h->NTr = threads;
+  return DPM_OK;
+}
+
+extern "C" int dpm2d_relax_threads(dpm_tissue* h) { return (h && h->dim == 2) ? h->NTr : 0; }
+
+extern "C" int dpm2d_fire(dpm_tissue* h, const dpm_fire_params* p, dpm_relax_stats* stats) {
+  if (!h || h->dim != 2) return fail(DPM_EINVAL, "not a 2D tissue");
+  TRY(check_fire(p));
+  TRY(ensure_relax_buffers(h));
+  TRY(launch_relax(h, RELAX_FIRE, 0, [&](auto& a) { a.fire = fire_params(p); }));
+  return read_stats(h, stats);
+}
+
+extern "C" int dpm2d_energy(dpm_tissue* h, dpm_relax_stats* stats) {
+  if (!h || h->dim != 2) return fail(DPM_EINVAL, "not a 2D tissue");
+  TRY(ensure_relax_buffers(h));
+  TRY(launch_relax(h, RELAX_ENERGY, 0, [](auto&) {}));
+  return read_stats(h, stats);
+}
+
+static int check_jam(dpm_tissue* h, const dpm_jam_params* p) {
+  if (!h || h->dim != 2) return fail(DPM_EINVAL, "not a 2D tissue");
+  if (!p) return fail(DPM_EINVAL, "null jamming parameters");
+  TRY(check_fire(&p->fire));
+  if (!(p->phi0 > 0) || !(p->phi0 < 1) || !(p->dphi > 0) || !(p->phi_tol > 0) || !(p->u_jam >= 0) ||
+      p->max_stages <= 0)
+    return fail(DPM_EINVAL, "invalid jamming parameters");
+  if (!(h->mask & DPM2D_VREPULSION))
+    return fail(DPM_EINVAL, "jamming classifies by the vertex-repulsion energy: enable DPM2D_VREPULSION");
+  return ensure_relax_buffers(h);
+}
+
+template <typename A>
+static void jam_args(A& a, const dpm_jam_params* p, int start, int stages) {
+  a.fire = fire_params(&p->fire);
+  a.jam.phi0 = p->phi0;
+  a.jam.dphi = p->dphi;
+  a.jam.phi_tol = p->phi_tol;
+  a.jam.u_jam = p->u_jam;
+  a.jam.max_stages = p->max_stages;
+  a.jam_start = start;
+  a.jam_stages = stages;
+}
+
+// The whole jamming protocol is one launch, captured into a hipGraph per
+// handle and parameter set and replayed by later calls with the same
+// parameters.
+extern "C" int dpm2d_jam(dpm_tissue* h, const dpm_jam_params* p, dpm_relax_stats* stats) {
+  TRY(check_jam(h, p));
+  std::vector<unsigned char> key(sizeof(dpm_jam_params) + 2 * sizeof(int));
+  std::memcpy(key.data(), p, sizeof(dpm_jam_params));
+  const int extra[2] = {h->NTr, (int)h->mask};
+  std::memcpy(key.data() + sizeof(dpm_jam_params), extra, sizeof(extra));
+  if (!h->jam_graph || key != h->jam_key) {
+    if (h->jam_graph) {
+      (void)hipGraphExecDestroy(h->jam_graph);
+      h->jam_graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIPCHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = launch_relax(h, RELAX_JAM, 0, [&](auto& a) { jam_args(a, p, 1, p->max_stages); });
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(DPM_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&h->jam_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      h->jam_graph = nullptr;
+      return fail(DPM_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    h->jam_key = key;
+  }
+  HIPCHECK(hipGraphLaunch(h->jam_graph, h->stream));
+  return read_stats(h, stats);
+}
+
+extern "C" int dpm2d_jam_step(dpm_tissue* h, const dpm_jam_params* p, int start, int stages,
+                              dpm_relax_stats* stats) {
+  TRY(check_jam(h, p));
+  if (stages <= 0) return fail(DPM_EINVAL, "stages must be positive");
+  TRY(launch_relax(h, RELAX_JAM, 0, [&](auto& a) { jam_args(a, p, start ? 1 : 0, stages); }));
+  return read_stats(h, stats);
+}
+
+// Workgroup-size autotuner of the relaxation kernel: times `iters` FIRE
+// iterations (ftol 0, so every packing runs all of them) for each candidate
+// size from the current positions, restores the positions after each trial
+// and keeps the fastest.  times_ms (may be NULL) receives the five timings
+// of 64, 128, 256, 512, 1024 threads (-1 where not run).
+extern "C" int dpm2d_autotune(dpm_tissue* h, int iters, int* best, float* times_ms) {
+  if (!h || h->dim != 2) return fail(DPM_EINVAL, "not a 2D tissue");
+  if (iters <= 0) return fail(DPM_EINVAL, "iters must be positive");
+  TRY(ensure_relax_buffers(h));
+  const size_t bytes = (size_t)h->V * h->rs;
+  void *bx = nullptr, *by = nullptr;
+  HIPCHECK(hipMalloc(&bx, bytes));
+  if (hipMalloc(&by, bytes) != hipSuccess) {
+    (void)hipFree(bx);
+    return fail(DPM_ENOMEM, "autotune buffers");
+  }
+  hipStream_t s = h->stream;
+  int rc = DPM_OK;
+  if (hipMemcpyAsync(bx, h->x2, bytes, hipMemcpyDeviceToDevice, s) ||
+      hipMemcpyAsync(by, h->y2, bytes, hipMemcpyDeviceToDevice, s))
+    rc = fail(DPM_EDEVICE, "autotune copy");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) || hipEventCreate(&e1))) rc = fail(DPM_EDEVICE, "autotune events");
+  const int cand[5] = {64, 128, 256, 512, 1024};
+  float bestt = -1.0f;
+  int bestn = h->NTr;
+  dpm_fire_params fp;
+  dpm_fire_defaults(&fp);
+  fp.ftol = 0.0;
+  fp.max_iter = iters;
+  for (int i = 0; i < 5 && !rc; i++) {
+    float ms = -1.0f;
+    (void)hipEventRecord(e0, s);
+    rc = launch_relax(h, RELAX_FIRE, cand[i], [&](auto& a) { a.fire = fire_params(&fp); });
+    (void)hipEventRecord(e1, s);
+    if (!rc && hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    if (times_ms) times_ms[i] = ms;
+    if (!rc && ms > 0 && (bestt < 0 || ms < bestt)) {
+      bestt = ms;
+      bestn = cand[i];
+    }
+    if (hipMemcpyAsync(h->x2, bx, bytes, hipMemcpyDeviceToDevice, s) ||
+        hipMemcpyAsync(h->y2, by, bytes, hipMemcpyDeviceToDevice, s))
+      rc = fail(DPM_EDEVICE, "autotune restore");
+  }
+  (void)hipStreamSynchronize(s);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(bx);
+  (void)hipFree(by);
+  if (rc) return rc;
+  h->NTr = bestn;
+  if (best) *best = bestn;
+  return DPM_OK;
+}
+
 extern "C" int dpm2d_probe(dpm_tissue* h, float* forces, int* overlap, int* attraction) {
   if (!h || h->dim != 2) return fail(DPM_EINVAL, "not a 2D tissue");
+  if (h->rs == 8) {
+    if (overlap || attraction)
+      return fail(DPM_EINVAL, "pair diagnostics are recorded by float tissues only");
+    TRY(dpm2d_energy(h, nullptr));
+    if (forces) TRY(dpm2d_get_forces(h, forces, h->V));
+    return DPM_OK;
+  }
   hipStream_t s = h->stream;
   int *dov = nullptr, *dat = nullptr;
   int rc = dev_alloc(&dov, h->V);

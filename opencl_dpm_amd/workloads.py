@@ -161,23 +161,24 @@ def _relax_disks(r: np.ndarray, L: float, rng, iters: int = 20000) -> np.ndarray
 
 def tissue2d(ncells: int = 16, nv: int = 24, ratio: float = 1.4, phi: float = 0.6, Ka: float = 1.0,
              Kl: float = 1.0, Kb: float = 0.0, Kre: float = 1.0, Kat: float = 0.0, seed: int = 0,
-             nv_range=None) -> Packing2D:
+             nv_range=None, calA: float = 1.0) -> Packing2D:
     """Bidisperse polygons (half r0 = 1, half r0 = sqrt(ratio)) at true
     packing fraction `phi` (the reference's L = sqrt(sum a)/phi0, so phi0 =
-    sqrt(phi)), centres relaxed to no disk overlap."""
+    sqrt(phi)), centres relaxed to no disk overlap; calA is the cells'
+    shape parameter (Cell2D's third argument, cell.cpp:12-33)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     r = np.where(np.arange(ncells) % 2 == 0, 1.0, math.sqrt(ratio)).astype(np.float32)
     if nv_range is None:
         nvs = np.full(ncells, nv, np.int32)
     else:
         nvs = rng.integers(nv_range[0], nv_range[1] + 1, ncells).astype(np.int32)
-    proto = [clDPM.Cell2D(0.0, 0.0, 1.0, int(nvs[c]), float(r[c])) for c in range(ncells)]
+    proto = [clDPM.Cell2D(0.0, 0.0, calA, int(nvs[c]), float(r[c])) for c in range(ncells)]
     T = clDPM.Tissue2D(proto, float(np.float32(math.sqrt(phi))))
     L = T.L
     P = _relax_disks(r.astype(np.float64), L, rng)
     Xs, a0, l0 = [], [], []
     for c in range(ncells):
-        cell = clDPM.Cell2D(float(P[c, 0]), float(P[c, 1]), 1.0, int(nvs[c]), float(r[c]))
+        cell = clDPM.Cell2D(float(P[c, 0]), float(P[c, 1]), calA, int(nvs[c]), float(r[c]))
         Xs.append(np.asarray(cell.Verts, np.float32))
         a0.append(cell.a0)
         l0.append(cell.l0)
@@ -187,13 +188,17 @@ def tissue2d(ncells: int = 16, nv: int = 24, ratio: float = 1.4, phi: float = 0.
                      Kre=Kre, Kat=Kat)
 
 
-def packings2d(n: int = 512, cells=(8, 128), verts=(16, 64), seed: int = 0, phi: float = 0.6):
+def packings2d(n: int = 512, cells=(8, 128), verts=(16, 64), seed: int = 0, phi: float = 0.4,
+               calA: float = 1.02):
     """Batching form: n packings, cell counts U{cells}, vertices per cell
-    U{verts}, bidisperse 1:1.4, Kb alternating 0 / 0.01."""
+    U{verts}, bidisperse 1:1.4, Kb alternating 0 / 0.01, shape parameter
+    calA (1.02: 2 % excess perimeter — with calA = 1 the regular polygon is a
+    quartic minimum of the area + perimeter energy and FIRE converges only
+    algebraically there)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = []
     for k in range(n):
         nc = int(rng.integers(cells[0], cells[1] + 1))
         out.append(tissue2d(nc, 0, 1.4, phi, Kb=0.0 if k % 2 == 0 else 0.01,
-                            seed=int(rng.integers(0, 2 ** 31)), nv_range=verts))
+                            seed=int(rng.integers(0, 2 ** 31)), nv_range=verts, calA=calA))
     return out

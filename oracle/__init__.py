@@ -69,6 +69,77 @@ def run2d(verts, stride, nv, Ka, Kl, Kb, a0, l0, r0, L, Kre, Kat, nsteps, dt, pb
     return V, F
 
 
+def energy2d(verts, stride, nv, Ka, Kl, Kb, a0, l0, L, Kre, pbc=1, mask=102, prec=64):
+    """(total, vertex-repulsion part) of the conservative 2D energy."""
+    dt_, ct, sfx = _t(prec)
+    nc = len(nv)
+    V = np.ascontiguousarray(np.asarray(verts, dt_).reshape(nc * stride, 2))
+    arr = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dt_), (nc,)))
+    ps = [arr(a) for a in (Ka, Kl, Kb, a0, l0)]
+    nvv = np.ascontiguousarray(np.asarray(nv, np.int32))
+    out = np.zeros(2)
+    getattr(_lib("liboracle.so"), "ora2d_energy" + sfx)(
+        ctypes.c_int(nc), ctypes.c_int(stride), _p(nvv), _p(V), *[_p(a) for a in ps], ct(L), ctypes.c_int(pbc),
+        ct(Kre), ctypes.c_uint(mask), _p(out))
+    return float(out[0]), float(out[1])
+
+
+FIRE_DEFAULTS = dict(dt0=0.01, dt_max=0.1, ftol=1e-10, f_inc=1.1, f_dec=0.5, alpha0=0.1, f_alpha=0.99,
+                     max_iter=100000, n_min=5)
+
+
+def fire2d(verts, stride, nv, Ka, Kl, Kb, a0, l0, r0, L, Kre, Kat, vel=None, pbc=1, mask=31, prec=64,
+           u_clear=-1.0, **fire):
+    """FIRE minimisation (ora2d_fire).  Returns positions, velocities and
+    {iterations, converged, fmax, dt, alpha, n_pos, cleared}."""
+    dt_, ct, sfx = _t(prec)
+    p = dict(FIRE_DEFAULTS, **fire)
+    nc = len(nv)
+    V = np.ascontiguousarray(np.asarray(verts, dt_).reshape(nc * stride, 2)).copy()
+    Vel = np.zeros_like(V) if vel is None else np.ascontiguousarray(np.asarray(vel, dt_).reshape(V.shape)).copy()
+    arr = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dt_), (nc,)))
+    ps = [arr(a) for a in (Ka, Kl, Kb, a0, l0, r0)]
+    nvv = np.ascontiguousarray(np.asarray(nv, np.int32))
+    fp = np.array([p["dt0"], p["dt_max"], p["ftol"], p["f_inc"], p["f_dec"], p["alpha0"], p["f_alpha"],
+                   u_clear])
+    ip = np.array([p["max_iter"], p["n_min"]], np.int32)
+    out = np.zeros(7)
+    f = getattr(_lib("liboracle.so"), "ora2d_fire" + sfx)
+    rc = f(ctypes.c_int(nc), ctypes.c_int(stride), _p(nvv), _p(V), _p(Vel), *[_p(a) for a in ps], ct(L),
+           ctypes.c_int(pbc), ct(Kre), ct(Kat), ctypes.c_uint(mask), _p(fp), _p(ip), _p(out))
+    assert rc == 0
+    st = dict(zip(["iterations", "converged", "fmax", "dt", "alpha", "n_pos", "cleared"], out.tolist()))
+    return V, Vel, st
+
+
+JAM_DEFAULTS = dict(phi0=0.35, dphi=0.02, phi_tol=1e-3, u_jam=1e-16, max_stages=40)
+
+
+def jam2d(verts, stride, nv, Ka, Kl, Kb, a0, l0, r0, L, Kre, Kat, pbc=1, mask=102, prec=64, fire=None, **jam):
+    """Jamming protocol (ora2d_jam).  Returns positions, L and
+    {phi_lo, phi_hi, stages, iterations, L, status, unconverged}."""
+    dt_, ct, sfx = _t(prec)
+    p = dict(FIRE_DEFAULTS, max_iter=20000)
+    p.update(fire or {})
+    j = dict(JAM_DEFAULTS, **jam)
+    nc = len(nv)
+    V = np.ascontiguousarray(np.asarray(verts, dt_).reshape(nc * stride, 2)).copy()
+    arr = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dt_), (nc,)))
+    ps = [arr(a) for a in (Ka, Kl, Kb, a0, l0, r0)]
+    nvv = np.ascontiguousarray(np.asarray(nv, np.int32))
+    fp = np.array([p["dt0"], p["dt_max"], p["ftol"], p["f_inc"], p["f_dec"], p["alpha0"], p["f_alpha"]])
+    ip = np.array([p["max_iter"], p["n_min"], j["max_stages"]], np.int32)
+    jp = np.array([j["phi0"], j["dphi"], j["phi_tol"], j["u_jam"]])
+    Lb = np.array([L], dt_)
+    out = np.zeros(7)
+    f = getattr(_lib("liboracle.so"), "ora2d_jam" + sfx)
+    rc = f(ctypes.c_int(nc), ctypes.c_int(stride), _p(nvv), _p(V), *[_p(a) for a in ps], _p(Lb),
+           ctypes.c_int(pbc), ct(Kre), ct(Kat), ctypes.c_uint(mask), _p(fp), _p(ip), _p(jp), _p(out))
+    assert rc == 0
+    st = dict(zip(["phi_lo", "phi_hi", "stages", "iterations", "L", "status", "unconverged"], out.tolist()))
+    return V, float(Lb[0]), st
+
+
 def adjacency(faces, nv, maxdeg=8):
     faces = np.asarray(faces, np.int64).reshape(-1, 3)
     nb = [set() for _ in range(nv)]

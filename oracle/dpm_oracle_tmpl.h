@@ -141,6 +141,28 @@ int FN(ora2d_forces)(int ncells, int stride, const int* NV, const R* V,
         F[index * 2 + 1] += (Ka[ci] / SQRT(a0[ci])) * 0.5 * areaStrain * (X(im1) - X(ip1));
       }
     }
+    /* Gradient area force (no reference counterpart): -dE/dx of
+     * E = Ka sqrt(a0)/2 (|A|/a0 - 1)^2 for either orientation; equals the
+     * reference's x component on counter-clockwise cells and flips its y
+     * component. */
+    if (mask & 32u) {
+      double As = 0.0;
+      for (int vj = 0; vj < n; vj++) {
+        int idx = base + vj;
+        int im = (vj == 0) ? (base + n - 1) : (idx - 1);
+        As += 0.5 * ((double)(X(im) + X(idx)) * (double)(Y(im) - Y(idx)));
+      }
+      const R sgn = As < 0.0 ? (R)-1 : (R)1;
+      const R s = (R)(fabs(As) / (double)a0[ci] - 1.0);
+      const R c = Ka[ci] / SQRT(a0[ci]);
+      for (int vi = 0; vi < n; vi++) {
+        int index = base + vi;
+        int im1 = (vi == 0) ? (base + n - 1) : (index - 1);
+        int ip1 = (vi == n - 1) ? base : (index + 1);
+        F[index * 2 + 0] += -c * s * sgn * (R)0.5 * (Y(im1) - Y(ip1));
+        F[index * 2 + 1] += -c * s * sgn * (R)0.5 * (X(ip1) - X(im1));
+      }
+    }
     for (int vi = 0; vi < n; vi++) {
       int index = base + vi;
       int im1 = (vi == 0) ? (base + n - 1) : (index - 1);
@@ -190,6 +212,29 @@ int FN(ora2d_forces)(int ncells, int stride, const int* NV, const R* V,
               F[index * 2 + 0] += ftmp * ux;
               F[index * 2 + 1] += ftmp * uy;
               if (attr_count) attr_count[index]++;
+            }
+          }
+        }
+      }
+      /* Vertex soft-disk repulsion (no reference counterpart): vertices of
+       * different cells closer than sigma = (l0_i + l0_j)/2 repel with
+       * U = Kre/2 (1 - d/sigma)^2, minimum image as the attraction. */
+      if (mask & 64u) {
+        for (int cj = 0; cj < ncells; cj++) {
+          if (cj == ci) continue;
+          const R sig = (R)0.5 * (l0[ci] + l0[cj]);
+          for (int vj = 0; vj < NV[cj]; vj++) {
+            int j = cj * stride + vj;
+            R rx = X(j) - X(index), ry = Y(j) - Y(index);
+            if (PBC) {
+              rx -= L * ROUND(rx / L);
+              ry -= L * ROUND(ry / L);
+            }
+            R dist = SQRT(FN(dot2)(rx, ry, rx, ry));
+            if (dist < sig && dist > 0) {
+              R ftmp = Kre / sig * (1 - dist / sig) / dist;
+              F[index * 2 + 0] -= ftmp * rx;
+              F[index * 2 + 1] -= ftmp * ry;
             }
           }
         }
@@ -249,6 +294,228 @@ int FN(ora2d_run)(int ncells, int stride, const int* NV, R* V, const R* Ka,
   }
   if (Fout) memcpy(Fout, F, sizeof(R) * (size_t)ncells * stride * 2);
   free(F);
+  return 0;
+}
+
+/* Energy of the conservative 2D terms (no reference counterpart), in
+ * double: area (mask 1 or 32) Ka sqrt(a0)/2 (|A|/a0 - 1)^2, perimeter (2)
+ * sum_e Kl sqrt(a0/l0) l0/2 (l/l0 - 1)^2, bending (4) Kb/2 sum |s_i|^2,
+ * vertex repulsion (64) sum over pairs Kre/2 (1 - d/sigma)^2.  out[0] = total,
+ * out[1] = the vertex-repulsion part. */
+void FN(ora2d_energy)(int ncells, int stride, const int* NV, const R* V,
+                      const R* Ka, const R* Kl, const R* Kb, const R* a0,
+                      const R* l0, R L, int PBC, R Kre, unsigned mask,
+                      double* out) {
+  double U = 0.0, Ui = 0.0;
+#define X(k) ((double)V[(k) * 2 + 0])
+#define Y(k) ((double)V[(k) * 2 + 1])
+  for (int ci = 0; ci < ncells; ci++) {
+    const int n = NV[ci], base = ci * stride;
+    if (mask & 33u) {
+      double As = 0.0;
+      for (int vj = 0; vj < n; vj++) {
+        int idx = base + vj, im = (vj == 0) ? (base + n - 1) : (idx - 1);
+        As += 0.5 * ((X(im) + X(idx)) * (Y(im) - Y(idx)));
+      }
+      const double s = fabs(As) / (double)a0[ci] - 1.0;
+      U += 0.5 * (double)Ka[ci] * sqrt((double)a0[ci]) * s * s;
+    }
+    for (int vi = 0; vi < n; vi++) {
+      const int k = base + vi;
+      const int kp = (vi == n - 1) ? base : k + 1, km = (vi == 0) ? base + n - 1 : k - 1;
+      if (mask & 2u) {
+        const double dx = X(kp) - X(k), dy = Y(kp) - Y(k);
+        const double e = sqrt(dx * dx + dy * dy) / (double)l0[ci] - 1.0;
+        U += 0.5 * (double)Kl[ci] * sqrt((double)a0[ci] / (double)l0[ci]) * (double)l0[ci] * e * e;
+      }
+      if (mask & 4u) {
+        const double sx = (X(kp) - X(k)) - (X(k) - X(km)), sy = (Y(kp) - Y(k)) - (Y(k) - Y(km));
+        U += 0.5 * (double)Kb[ci] * (sx * sx + sy * sy);
+      }
+      if (mask & 64u)
+        for (int cj = ci + 1; cj < ncells; cj++) {
+          const double sig = 0.5 * ((double)l0[ci] + (double)l0[cj]);
+          for (int vj = 0; vj < NV[cj]; vj++) {
+            const int j = cj * stride + vj;
+            double rx = X(j) - X(k), ry = Y(j) - Y(k);
+            if (PBC) {
+              rx -= (double)L * round(rx / (double)L);
+              ry -= (double)L * round(ry / (double)L);
+            }
+            const double d = sqrt(rx * rx + ry * ry);
+            if (d < sig) Ui += 0.5 * (double)Kre * (1.0 - d / sig) * (1.0 - d / sig);
+          }
+        }
+    }
+  }
+#undef X
+#undef Y
+  out[0] = U + Ui;
+  out[1] = Ui;
+}
+
+/* FIRE minimisation of one 2D tissue with the reference force law, no
+ * reference counterpart (the reference only has the Euler step).  Bitzek,
+ * Koskinen, Gaehler, Moseler & Gumbsch, PRL 97, 170201 (2006), with the
+ * semi-implicit Euler MD step (unit masses), iteration by iteration:
+ *   F = F(x); with u_clear >= 0 stop ("cleared") if the vertex-repulsion
+ *   energy per cell is at most u_clear; stop if max_v |F_v| < ftol;
+ *   P = sum F.v, |F|^2, |v|^2 (sums in double);
+ *   P > 0: v = (1 - a) v + a |v| F/|F|; if n_pos > n_min: dt = min(dt f_inc,
+ *          dt_max), a = a f_a; n_pos++;
+ *   P < 0: v = 0, dt = dt f_dec, a = a0, n_pos = 0;
+ *   (P = 0 happens only with v = 0 or F = 0, e.g. at the start, and changes
+ *    nothing;)
+ *   v += F dt; x += v dt.
+ * V: padded positions (updated), Vel: padded velocities (in/out).
+ * fp = {dt0, dt_max, ftol, f_inc, f_dec, alpha0, f_alpha, u_clear}; ip =
+ * {max_iter, n_min}.  out = {iterations, converged, fmax, dt, alpha, n_pos,
+ * cleared}. */
+int FN(ora2d_fire)(int ncells, int stride, const int* NV, R* V, R* Vel,
+                   const R* Ka, const R* Kl, const R* Kb, const R* a0,
+                   const R* l0, const R* r0, R L, int PBC, R Kre, R Kat,
+                   unsigned mask, const double* fp, const int* ip, double* out) {
+  const size_t n2 = (size_t)ncells * stride * 2;
+  R* F = (R*)calloc(n2, sizeof(R));
+  if (!F) return -1;
+  double dt = fp[0], alpha = fp[5], fmax = 0.0;
+  int npos = 0, it = 0, conv = 0, cleared = 0;
+  for (it = 0; it < ip[0]; it++) {
+    FN(ora2d_forces)(ncells, stride, NV, V, Ka, Kl, Kb, a0, l0, r0, L, PBC, Kre,
+                     Kat, mask, F, NULL, NULL);
+    double P = 0.0, F2 = 0.0, v2 = 0.0;
+    fmax = 0.0;
+    for (int ci = 0; ci < ncells; ci++)
+      for (int vi = 0; vi < NV[ci]; vi++) {
+        const size_t k = ((size_t)ci * stride + vi) * 2;
+        const double fx = F[k], fy = F[k + 1], vx = Vel[k], vy = Vel[k + 1];
+        const double fn = sqrt(fx * fx + fy * fy);
+        if (fn > fmax) fmax = fn;
+        P += fx * vx + fy * vy;
+        F2 += fx * fx + fy * fy;
+        v2 += vx * vx + vy * vy;
+      }
+    if (fp[7] >= 0.0) {
+      double eo[2];
+      FN(ora2d_energy)(ncells, stride, NV, V, Ka, Kl, Kb, a0, l0, L, PBC, Kre, 64u, eo);
+      if (eo[1] / ncells <= fp[7]) { cleared = 1; break; }
+    }
+    if (fmax < fp[2]) { conv = 1; break; }
+    double mix_v = 1.0, mix_f = 0.0;
+    int zero_v = 0;
+    if (P > 0.0) {
+      mix_v = 1.0 - alpha;
+      mix_f = alpha * sqrt(v2 / F2);
+      if (npos > ip[1]) {
+        dt = fmin(dt * fp[3], fp[1]);
+        alpha *= fp[6];
+      }
+      npos++;
+    } else if (P < 0.0) {
+      zero_v = 1;
+      dt *= fp[4];
+      alpha = fp[5];
+      npos = 0;
+    }
+    for (int ci = 0; ci < ncells; ci++)
+      for (int vi = 0; vi < NV[ci]; vi++) {
+        const size_t k = ((size_t)ci * stride + vi) * 2;
+        for (int c = 0; c < 2; c++) {
+          R v = zero_v ? (R)0 : (R)(mix_v * (double)Vel[k + c] + mix_f * (double)F[k + c]);
+          v = (R)((double)v + (double)F[k + c] * dt);
+          Vel[k + c] = v;
+          V[k + c] = (R)((double)V[k + c] + (double)v * dt);
+        }
+      }
+  }
+  out[0] = it; out[1] = conv; out[2] = fmax; out[3] = dt; out[4] = alpha; out[5] = npos;
+  out[6] = cleared;
+  free(F);
+  return 0;
+}
+
+/* Jamming protocol (no reference counterpart): O'Hern, Silbert, Liu &
+ * Nagel, PRE 68, 011306 (2003), as compression and bisection on
+ * phi = sum(a0)/L^2.  Cells move rigidly so that their centres of mass
+ * (double mean of the vertices) scale with the box; at each phi a FIRE
+ * minimisation (ora2d_fire, velocities from rest) that ends early as
+ * "cleared" once the vertex-repulsion energy per cell is at most u_jam;
+ * jammed iff it was not cleared (converged to ftol or stopped at max_iter
+ * with energy above u_jam).  phi grows by dphi from
+ * phi0 until the first jammed state, then bisects between the largest
+ * unjammed and the smallest jammed phi, restarting from the saved unjammed
+ * state, until phi_hi - phi_lo < phi_tol or max_stages minimisations; ends
+ * on the largest unjammed state.
+ * jp = {phi0, dphi, phi_tol, u_jam}; ip = {max_iter, n_min, max_stages};
+ * out = {phi_lo, phi_hi, stages, iterations, L, status, unconverged};
+ * status 0 bracket within phi_tol, 1 phi0 jammed, 3 stopped at max_stages. */
+static void FN(rescale2d)(int ncells, int stride, const int* NV, R* V, R* L, R Lnew) {
+  const R s = Lnew / *L - (R)1;
+  for (int ci = 0; ci < ncells; ci++) {
+    double sx = 0.0, sy = 0.0;
+    for (int vi = 0; vi < NV[ci]; vi++) {
+      sx += (double)V[((size_t)ci * stride + vi) * 2];
+      sy += (double)V[((size_t)ci * stride + vi) * 2 + 1];
+    }
+    const R cx = (R)(sx / NV[ci]), cy = (R)(sy / NV[ci]);
+    for (int vi = 0; vi < NV[ci]; vi++) {
+      V[((size_t)ci * stride + vi) * 2] += s * cx;
+      V[((size_t)ci * stride + vi) * 2 + 1] += s * cy;
+    }
+  }
+  *L = Lnew;
+}
+
+int FN(ora2d_jam)(int ncells, int stride, const int* NV, R* V, const R* Ka,
+                  const R* Kl, const R* Kb, const R* a0, const R* l0,
+                  const R* r0, R* Lio, int PBC, R Kre, R Kat, unsigned mask,
+                  const double* fp, const int* ip, const double* jp, double* out) {
+  const size_t n2 = (size_t)ncells * stride * 2;
+  R* Vel = (R*)calloc(n2, sizeof(R));
+  R* Save = (R*)malloc(n2 * sizeof(R));
+  if (!Vel || !Save) return -1;
+  double A0 = 0.0;
+  for (int c = 0; c < ncells; c++) A0 += (double)a0[c];
+  R L = *Lio;
+  FN(rescale2d)(ncells, stride, NV, V, &L, (R)sqrt(A0 / jp[0]));
+  double phi = jp[0], phi_lo = 0.0, phi_hi = 0.0, iters = 0.0, unconv = 0.0, status = 0.0;
+  int stage = 0;
+  const int fip[2] = {ip[0], ip[1]};
+  double ffp[8];
+  for (int k = 0; k < 7; k++) ffp[k] = fp[k];
+  ffp[7] = jp[3];
+  while (stage < ip[2]) {
+    double fo[7];
+    for (size_t k = 0; k < n2; k++) Vel[k] = 0;
+    FN(ora2d_fire)(ncells, stride, NV, V, Vel, Ka, Kl, Kb, a0, l0, r0, L, PBC, Kre, Kat, mask, ffp, fip, fo);
+    iters += fo[0];
+    unconv += fo[1] == 0.0 && fo[6] == 0.0;
+    stage++;
+    const int jammed = fo[6] == 0.0;
+    if (!jammed) {
+      phi_lo = phi;
+      memcpy(Save, V, n2 * sizeof(R));
+    } else {
+      phi_hi = phi;
+      if (phi_lo == 0.0) { status = 1; break; }
+      memcpy(V, Save, n2 * sizeof(R));
+      L = (R)sqrt(A0 / phi_lo);
+    }
+    if (phi_hi > 0.0 && phi_hi - phi_lo < jp[2]) break;
+    const double next = phi_hi > 0.0 ? 0.5 * (phi_lo + phi_hi) : phi + jp[1];
+    FN(rescale2d)(ncells, stride, NV, V, &L, (R)sqrt(A0 / next));
+    phi = next;
+  }
+  if (status == 0.0 && !(phi_hi > 0.0 && phi_hi - phi_lo < jp[2])) status = 3;  /* stage limit */
+  if (phi_lo > 0.0 && phi != phi_lo) {
+    memcpy(V, Save, n2 * sizeof(R));
+    L = (R)sqrt(A0 / phi_lo);
+  }
+  *Lio = L;
+  out[0] = phi_lo; out[1] = phi_hi; out[2] = stage; out[3] = iters; out[4] = (double)L;
+  out[5] = status; out[6] = unconv;
+  free(Vel);
+  free(Save);
   return 0;
 }
 

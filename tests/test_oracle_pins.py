@@ -105,6 +105,102 @@ def test_area_strain_zero_on_fresh_polygon():
     assert np.abs(F).max() < 2e-6
 
 
+# ------------------------------------------ conservative 2D terms, FIRE, jamming
+
+@pytest.mark.parametrize("orient", [1, -1])
+def test_gradient_area_force_is_gradient_either_orientation(poly, orient):
+    P = poly[::orient].copy()
+    Ka, a0 = 1.3, 2.5
+    E = lambda Q: Ka * math.sqrt(a0) * 0.5 * (shoelace(Q) / a0 - 1.0) ** 2
+    np.testing.assert_allclose(f2d(P, 32, Ka=Ka, a0=a0), -fd_grad(E, P), rtol=1e-6, atol=1e-9)
+    if orient == 1:  # counter-clockwise: the reference's x, the flipped y
+        Fr, Fg = f2d(P, 1, Ka=Ka, a0=a0), f2d(P, 32, Ka=Ka, a0=a0)
+        np.testing.assert_allclose(Fg[:, 0], Fr[:, 0], rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(Fg[:, 1], -Fr[:, 1], rtol=1e-12, atol=1e-15)
+
+
+def overlapping_pair(offset=1.9, L=1000.0):
+    A = regular_polygon(16, 1.0, 0.0, 0.0)
+    B = regular_polygon(16, 1.0, offset, 0.05)
+    P = np.concatenate([A, B])
+    return P, 16
+
+
+def vrep_energy_np(P, stride, nv, l0, Kre, L):
+    # independent restatement: pairs of vertices of different cells
+    U = 0.0
+    off = [0, stride]
+    for ci in range(2):
+        for cj in range(ci + 1, 2):
+            sig = 0.5 * (l0[ci] + l0[cj])
+            A = P[off[ci]:off[ci] + nv[ci]]
+            B = P[off[cj]:off[cj] + nv[cj]]
+            d = B[None] - A[:, None]
+            d -= L * np.round(d / L)
+            r = np.sqrt((d ** 2).sum(-1))
+            U += np.sum(np.where(r < sig, Kre / 2 * (1 - r / sig) ** 2, 0.0))
+    return U
+
+
+@pytest.mark.parametrize("L", [1000.0, 3.5])
+def test_vertex_repulsion_is_gradient_and_pairwise(L):
+    P, stride = overlapping_pair(1.9 if L > 100 else 3.5 - 1.6)
+    nv, l0, Kre = [16, 16], [0.5, 0.3], 0.8
+    F, _, _ = oracle.forces2d(P, stride, nv, 1, 1, 0, 3, l0, 1, L, Kre, 0.0, mask=64, prec=64)
+    E = lambda Q: vrep_energy_np(Q, stride, nv, l0, Kre, L)
+    assert E(P) > 0
+    np.testing.assert_allclose(F, -fd_grad(E, P), rtol=1e-6, atol=1e-9)
+    assert np.abs(F.sum(0)).max() < 1e-12  # equal and opposite pair forces
+    e, ei = oracle.energy2d(P, stride, nv, 1, 1, 0, 3, l0, L, Kre, mask=64)
+    assert e == ei and abs(ei - E(P)) < 1e-12 * E(P)
+
+
+def test_energy2d_is_potential_of_conservative_forces():
+    P, stride = overlapping_pair(1.85)
+    rng = np.random.Generator(np.random.PCG64(4))
+    P = P + rng.normal(0, 0.02, P.shape)
+    nv, Ka, Kl, Kb, a0, l0, Kre = [16, 16], [1.3, 0.9], [0.7, 1.1], [0.05, 0.02], [2.9, 3.3], [0.4, 0.45], 0.8
+    F, _, _ = oracle.forces2d(P, stride, nv, Ka, Kl, Kb, a0, l0, 1, 1000.0, Kre, 0.0, mask=102, prec=64)
+    E = lambda Q: oracle.energy2d(Q, stride, nv, Ka, Kl, Kb, a0, l0, 1000.0, Kre, mask=102)[0]
+    np.testing.assert_allclose(F, -fd_grad(E, P), rtol=1e-6, atol=1e-9)
+
+
+def test_fire_minimises_to_tolerance_and_lowers_energy():
+    P, stride = overlapping_pair(1.8)
+    nv, Ka, Kl, Kb, a0, l0 = [16, 16], 1.0, 1.0, 0.01, [3.06, 3.06], [0.39, 0.39]
+    e0 = oracle.energy2d(P, stride, nv, Ka, Kl, Kb, a0, l0, 1000.0, 1.0)[0]
+    V, Vel, st = oracle.fire2d(P, stride, nv, Ka, Kl, Kb, a0, l0, 1, 1000.0, 1.0, 0.0, mask=102, max_iter=50000)
+    assert st["converged"] == 1 and st["fmax"] < 1e-10
+    F, _, _ = oracle.forces2d(V, stride, nv, Ka, Kl, Kb, a0, l0, 1, 1000.0, 1.0, 0.0, mask=102, prec=64)
+    assert np.abs(F).max() < 1e-10
+    assert oracle.energy2d(V, stride, nv, Ka, Kl, Kb, a0, l0, 1000.0, 1.0)[0] < e0
+
+
+def test_fire_restart_rule():
+    # with ftol 0 and one iteration from rest: P = 0 (v = 0) changes neither
+    # dt nor alpha, and the step is v = F dt0, x += v dt0
+    P, stride = overlapping_pair(1.8)
+    nv = [16, 16]
+    F, _, _ = oracle.forces2d(P, stride, nv, 1, 1, 0, 3.06, 0.39, 1, 1000.0, 1.0, 0.0, mask=102, prec=64)
+    V, Vel, st = oracle.fire2d(P, stride, nv, 1, 1, 0, 3.06, 0.39, 1, 1000.0, 1.0, 0.0, mask=102, max_iter=1,
+                               ftol=0.0)
+    assert st["dt"] == 0.01 and st["alpha"] == 0.1 and st["n_pos"] == 0
+    np.testing.assert_allclose(Vel, F * 0.01, rtol=1e-15)
+    np.testing.assert_allclose(V, P + F * 0.01 * 0.01, rtol=1e-14)
+
+
+def test_jamming_brackets_and_ends_unjammed():
+    m = W.tissue2d(4, 16, 1.4, 0.4, seed=11, calA=1.02)
+    P, stride = m.padded()
+    V, L, st = oracle.jam2d(P, stride, m.nv, m.Ka, m.Kl, m.Kb, m.a0, m.l0, m.r0, m.L, m.Kre, 0.0,
+                            fire=dict(max_iter=20000), phi0=0.4, dphi=0.1, phi_tol=0.02)
+    assert st["status"] == 0 and 0.4 <= st["phi_lo"] < st["phi_hi"] <= st["phi_lo"] + 0.02 + 1e-12
+    A0 = float(np.sum(m.a0.astype(np.float64)))
+    assert abs(A0 / L ** 2 - st["phi_lo"]) < 1e-12
+    e, ei = oracle.energy2d(V, stride, m.nv, m.Ka, m.Kl, m.Kb, m.a0, m.l0, L, m.Kre)
+    assert ei / 4 <= 1e-16  # the returned state is the unjammed one
+
+
 def two_cells_2d(offset):
     A = regular_polygon(16, 1.0, 0.0, 0.0)
     B = regular_polygon(16, 1.0, offset, 0.1)
