@@ -159,6 +159,23 @@ int dpm3d_get_forces_f64(dpm_tissue* h, double* xyz, size_t n);
  * it is the drift diagnostic of the precision study.  energy[ncells]. */
 int dpm3d_energy(dpm_tissue* h, double* energy);
 
+/* Differentiation form (no reference counterpart).  dpm3d_trajectory runs
+ * nsteps Euler steps (as dpm_euler) and keeps the positions every `every`
+ * steps (nsteps a multiple of every) on the device.  dpm3d_adjoint then
+ * computes, by the discrete adjoint of x_{n+1} = x_n + dt F(x_n; theta)
+ * with the segments between checkpoints recomputed,
+ *   dtheta[c*4 + k] = dL/dtheta_k of cell c, theta = (Kv, Ka, Kb, Ks),
+ *   lambda0 = dL/dx_0 (AoS, may be NULL),
+ * from lambda_final = dL/dx_N (AoS).  Covers the volume, surface, bending,
+ * substrate (with its stiffness gradient) and contact terms; tissues with
+ * DPM3D_ADHESION are rejected.  The cut-off decisions of the force law
+ * (facing-down faces, substrate heights, contact pairs) are held fixed, as
+ * their derivatives vanish almost everywhere.  After the call the positions
+ * are those at the end of the trajectory. */
+int dpm3d_trajectory(dpm_tissue* h, int nsteps, float dt, int every);
+int dpm3d_adjoint(dpm_tissue* h, const float* lambda_final, double* dtheta, float* lambda0);
+int dpm3d_adjoint_f64(dpm_tissue* h, const double* lambda_final, double* dtheta, double* lambda0);
+
 /* ------------------------------------------------------------------ 2D ---
  * A batch of tissues of polygon cells with per-cell vertex counts
  * (Tissue2D.cpp:117-140).  Vertices are packed cell after cell (no padding):

@@ -34,7 +34,7 @@ EXPORTS = [
     "dpm3d_create", "dpm3d_set_positions", "dpm3d_get_positions", "dpm3d_get_forces",
     "dpm3d_set_cell_params", "dpm3d_set_tissue_params", "dpm3d_cell_stats", "dpm3d_probe",
     "dpm3d_create_f64", "dpm3d_set_positions_f64", "dpm3d_get_positions_f64",
-    "dpm3d_get_forces_f64", "dpm3d_energy",
+    "dpm3d_get_forces_f64", "dpm3d_energy", "dpm3d_trajectory", "dpm3d_adjoint", "dpm3d_adjoint_f64",
     "dpm2d_create", "dpm2d_set_positions", "dpm2d_get_positions", "dpm2d_get_forces",
     "dpm2d_set_cell_params", "dpm2d_set_tissue_params", "dpm2d_probe",
     "dpm2d_create_f64", "dpm2d_set_positions_f64", "dpm2d_get_positions_f64", "dpm2d_get_forces_f64",
@@ -78,6 +78,9 @@ def load() -> ctypes.CDLL:
     lib.dpm3d_create.argtypes = [vp, vp]
     lib.dpm3d_create_f64.argtypes = [vp, vp]
     lib.dpm3d_energy.argtypes = [vp, vp]
+    lib.dpm3d_trajectory.argtypes = [vp, i, fp, i]
+    lib.dpm3d_adjoint.argtypes = [vp, vp, vp, vp]
+    lib.dpm3d_adjoint_f64.argtypes = [vp, vp, vp, vp]
     lib.dpm2d_create.argtypes = [vp, vp]
     lib.dpm2d_create_f64.argtypes = [vp, vp]
     lib.dpm2d_get_box.argtypes = [vp, vp]
@@ -281,6 +284,24 @@ class Tissues3D(_Handle):
 
     def forces(self, dtype=None) -> np.ndarray:
         return self._get(load().dpm3d_get_forces, load().dpm3d_get_forces_f64, dtype)
+
+    def trajectory(self, nsteps: int, dt: float, every: int = 20) -> None:
+        """nsteps Euler steps keeping checkpoints every `every` steps."""
+        _check(load().dpm3d_trajectory(self._h, int(nsteps), float(dt), int(every)))
+
+    def adjoint(self, lambda_final):
+        """dL/d(Kv, Ka, Kb, Ks) per cell [C, 4] and dL/dx_0 [nverts, 3] of the
+        recorded trajectory, given dL/dx_N."""
+        dth = np.zeros((self.C, 4), np.float64)
+        if self.precision == 64:
+            lf = np.ascontiguousarray(np.asarray(lambda_final, np.float64).reshape(-1, 3))
+            l0 = np.empty_like(lf)
+            _check(load().dpm3d_adjoint_f64(self._h, _p(lf), _p(dth), _p(l0)))
+        else:
+            lf = _f32(lambda_final).reshape(-1, 3)
+            l0 = np.empty_like(lf)
+            _check(load().dpm3d_adjoint(self._h, _p(lf), _p(dth), _p(l0)))
+        return dth, l0
 
     def energy(self) -> np.ndarray:
         """Per-cell elastic energy (volume + edge springs + bending, per mask)."""

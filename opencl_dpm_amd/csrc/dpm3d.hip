@@ -33,6 +33,7 @@ using namespace dpm;
 
 #define DPM3D_MAXDEG 8
 #define DPM3D_MAXCLUS 64
+#define DPM3D_MAXPART 4
 
 enum : unsigned {
   M_VOLUME = 1u,
@@ -107,6 +108,13 @@ struct Step3D {
   int* p_adh;
   int ncmax;
   double* energy;  // k3d_energy output, per cell
+  R* w_out;        // per vertex: sum of |w| over its contact partners (nullptr: not recorded)
+  // contact partners per vertex for the adjoint (nullptr: not recorded):
+  // up to DPM3D_MAXPART of them, cell (-1 pad), winding number, shift
+  int* rec_cell;      // [nverts * MAXPART]
+  R* rec_w;           // [nverts * MAXPART]
+  R* rec_shift;       // [nverts * MAXPART * 3]
+  int* rec_overflow;  // set when a vertex has more partners
 };
 
 // Target edge length l0 = sqrt(4 a0)/sqrt(3) of a cell (Tissue3D.cpp:177),
@@ -521,6 +529,8 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
   const bool do_contact = (mask & M_CONTACT) && Kre != R(0);
   const bool do_adh = (mask & M_ADHESION) && Kat != R(0);
   unsigned long long n_wind = 0, n_contact = 0, n_cand = 0;
+  R wsum = R(0);  // the contact force is wsum * Kre/2 * normalize(C_i - p)
+  int npart = 0;
   if (do_contact || do_adh) {  // block-uniform
     const R L0i = fmax(l0i, R(1e-12f));
     Q4<R>* ut = l.utab + (size_t)wave * nv;
@@ -604,6 +614,20 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
               const R mag = fabs(wv) * R(0.5f) * Kre;
               F = vadd(F, vscale(dir, mag));
               n_contact++;
+              wsum += fabs(wv);
+              if (a.rec_cell) {
+                if (npart < DPM3D_MAXPART) {
+                  const size_t r = (base + tid) * DPM3D_MAXPART + npart;
+                  a.rec_cell[r] = cj2;
+                  a.rec_w[r] = wv;
+                  a.rec_shift[3 * r] = sh.x;
+                  a.rec_shift[3 * r + 1] = sh.y;
+                  a.rec_shift[3 * r + 2] = sh.z;
+                } else {
+                  atomicOr(a.rec_overflow, 1);
+                }
+                npart++;
+              }
               if (a.p_contact) a.p_contact[pidx] = 1;
             }
           }
@@ -687,6 +711,9 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
   }
 
   // ------------------------------------------------------- Euler and stats
+  if (own && a.w_out) a.w_out[base + tid] = wsum;
+  if (own && a.rec_cell)
+    for (int q = npart; q < DPM3D_MAXPART; q++) a.rec_cell[(base + tid) * DPM3D_MAXPART + q] = -1;
   if (own && (a.flags & F_WRITE_FORCES)) {
     a.fx[base + tid] = F.x;
     a.fy[base + tid] = F.y;
@@ -798,9 +825,413 @@ __device__ void k3d_energy_body(const Step3D<R>& a) {
   }
 }
 
+// ------------------------------------------------------------------ adjoint
+//
+// Vector-Jacobian product of one Euler step x' = x + dt F(x; theta), for the
+// discrete adjoint of a trajectory:
+//   lambda_n = lambda_{n+1} + dt J_x(x_n)^T lambda_{n+1},
+//   dL/dtheta += dt J_theta(x_n)^T lambda_{n+1},
+// theta = per-cell (Kv, Ka, Kb, Ks).  Every retained term is cell-local:
+//   volume    F = -Kv s grad V, s = V/v0 - 1: J^T l = -Kv/v0 g (g.l)
+//             - Kv s grad(g.l), g = grad V, with grad(g.l) per face
+//             (l_b x x_c + x_b x l_c)/6 at corner a (cyclic), CoM-relative;
+//   surface   each face edge a->b is a spring with F_a = sc d u, F_b = -F_a:
+//             J^T l at a = -sc M (l_a - l_b), at b the opposite,
+//             M = u u^T / l0 + (d/len)(I - u u^T);
+//   bending   F = -Kb L L x is linear: J^T l = -Kb L L l;
+//   substrate per facing-down face corner (the facing decision and the
+//             height cut-offs are piecewise constant): derivatives of
+//             ks(x) = Ks (1 + g x_wrapped / L), of |z| and of
+//             normalize(p_xy - C) with respect to p and to the centre of
+//             mass C (which moves with every vertex by 1/nv);
+//   contact   F = sum_j |w_j| Kre/2 normalize(C - p) over the partners j
+//             recorded by the forward step.  The winding number of a closed
+//             surface would be piecewise constant, but the reference skips
+//             faces with denominator < 1e-8 (those subtending more than a
+//             hemisphere, near p), so w = (4 pi)^-1 sum over kept faces and
+//             dw = -(4 pi)^-1 sum over skipped faces of dOmega_f, with
+//             Omega = 2 atan2(N, D): dOmega = 2 (D dN - N dD)/(N^2 + D^2)
+//             through the normalised directions; the skipped faces belong to
+//             the partner cell, whose share goes to cx/cy/cz atomically;
+// vertex adhesion couples cells and is not part of the adjoint.
+template <typename R>
+struct Vjp3D {
+  const R* x;  // positions x_n, all cells
+  const R* y;
+  const R* z;
+  const R* w;  // contact weight W per vertex at step n (nullptr: no contact)
+  const int* rec_cell;  // contact partners at step n (see Step3D)
+  const R* rec_w;
+  const R* rec_shift;
+  R* cx;  // [3 * nverts] contributions to other cells' lambda (atomic)
+  R* cy;
+  R* cz;
+  R* lx;       // adjoint: lambda_{n+1} in, lambda_n out (in place)
+  R* ly;
+  R* lz;
+  double* gtheta;  // [ncells*4] dL/d(Kv, Ka, Kb, Ks) accumulated
+  const float* Kv;
+  const float* Ka;
+  const float* Ks;
+  const float* Kb;
+  const float* v0;
+  const float* a0;
+  const int* cell_tissue;
+  const float* tL;
+  const float* tKre;
+  const float* tksg;
+  const int* faces;
+  const int* vf;
+  const int* adj;
+  int nv, nf;
+  R dt;
+  unsigned mask;
+};
+
+// LDS of the adjoint kernel (byte offsets): positions, lambda, bending
+// scratch [3 nv] each, face table, per (face, corner) contributions, and
+// reduction scratch — disjoint, as every region is live at once.
+struct LdsVjpLayout {
+  size_t x, lam, sb, faces, fc, red, total;
+};
+__host__ __device__ inline LdsVjpLayout ldsvjp_layout(int nv, int nf, int rsize) {
+  LdsVjpLayout L;
+  const size_t rs = (size_t)rsize;
+  size_t o = 0;
+  L.x = o;     o += 3u * rs * (size_t)nv;
+  L.lam = o;   o += 3u * rs * (size_t)nv;
+  L.sb = o;    o += 3u * rs * (size_t)nv;
+  o = (o + 7u) & ~(size_t)7u;
+  L.faces = o; o += 8u * (size_t)nf;
+  o = (o + 15u) & ~(size_t)15u;
+  L.fc = o;    o += 9u * rs * (size_t)nf;
+  o = (o + 7u) & ~(size_t)7u;
+  L.red = o;   o += 8u * 32u;
+  L.total = o;
+  return L;
+}
+
+template <typename R>
+__device__ __forceinline__ V3<R> jnorm(V3<R> v, V3<R> l) {
+  // N(v) l = (l - (u.l) u)/|v|, u = v/|v|: the Jacobian of normalize(v)
+  const R n = sqrt(vdot(v, v));
+  if (n == R(0)) return {R(0), R(0), R(0)};
+  const V3<R> u = vscale(v, R(1) / n);
+  return vscale(vsub(l, vscale(u, vdot(u, l))), R(1) / n);
+}
+
+template <typename R>
+__device__ void k3d_vjp_body(const Vjp3D<R>& a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nv = a.nv, nf = a.nf, NT = blockDim.x, tid = threadIdx.x;
+  const int ci = xcd_remap(blockIdx.x, gridDim.x);
+  const LdsVjpLayout LL = ldsvjp_layout(nv, nf, (int)sizeof(R));
+  Lds3D<R> l{};
+  l.sx = reinterpret_cast<R*>(smem + LL.x);
+  l.sy = l.sx + nv;
+  l.sz = l.sy + nv;
+  l.nx = reinterpret_cast<R*>(smem + LL.lam);  // lambda
+  l.ny = l.nx + nv;
+  l.nz = l.ny + nv;
+  l.sb = reinterpret_cast<R*>(smem + LL.sb);
+  l.sf = reinterpret_cast<uint2*>(smem + LL.faces);
+  l.fcx = reinterpret_cast<R*>(smem + LL.fc);
+  l.fcy = l.fcx + 3 * nf;
+  l.fcz = l.fcy + 3 * nf;
+  l.red = reinterpret_cast<double*>(smem + LL.red);
+  const size_t base = (size_t)ci * nv;
+  const bool own = tid < nv;
+  const V3<R> zero = {R(0), R(0), R(0)};
+  for (int f = tid; f < nf; f += NT)
+    l.sf[f] = make_uint2((unsigned)a.faces[3 * f] | ((unsigned)a.faces[3 * f + 1] << 16),
+                         (unsigned)a.faces[3 * f + 2]);
+  for (int v = tid; v < nv; v += NT) {
+    l.sx[v] = a.x[base + v];
+    l.sy[v] = a.y[base + v];
+    l.sz[v] = a.z[base + v];
+    l.nx[v] = a.lx[base + v];  // lambda
+    l.ny[v] = a.ly[base + v];
+    l.nz[v] = a.lz[base + v];
+  }
+  int vfr[DPM3D_MAXDEG], adjr[DPM3D_MAXDEG];
+#pragma unroll
+  for (int k = 0; k < DPM3D_MAXDEG; k++) {
+    vfr[k] = own ? a.vf[tid * DPM3D_MAXDEG + k] : -1;
+    adjr[k] = own ? a.adj[tid * DPM3D_MAXDEG + k] : -1;
+  }
+  __syncthreads();
+
+  const int t = a.cell_tissue[ci];
+  const R L = (R)a.tL[t], Kre = (R)a.tKre[t], ksg = (R)a.tksg[t];
+  const R Kvi = (R)a.Kv[ci], Kai = (R)a.Ka[ci], Ksi = (R)a.Ks[ci], Kbi = (R)a.Kb[ci];
+  const R v0i = (R)a.v0[ci], a0i = (R)a.a0[ci], l0i = cell_l0<R>(a.a0, ci);
+  const unsigned mask = a.mask;
+
+  // centre of mass and volume of x_n, as the forward step's statistics
+  double px = 0.0, py = 0.0, pz = 0.0;
+  for (int v = tid; v < nv; v += NT) {
+    px += (double)l.sx[v];
+    py += (double)l.sy[v];
+    pz += (double)l.sz[v];
+  }
+  px = block_sum(px, l.red);
+  py = block_sum(py, l.red);
+  pz = block_sum(pz, l.red);
+  const V3<R> C = {(R)(px / nv), (R)(py / nv), (R)(pz / nv)};
+  double vol = 0.0, S1 = 0.0;
+  const bool vol_on = (mask & M_VOLUME) && Kvi != R(0);
+  for (int f = tid; f < nf; f += NT) {
+    int i0, i1, i2;
+    face_of(l, f, i0, i1, i2);
+    const V3<R> P0 = vsub(lds_v(l.sx, l.sy, l.sz, i0), C);
+    const V3<R> P1 = vsub(lds_v(l.sx, l.sy, l.sz, i1), C);
+    const V3<R> P2 = vsub(lds_v(l.sx, l.sy, l.sz, i2), C);
+    vol += (double)vdot(vcross(P0, P1), P2);
+    if (vol_on) {  // S1 = sum_i grad_i V . lambda_i
+      S1 += (double)vdot(vcross(P1, P2), lds_v(l.nx, l.ny, l.nz, i0)) +
+            (double)vdot(vcross(P2, P0), lds_v(l.nx, l.ny, l.nz, i1)) +
+            (double)vdot(vcross(P0, P1), lds_v(l.nx, l.ny, l.nz, i2));
+    }
+  }
+  vol = fabs(block_sum(vol, l.red) / 6.0);
+  S1 = block_sum(S1, l.red) / 6.0;
+  const R strain = (R)(vol / (double)v0i - 1.0);
+
+  // per face-corner adjoint contributions (into the shape-force bytes)
+  const bool surf_on = (mask & M_SURFACE) && !(Kai < R(1e-8f));
+  const bool sub_on = (mask & M_SUBSTRATE) && !(Ksi < R(1e-12f));
+  const R sc = Kai * sqrt(a0i) / l0i * R(0.3f);
+  const R kv1 = -Kvi / v0i * (R)S1;  // coefficient of g_j
+  const R kv2 = -Kvi * strain;       // coefficient of grad(g.l)_j
+  double gKa = 0.0, gKs = 0.0;
+  double Lcx = 0.0, Lcy = 0.0, Lcz = 0.0;  // sum of (dF/dC)^T lambda
+  for (int f = tid; f < nf; f += NT) {
+    int id[3];
+    face_of(l, f, id[0], id[1], id[2]);
+    V3<R> P[3], Q[3], lam[3], c[3];
+    for (int k = 0; k < 3; k++) {
+      P[k] = lds_v(l.sx, l.sy, l.sz, id[k]);
+      Q[k] = vsub(P[k], C);
+      lam[k] = lds_v(l.nx, l.ny, l.nz, id[k]);
+      c[k] = zero;
+    }
+    if (vol_on)
+      for (int k = 0; k < 3; k++) {
+        const int b = (k + 1) % 3, cc = (k + 2) % 3;
+        const V3<R> g = vscale(vcross(Q[b], Q[cc]), R(1) / R(6));
+        const V3<R> h = vscale(vadd(vcross(lam[b], Q[cc]), vcross(Q[b], lam[cc])), R(1) / R(6));
+        c[k] = vadd(c[k], vadd(vscale(g, kv1), vscale(h, kv2)));
+      }
+    if (surf_on) {
+      const R inv_l0 = R(1) / l0i;
+      for (int e = 0; e < 3; e++) {  // edge e from corner e to corner e+1
+        const int ea = e, eb = (e + 1) % 3;
+        const V3<R> lv = vsub(P[eb], P[ea]);
+        const R len = sqrt(vdot(lv, lv));
+        if (len < R(1e-12f)) continue;
+        const V3<R> u = vscale(lv, R(1) / len);
+        const R d = len * inv_l0 - R(1);
+        const V3<R> dl = vsub(lam[ea], lam[eb]);
+        const R ud = vdot(u, dl);
+        // sc M dl, M = u u^T / l0 + (d/len)(I - u u^T)
+        const V3<R> Md = vscale(vadd(vscale(u, ud * inv_l0), vscale(vsub(dl, vscale(u, ud)), d / len)), sc);
+        c[ea] = vsub(c[ea], Md);
+        c[eb] = vadd(c[eb], Md);
+        // lambda . F_surf / Ka: the edge pulls a by sc d u and b by -sc d u
+        gKa += (double)(sc * d * ud) / (double)Kai;
+      }
+    }
+    if (sub_on) {
+      const V3<R> un = vnormalize(vcross(vsub(P[1], P[0]), vsub(P[2], P[0])));
+      if (un.z < R(-0.1f)) {
+        for (int k = 0; k < 3; k++) {
+          const V3<R> pos = P[k], lk = lam[k];
+          R ks = Ksi, dks = R(0);  // ks and d ks / d x
+          if (ksg != R(0)) {
+            const R xw = pos.x - L * floor(pos.x / L);
+            ks = Ksi * (R(1) + ksg * xw / L);
+            dks = Ksi * ksg / L;
+          }
+          V3<R> mu = zero;
+          R fdotl = R(0);  // lambda . F_corner
+          if (pos.z < R(0)) {  // f1 = (0, 0, ks |z|)
+            mu.z += -ks * lk.z;
+            mu.x += fabs(pos.z) * dks * lk.z;
+            fdotl += ks * fabs(pos.z) * lk.z;
+          }
+          const R height = fabs(pos.z);
+          if (height < l0i * R(2)) {  // f2 = normalize(c2v) m
+            const V3<R> c2v = {pos.x - C.x, pos.y - C.y, R(0) - C.z};
+            const R m = ks * (R(1) - height / l0i);
+            const V3<R> n = vnormalize(c2v);
+            const V3<R> Nl = jnorm(c2v, lk);
+            const R nl = vdot(n, lk);
+            mu.x += m * Nl.x + nl * dks * (R(1) - height / l0i);
+            mu.y += m * Nl.y;
+            mu.z += nl * (-ks * (pos.z < R(0) ? R(-1) : (pos.z > R(0) ? R(1) : R(0))) / l0i);
+            Lcx -= (double)(m * Nl.x);
+            Lcy -= (double)(m * Nl.y);
+            Lcz -= (double)(m * Nl.z);
+            fdotl += m * nl;
+          }
+          c[k] = vadd(c[k], mu);
+          gKs += (double)fdotl / (double)Ksi;
+        }
+      }
+    }
+    for (int k = 0; k < 3; k++) {
+      l.fcx[3 * f + k] = c[k].x;
+      l.fcy[3 * f + k] = c[k].y;
+      l.fcz[3 * f + k] = c[k].z;
+    }
+  }
+  __syncthreads();
+  V3<R> mu = zero;
+  if (own) {
+#pragma unroll
+    for (int k = 0; k < DPM3D_MAXDEG; k++) {
+      const int e = vfr[k];
+      if (e < 0) break;
+      mu = vadd(mu, V3<R>{l.fcx[e], l.fcy[e], l.fcz[e]});
+    }
+  }
+
+  // bending: mu += -Kb L (L lambda); dL/dKb = -(L lambda).(L x)
+  double gKb = 0.0;
+  if ((mask & M_BENDING) && Kbi != R(0)) {
+    if (own) {
+      V3<R> sl = zero, sx = zero;
+      const V3<R> P = lds_v(l.sx, l.sy, l.sz, tid), lt = lds_v(l.nx, l.ny, l.nz, tid);
+#pragma unroll
+      for (int k = 0; k < DPM3D_MAXDEG; k++) {
+        const int u = adjr[k];
+        if (u < 0) break;
+        sl = vadd(sl, vsub(lds_v(l.nx, l.ny, l.nz, u), lt));
+        sx = vadd(sx, vsub(lds_v(l.sx, l.sy, l.sz, u), P));
+      }
+      l.sb[3 * tid] = sl.x;
+      l.sb[3 * tid + 1] = sl.y;
+      l.sb[3 * tid + 2] = sl.z;
+      gKb = -(double)vdot(sl, sx);
+    }
+    __syncthreads();
+    if (own) {
+      V3<R> acc = zero;
+      int deg = 0;
+#pragma unroll
+      for (int k = 0; k < DPM3D_MAXDEG; k++) {
+        const int u = adjr[k];
+        if (u < 0) break;
+        deg++;
+        acc = vadd(acc, V3<R>{l.sb[3 * u], l.sb[3 * u + 1], l.sb[3 * u + 2]});
+      }
+      const V3<R> sv = {l.sb[3 * tid], l.sb[3 * tid + 1], l.sb[3 * tid + 2]};
+      mu = vadd(mu, vscale(vsub(acc, vscale(sv, (R)deg)), -Kbi));
+    }
+  }
+
+  // contact: F = W Kre/2 normalize(C - p), W = sum_j |w_j(p, x_j)|
+  if (own && a.w && (mask & M_CONTACT)) {
+    const R W = a.w[base + tid];
+    if (W != R(0)) {
+      const V3<R> P = lds_v(l.sx, l.sy, l.sz, tid), lt = lds_v(l.nx, l.ny, l.nz, tid);
+      const V3<R> Nl = jnorm(vsub(C, P), lt);
+      const R k = W * R(0.5f) * Kre;
+      mu = vsub(mu, vscale(Nl, k));
+      Lcx += (double)(k * Nl.x);
+      Lcy += (double)(k * Nl.y);
+      Lcz += (double)(k * Nl.z);
+      // the winding numbers' own dependence (skipped faces only)
+      const R nl = vdot(vnormalize(vsub(C, P)), lt);
+      const R inv4pi = R(1) / (R(4) * R(3.14159265358979323846));
+      for (int q = 0; q < DPM3D_MAXPART; q++) {
+        const size_t r = (base + tid) * DPM3D_MAXPART + q;
+        const int cj = a.rec_cell[r];
+        if (cj < 0) break;
+        const R wq = a.rec_w[r];
+        const R coef = (wq > R(0) ? R(1) : R(-1)) * R(0.5f) * Kre * nl * inv4pi;
+        const V3<R> sh = {a.rec_shift[3 * r], a.rec_shift[3 * r + 1], a.rec_shift[3 * r + 2]};
+        const size_t bj = (size_t)cj * nv;
+        for (int f = 0; f < nf; f++) {
+          int ia, ib, ic;
+          face_of(l, f, ia, ib, ic);
+          // the forward step's operation order (Cell3D_Kernel.cl:285-287)
+          const V3<R> A = {(a.x[bj + ia] + sh.x) - P.x, (a.y[bj + ia] + sh.y) - P.y, (a.z[bj + ia] + sh.z) - P.z};
+          const V3<R> B = {(a.x[bj + ib] + sh.x) - P.x, (a.y[bj + ib] + sh.y) - P.y, (a.z[bj + ib] + sh.z) - P.z};
+          const V3<R> Cc = {(a.x[bj + ic] + sh.x) - P.x, (a.y[bj + ic] + sh.y) - P.y, (a.z[bj + ic] + sh.z) - P.z};
+          const V3<R> u = vnormalize(A), v = vnormalize(B), w = vnormalize(Cc);
+          const R den = ((R(1) + vdot(u, v)) + vdot(v, w)) + vdot(w, u);
+          if (!(den < R(1e-8f))) continue;  // kept face: no contribution
+          const R num = vdot(u, vcross(v, w));
+          const R s2 = R(2) / (num * num + den * den);
+          const R cN = s2 * den, cD = -s2 * num;
+          const V3<R> du = vadd(vscale(vcross(v, w), cN), vscale(vadd(v, w), cD));
+          const V3<R> dv = vadd(vscale(vcross(w, u), cN), vscale(vadd(u, w), cD));
+          const V3<R> dw = vadd(vscale(vcross(u, v), cN), vscale(vadd(u, v), cD));
+          const V3<R> ga = jnorm(A, du), gb = jnorm(B, dv), gc = jnorm(Cc, dw);
+          // dw/dp = +(ga + gb + gc)/(4 pi), dw/dx_a = -ga/(4 pi) (skipped face)
+          mu = vadd(mu, vscale(vadd(vadd(ga, gb), gc), coef));
+          atomicAdd(&a.cx[bj + ia], -coef * ga.x);
+          atomicAdd(&a.cy[bj + ia], -coef * ga.y);
+          atomicAdd(&a.cz[bj + ia], -coef * ga.z);
+          atomicAdd(&a.cx[bj + ib], -coef * gb.x);
+          atomicAdd(&a.cy[bj + ib], -coef * gb.y);
+          atomicAdd(&a.cz[bj + ib], -coef * gb.z);
+          atomicAdd(&a.cx[bj + ic], -coef * gc.x);
+          atomicAdd(&a.cy[bj + ic], -coef * gc.y);
+          atomicAdd(&a.cz[bj + ic], -coef * gc.z);
+        }
+      }
+    }
+  }
+  Lcx = block_sum(Lcx, l.red);
+  Lcy = block_sum(Lcy, l.red);
+  Lcz = block_sum(Lcz, l.red);
+  gKa = block_sum(gKa, l.red);
+  gKs = block_sum(gKs, l.red);
+  gKb = block_sum(gKb, l.red);
+  if (own) {
+    // the centre of mass moves with each vertex by 1/nv
+    mu = vadd(mu, V3<R>{(R)(Lcx / nv), (R)(Lcy / nv), (R)(Lcz / nv)});
+    const V3<R> lt = lds_v(l.nx, l.ny, l.nz, tid);
+    a.lx[base + tid] = lt.x + a.dt * mu.x;
+    a.ly[base + tid] = lt.y + a.dt * mu.y;
+    a.lz[base + tid] = lt.z + a.dt * mu.z;
+  }
+  if (tid == 0) {
+    const double dt = (double)a.dt;
+    a.gtheta[4 * ci + 0] += dt * (vol_on ? -(double)strain * S1 : 0.0);
+    a.gtheta[4 * ci + 1] += dt * gKa;
+    a.gtheta[4 * ci + 2] += dt * gKb;
+    a.gtheta[4 * ci + 3] += dt * gKs;
+  }
+}
+
 extern "C" __global__ void __launch_bounds__(1024) k3d_stats_f32(const Step3D<float> a) { k3d_stats_body(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_stats_f64(const Step3D<double> a) { k3d_stats_body(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_step_f32(const Step3D<float> a) { k3d_step_body(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_step_f64(const Step3D<double> a) { k3d_step_body(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_energy_f32(const Step3D<float> a) { k3d_energy_body(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_energy_f64(const Step3D<double> a) { k3d_energy_body(a); }
+// lambda += dt * (contributions from other cells), contributions cleared.
+template <typename R>
+__device__ void k3d_vjp_cross_body(R* lx, R* ly, R* lz, R* cx, R* cy, R* cz, R dt, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    lx[i] += dt * cx[i];
+    ly[i] += dt * cy[i];
+    lz[i] += dt * cz[i];
+    cx[i] = R(0);
+    cy[i] = R(0);
+    cz[i] = R(0);
+  }
+}
+extern "C" __global__ void k3d_vjp_cross_f32(float* lx, float* ly, float* lz, float* cx, float* cy, float* cz,
+                                             float dt, size_t n) {
+  k3d_vjp_cross_body(lx, ly, lz, cx, cy, cz, dt, n);
+}
+extern "C" __global__ void k3d_vjp_cross_f64(double* lx, double* ly, double* lz, double* cx, double* cy,
+                                             double* cz, double dt, size_t n) {
+  k3d_vjp_cross_body(lx, ly, lz, cx, cy, cz, dt, n);
+}
+extern "C" __global__ void __launch_bounds__(1024) k3d_vjp_f32(const Vjp3D<float> a) { k3d_vjp_body(a); }
+extern "C" __global__ void __launch_bounds__(1024) k3d_vjp_f64(const Vjp3D<double> a) { k3d_vjp_body(a); }

@@ -100,6 +100,22 @@ struct dpm_tissue {
   int graph_cur = -1;
   float graph_dt = 0.0f;
   int* flag = nullptr;  // device non-finite flag
+  // recorded trajectory for the adjoint
+  int traj_steps = 0, traj_every = 0;
+  float traj_dt = 0.0f;
+  size_t lds_vjp = 0;
+  void* ck = nullptr;    // checkpoints [(steps/every + 1) * 3 * NVT] reals
+  void* seg = nullptr;   // recomputed segment positions [every * 3 * NVT]
+  void* segw = nullptr;  // contact weights of the segment [every * NVT]
+  void* lam = nullptr;   // adjoint [3 * NVT]
+  double* gtheta = nullptr;  // [C * 4]
+  void* segwp = nullptr;     // recorded partner winding numbers [every * NVT * MAXPART]
+  void* segsh = nullptr;     // recorded partner shifts [every * NVT * MAXPART * 3]
+  int* segcell = nullptr;    // recorded partner cells [every * NVT * MAXPART]
+  void* cross = nullptr;     // other cells' adjoint contributions [3 * NVT]
+  int* rec_overflow = nullptr;
+  size_t ck_cap = 0, seg_cap = 0, segw_cap = 0, lam_cap = 0, segwp_cap = 0, segsh_cap = 0, cross_cap = 0,
+         segcell_cap = 0;
 
   // ------------------------------------------------------------------ 2D
   int V = 0, NT2 = 0, NTr = 0;  // Euler threads, relax threads per tissue
@@ -398,6 +414,9 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
                             {(const void*)k3d_step_f64, (const void*)k3d_stats_f64, (const void*)k3d_energy_f64}};
   for (const void* k : kern[rs == 8 ? 1 : 0])
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
+  h->lds_vjp = ldsvjp_layout(nv, nf, rs).total;
+  (void)hipFuncSetAttribute(rs == 8 ? (const void*)k3d_vjp_f64 : (const void*)k3d_vjp_f32,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_vjp);
   *out = h;
   return DPM_OK;
 }
@@ -746,6 +765,214 @@ extern "C" int dpm3d_probe(dpm_tissue* h, float* forces, unsigned char* contact,
   h->forces_valid = true;
   if (forces) TRY(dpm3d_get_forces(h, forces, NVT));
   return DPM_OK;
+}
+
+// ------------------------------------------------------------- 3D adjoint
+
+template <typename In>
+static int upload_real3(dpm_tissue* h, void* dst, const In* src, size_t n) {
+  if (h->rs == 8) {
+    std::vector<double> v(src, src + n);
+    TRY(upload((double*)dst, v.data(), n, h->stream));
+  } else {
+    std::vector<float> v(n);
+    for (size_t i = 0; i < n; i++) v[i] = (float)src[i];
+    TRY(upload((float*)dst, v.data(), n, h->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return DPM_OK;
+}
+
+static int ensure_real(dpm_tissue* h, void** p, size_t& cap, size_t n) {
+  if (*p && cap >= n) return DPM_OK;
+  if (*p) {
+    (void)hipFree(*p);
+    h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), *p), h->allocs.end());
+    *p = nullptr;
+  }
+  TRY(h->alloc_real(p, n));
+  cap = n;
+  return DPM_OK;
+}
+
+static int copy_positions_dd(dpm_tissue* h, void* dx, void* dy, void* dz, const void* sx, const void* sy,
+                             const void* sz) {
+  const size_t b = (size_t)h->C * h->nv * h->rs;
+  HIPCHECK(hipMemcpyAsync(dx, sx, b, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHECK(hipMemcpyAsync(dy, sy, b, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHECK(hipMemcpyAsync(dz, sz, b, hipMemcpyDeviceToDevice, h->stream));
+  return DPM_OK;
+}
+
+static void* slot(const dpm_tissue* h, void* base, size_t k) {
+  return (unsigned char*)base + k * (size_t)h->C * h->nv * h->rs;
+}
+
+extern "C" int dpm3d_trajectory(dpm_tissue* h, int nsteps, float dt, int every) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  if (nsteps <= 0 || every <= 0 || nsteps % every) return fail(DPM_EINVAL, "nsteps must be a positive multiple of every");
+  if (!(dt > 0.0f) || dt > 0.1f) return fail(DPM_EINVAL, "dt must be positive and reasonable");
+  if (h->mask & DPM3D_ADHESION)
+    return fail(DPM_EINVAL, "the adjoint does not include vertex adhesion: clear DPM3D_ADHESION");
+  if ((int)h->lds_vjp > max_lds_per_block(h->device))
+    return fail(DPM_ERANGE, "cell mesh does not fit the LDS of the adjoint kernel");
+  const size_t NVT = (size_t)h->C * h->nv;
+  const int nck = nsteps / every + 1;
+  TRY(ensure_real(h, &h->ck, h->ck_cap, (size_t)nck * 3 * NVT));
+  TRY(ensure_stats(h));
+  for (int k = 0; k < nck; k++) {
+    void* c = slot(h, h->ck, 3 * (size_t)k);
+    TRY(copy_positions_dd(h, c, slot(h, c, 1), slot(h, c, 2), h->x[h->cur], h->y[h->cur], h->z[h->cur]));
+    if (k < nck - 1) TRY(euler3d(h, every, dt));
+  }
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  h->traj_steps = nsteps;
+  h->traj_every = every;
+  h->traj_dt = dt;
+  return DPM_OK;
+}
+
+template <typename R>
+static Vjp3D<R> vjp_args(dpm_tissue* h, size_t s) {
+  Vjp3D<R> a{};
+  void* xs = slot(h, h->seg, 3 * s);
+  a.x = (const R*)xs;
+  a.y = (const R*)slot(h, xs, 1);
+  a.z = (const R*)slot(h, xs, 2);
+  a.w = (const R*)slot(h, h->segw, s);
+  const size_t NP = (size_t)h->C * h->nv * DPM3D_MAXPART;
+  a.rec_cell = h->segcell + s * NP;
+  a.rec_w = (const R*)h->segwp + s * NP;
+  a.rec_shift = (const R*)h->segsh + s * NP * 3;
+  a.cx = (R*)h->cross;
+  a.cy = (R*)slot(h, h->cross, 1);
+  a.cz = (R*)slot(h, h->cross, 2);
+  a.lx = (R*)h->lam;
+  a.ly = (R*)slot(h, h->lam, 1);
+  a.lz = (R*)slot(h, h->lam, 2);
+  a.gtheta = h->gtheta;
+  a.Kv = h->Kv;
+  a.Ka = h->Ka;
+  a.Ks = h->Ks;
+  a.Kb = h->Kb;
+  a.v0 = h->v0;
+  a.a0 = h->a0;
+  a.cell_tissue = h->cell_tissue;
+  a.tL = h->tL;
+  a.tKre = h->tKre;
+  a.tksg = h->tksg;
+  a.faces = h->faces;
+  a.vf = h->vf;
+  a.adj = h->adj;
+  a.nv = h->nv;
+  a.nf = h->nf;
+  a.dt = (R)h->traj_dt;
+  a.mask = h->mask;
+  return a;
+}
+
+// lambda_final and lambda0 are AoS (reference order) in In/Out precision.
+template <typename In, typename Out>
+static int adjoint3d(dpm_tissue* h, const In* lambda_final, double* dtheta, Out* lambda0) {
+  if (!h || h->dim != 3 || !lambda_final) return fail(DPM_EINVAL, "bad arguments");
+  if (!h->traj_steps) return fail(DPM_EINVAL, "no recorded trajectory: call dpm3d_trajectory first");
+  if (h->mask & DPM3D_ADHESION)
+    return fail(DPM_EINVAL, "the adjoint does not include vertex adhesion: clear DPM3D_ADHESION");
+  const size_t NVT = (size_t)h->C * h->nv;
+  const int every = h->traj_every, nseg = h->traj_steps / every;
+  TRY(ensure_real(h, &h->seg, h->seg_cap, (size_t)every * 3 * NVT));
+  TRY(ensure_real(h, &h->segw, h->segw_cap, (size_t)every * NVT));
+  TRY(ensure_real(h, &h->lam, h->lam_cap, 3 * NVT));
+  TRY(ensure_real(h, &h->segwp, h->segwp_cap, (size_t)every * NVT * DPM3D_MAXPART));
+  TRY(ensure_real(h, &h->segsh, h->segsh_cap, (size_t)every * NVT * DPM3D_MAXPART * 3));
+  TRY(ensure_real(h, &h->cross, h->cross_cap, 3 * NVT));
+  if (h->segcell_cap < (size_t)every * NVT * DPM3D_MAXPART) {
+    if (h->segcell) {
+      (void)hipFree(h->segcell);
+      h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), (void*)h->segcell), h->allocs.end());
+      h->segcell = nullptr;
+    }
+    TRY(h->alloc(&h->segcell, (size_t)every * NVT * DPM3D_MAXPART));
+    h->segcell_cap = (size_t)every * NVT * DPM3D_MAXPART;
+  }
+  if (!h->gtheta) TRY(h->alloc(&h->gtheta, (size_t)h->C * 4));
+  if (!h->rec_overflow) TRY(h->alloc(&h->rec_overflow, 1));
+  HIPCHECK(hipMemsetAsync(h->rec_overflow, 0, sizeof(int), h->stream));
+  HIPCHECK(hipMemsetAsync(h->cross, 0, 3 * NVT * h->rs, h->stream));
+  // lambda_N
+  {
+    std::vector<In> a(NVT), b(NVT), c(NVT);
+    for (size_t i = 0; i < NVT; i++) {
+      a[i] = lambda_final[3 * i];
+      b[i] = lambda_final[3 * i + 1];
+      c[i] = lambda_final[3 * i + 2];
+    }
+    TRY(upload_real3(h, h->lam, a.data(), NVT));
+    TRY(upload_real3(h, slot(h, h->lam, 1), b.data(), NVT));
+    TRY(upload_real3(h, slot(h, h->lam, 2), c.data(), NVT));
+  }
+  HIPCHECK(hipMemsetAsync(h->gtheta, 0, sizeof(double) * h->C * 4, h->stream));
+  const float dt = h->traj_dt;
+  for (int sg = nseg - 1; sg >= 0; sg--) {
+    // recompute the segment from its checkpoint, keeping x_n and W_n
+    void* c = slot(h, h->ck, 3 * (size_t)sg);
+    h->cur = 0;
+    TRY(copy_positions_dd(h, h->x[0], h->y[0], h->z[0], c, slot(h, c, 1), slot(h, c, 2)));
+    h->stats_valid = false;
+    TRY(ensure_stats(h));
+    for (int s = 0; s < every; s++) {
+      void* xs = slot(h, h->seg, 3 * (size_t)s);
+      TRY(copy_positions_dd(h, xs, slot(h, xs, 1), slot(h, xs, 2), h->x[h->cur], h->y[h->cur], h->z[h->cur]));
+      void* w = slot(h, h->segw, s);
+      const size_t NP = NVT * DPM3D_MAXPART;
+      TRY(launch3d(h, K_STEP, h->cur, dt, F_MOVE, [&](auto& a) {
+        using R = std::remove_reference_t<decltype(*a.w_out)>;
+        a.w_out = (R*)w;
+        a.rec_cell = h->segcell + s * NP;
+        a.rec_w = (R*)h->segwp + s * NP;
+        a.rec_shift = (R*)h->segsh + s * NP * 3;
+        a.rec_overflow = h->rec_overflow;
+      }));
+      h->cur = 1 - h->cur;
+    }
+    // reverse sweep through the segment
+    for (int s = every - 1; s >= 0; s--) {
+      const int cb = (int)std::min<size_t>((NVT + 255) / 256, 1024);
+      if (h->rs == 8) {
+        Vjp3D<double> a = vjp_args<double>(h, s);
+        hipLaunchKernelGGL(k3d_vjp_f64, dim3(h->C), dim3(h->NT), h->lds_vjp, h->stream, a);
+        hipLaunchKernelGGL(k3d_vjp_cross_f64, dim3(cb), dim3(256), 0, h->stream, a.lx, a.ly, a.lz, a.cx, a.cy,
+                           a.cz, (double)a.dt, NVT);
+      } else {
+        Vjp3D<float> a = vjp_args<float>(h, s);
+        hipLaunchKernelGGL(k3d_vjp_f32, dim3(h->C), dim3(h->NT), h->lds_vjp, h->stream, a);
+        hipLaunchKernelGGL(k3d_vjp_cross_f32, dim3(cb), dim3(256), 0, h->stream, a.lx, a.ly, a.lz, a.cx, a.cy,
+                           a.cz, a.dt, NVT);
+      }
+      HIPCHECK(hipGetLastError());
+    }
+  }
+  // back to the final state of the trajectory
+  void* last = slot(h, h->ck, 3 * (size_t)nseg);
+  h->cur = 0;
+  TRY(copy_positions_dd(h, h->x[0], h->y[0], h->z[0], last, slot(h, last, 1), slot(h, last, 2)));
+  h->stats_valid = false;
+  if (dtheta)
+    HIPCHECK(hipMemcpyAsync(dtheta, h->gtheta, sizeof(double) * h->C * 4, hipMemcpyDeviceToHost, h->stream));
+  if (lambda0) TRY(download3(h, h->lam, slot(h, h->lam, 1), slot(h, h->lam, 2), lambda0, NVT));
+  int overflow = 0;
+  HIPCHECK(hipMemcpyAsync(&overflow, h->rec_overflow, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  if (overflow)
+    return fail(DPM_ERANGE, "a vertex was in contact with more than 4 cells: the adjoint does not cover it");
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_adjoint(dpm_tissue* h, const float* lambda_final, double* dtheta, float* lambda0) {
+  return adjoint3d(h, lambda_final, dtheta, lambda0);
+}
+extern "C" int dpm3d_adjoint_f64(dpm_tissue* h, const double* lambda_final, double* dtheta, double* lambda0) {
+  return adjoint3d(h, lambda_final, dtheta, lambda0);
 }
 
 // ================================================================== 2D ====

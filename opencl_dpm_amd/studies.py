@@ -2,6 +2,7 @@
 
   python -m opencl_dpm_amd.studies precision [--steps 5000] [--every 500]
   python -m opencl_dpm_amd.studies batch [--n 512] [--fire-iters 200] [--jam]
+  python -m opencl_dpm_amd.studies adjoint [--steps 200] [--every 20] [--fd-cells 8]
 
 precision: the float production step (float positions and forces, double
 per-cell sums) against the all-double step (dpm3d_create_f64) from the
@@ -208,6 +209,105 @@ def batch(n: int, fire_iters: int, jam: bool, max_iter: int, seed: int = 0, jam_
     return out
 
 
+ADJ_MASK = (capi.DPM3D_VOLUME | capi.DPM3D_SURFACE | capi.DPM3D_SUBSTRATE | capi.DPM3D_CONTACT |
+            capi.DPM3D_BENDING)
+ADJ_NAMES = ("Kv", "Ka", "Kb", "Ks")
+
+
+def _adj_tissue(m, precision, th):
+    return capi.Tissues3D(m.mesh.faces, [0, m.C], th["Kv"], th["Ka"], th["Ks"], m.v0, m.a0, m.L, m.Kre, Kat=0.0,
+                          Kb=th["Kb"], ks_grad=m.ks_grad, mask=ADJ_MASK, nv=m.mesh.nv, precision=precision)
+
+
+def _com_loss(m, XN, X0):
+    d = XN.reshape(m.C, -1, 3).mean(1) - X0.reshape(m.C, -1, 3).mean(1)
+    n = np.linalg.norm(d, axis=1)
+    lam = np.repeat(d / n[:, None] / m.C / m.mesh.nv, m.mesh.nv, axis=0)
+    return float(n.mean()), lam
+
+
+def adjoint(nsteps: int, every: int, fd_cells: int, dt: float = 0.002, eps: float = 1e-6):
+    """Differentiation form: 64 cells x 162 vertices on a substrate whose
+    stiffness grows 10x across x; loss = mean |COM_N - COM_0| after nsteps;
+    gradients with respect to per-cell (Kv, Ka, Kb, Ks) by the float and the
+    double adjoint (checkpoints every `every` steps), against central
+    differences of the double forward (step eps, between float-stored
+    values) for the first fd_cells cells."""
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    m = W.monolayer_config("cfg5")
+    C = m.C
+    b = lambda v: np.broadcast_to(np.asarray(v, np.float32), (C,)).astype(np.float64)
+    th = {"Kv": b(m.Kv), "Ka": b(m.Ka), "Kb": b(m.Kb), "Ks": b(m.Ks)}
+    X0 = m.X.astype(np.float64)
+    out = {"study": "adjoint", "cells": C, "verts_per_cell": m.mesh.nv, "steps": nsteps, "every": every}
+    grads = {}
+    for prec in (32, 64):
+        t = _adj_tissue(m, prec, th)
+        t.set_positions(X0 if prec == 64 else m.X)
+        t.counters(reset=True)
+        (_, ms_f) = _timed_call(t, lambda: t.trajectory(nsteps, dt, every))
+        contacts = int(t.counters()[2])
+        L, lam = _com_loss(m, t.positions(np.float64), X0)
+        (res, ms_b) = _timed_call(t, lambda: t.adjoint(lam))
+        grads[prec] = res[0]
+        log(f"[adjoint] f{prec}: loss {L:.9e}; forward {ms_f:.1f} ms, adjoint {ms_b:.1f} ms; "
+            f"contact pairs along the trajectory {contacts}")
+        out[f"f{prec}"] = {"loss": L, "forward_ms": ms_f, "adjoint_ms": ms_b, "contact_pairs": contacts}
+    # (1) central differences with step eps; (2) the median of the
+    # difference quotients over consecutive float values theta_0 + k ulp,
+    # k = -8..8, which ignores the isolated jumps that the force law's
+    # cut-offs (substrate height and facing tests, contact onsets) put in the
+    # loss; jumps are counted as quotients further than 20 median absolute
+    # deviations from the median.
+    def loss_at(c, name, value):
+        tp = {q: a.copy() for q, a in th.items()}
+        tp[name][c] = value
+        t = _adj_tissue(m, 64, tp)
+        t.set_positions(X0)
+        t.euler(nsteps, dt)
+        return _com_loss(m, t.positions(np.float64), X0)[0]
+
+    fd = np.zeros((fd_cells, 4))
+    med = np.zeros((fd_cells, 4))
+    jumps = 0
+    for c in range(fd_cells):
+        for k, name in enumerate(ADJ_NAMES):
+            x0 = np.float32(th[name][c])
+            xp, xm = float(np.float32(x0 * (1 + eps))), float(np.float32(x0 * (1 - eps)))
+            fd[c, k] = (loss_at(c, name, xp) - loss_at(c, name, xm)) / (xp - xm)
+            xs = [float(x0)]
+            up, dn = x0, x0
+            for _ in range(8):
+                up = np.nextafter(up, np.float32(np.inf))
+                dn = np.nextafter(dn, np.float32(-np.inf))
+                xs = [float(dn)] + xs + [float(up)]
+            Ls = [loss_at(c, name, x) for x in xs]
+            q = np.diff(Ls) / np.diff(xs)
+            mq = float(np.median(q))
+            mad = float(np.median(np.abs(q - mq))) + 1e-300
+            jumps += int(np.sum(np.abs(q - mq) > 20 * mad))
+            med[c, k] = mq
+        log(f"[adjoint] differences: {c + 1}/{fd_cells} cells")
+    out["jumps_in_ulp_scans"] = jumps
+    log(f"[adjoint] quotients flagged as jumps in the 1-ulp scans: {jumps} of {fd.size * 16}")
+    for prec in (32, 64):
+        g = grads[prec][:fd_cells]
+        for label, ref in (("central differences", fd), ("median 1-ulp quotients", med)):
+            err = float(np.linalg.norm(g - ref) / np.linalg.norm(ref))
+            per = [float(np.linalg.norm(g[:, k] - ref[:, k]) / np.linalg.norm(ref[:, k])) for k in range(4)]
+            log(f"[adjoint] f{prec} adjoint vs FP64 {label}: relative error {err:.3e} (Kv {per[0]:.2e}, "
+                f"Ka {per[1]:.2e}, Kb {per[2]:.2e}, Ks {per[3]:.2e}) over {ref.size} parameters")
+            key = "central" if label.startswith("central") else "ulp_median"
+            out[f"f{prec}"][f"rel_error_{key}"] = err
+            out[f"f{prec}"][f"rel_error_{key}_per_kind"] = dict(zip(ADJ_NAMES, per))
+    gd = float(np.linalg.norm(grads[32] - grads[64]) / np.linalg.norm(grads[64]))
+    log(f"[adjoint] f32 vs f64 adjoint: relative difference {gd:.3e}")
+    out["f32_vs_f64_adjoint"] = gd
+    out["fd_cells"] = fd_cells
+    print(json.dumps(out), flush=True)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="study", required=True)
@@ -223,12 +323,18 @@ def main(argv=None):
     p.add_argument("--jam", action="store_true")
     p.add_argument("--max-iter", type=int, default=50000)
     p.add_argument("--jam-stages", type=int, default=2)
+    p = sub.add_parser("adjoint")
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--every", type=int, default=20)
+    p.add_argument("--fd-cells", type=int, default=8)
     a = ap.parse_args(argv)
     if capi.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
     if a.study == "precision":
         precision(a.steps, a.every, seed=a.seed, config=a.config, mask=a.terms)
+    elif a.study == "adjoint":
+        adjoint(a.steps, a.every, a.fd_cells)
     elif a.study == "batch":
         batch(a.n, a.fire_iters, a.jam, a.max_iter, jam_stages=a.jam_stages)
     return 0
