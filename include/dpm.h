@@ -173,6 +173,15 @@ int dpm3d_energy(dpm_tissue* h, double* energy);
  * their derivatives vanish almost everywhere.  After the call the positions
  * are those at the end of the trajectory. */
 int dpm3d_trajectory(dpm_tissue* h, int nsteps, float dt, int every);
+/* Compensated positions for float tissues (no reference counterpart; off
+ * by default, where the update is the reference's x += F dt in float): each
+ * vertex carries the rounding residual of its position, x + lo, advanced by
+ * an exact TwoSum, so that displacements far below one float ulp of the
+ * coordinates accumulate instead of being lost.  Forces are evaluated at the
+ * float positions x.  dpm3d_get_positions_f64 returns x + lo; the float
+ * accessors return x.  Positions set afterwards keep their double part in
+ * lo. */
+int dpm3d_set_compensated(dpm_tissue* h, int on);
 int dpm3d_adjoint(dpm_tissue* h, const float* lambda_final, double* dtheta, float* lambda0);
 int dpm3d_adjoint_f64(dpm_tissue* h, const double* lambda_final, double* dtheta, double* lambda0);
 

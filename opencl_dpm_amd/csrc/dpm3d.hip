@@ -115,7 +115,21 @@ struct Step3D {
   R* rec_w;           // [nverts * MAXPART]
   R* rec_shift;       // [nverts * MAXPART * 3]
   int* rec_overflow;  // set when a vertex has more partners
+  // compensated positions (nullptr: plain update): the position of a vertex
+  // is x + xlo, xlo the rounding residual carried from step to step
+  R* xlo;
+  R* ylo;
+  R* zlo;
 };
+
+// a + b = s + e exactly (Knuth's TwoSum, any magnitudes).
+template <typename R>
+__device__ __forceinline__ R two_sum(R a, R b, R& e) {
+  const R s = a + b;
+  const R bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+  return s;
+}
 
 // Target edge length l0 = sqrt(4 a0)/sqrt(3) of a cell (Tissue3D.cpp:177),
 // in the real type (for float, the same correctly rounded operations as the
@@ -722,7 +736,18 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
   if (a.flags & F_MOVE) {  // block-uniform
     V3<R> Pn = P;
     if (own) {
-      Pn = v3(fma(F.x, a.dt, P.x), fma(F.y, a.dt, P.y), fma(F.z, a.dt, P.z));
+      if (a.xlo) {
+        // x + lo + F dt, the displacement rounded once onto the residual
+        R ex, ey, ez;
+        Pn = v3(two_sum(P.x, fma(F.x, a.dt, a.xlo[base + tid]), ex),
+                two_sum(P.y, fma(F.y, a.dt, a.ylo[base + tid]), ey),
+                two_sum(P.z, fma(F.z, a.dt, a.zlo[base + tid]), ez));
+        a.xlo[base + tid] = ex;
+        a.ylo[base + tid] = ey;
+        a.zlo[base + tid] = ez;
+      } else {
+        Pn = v3(fma(F.x, a.dt, P.x), fma(F.y, a.dt, P.y), fma(F.z, a.dt, P.z));
+      }
       a.xout[base + tid] = Pn.x;
       a.yout[base + tid] = Pn.y;
       a.zout[base + tid] = Pn.z;

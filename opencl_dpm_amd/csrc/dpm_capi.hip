@@ -101,7 +101,7 @@ struct dpm_tissue {
   float graph_dt = 0.0f;
   int* flag = nullptr;  // device non-finite flag
   // recorded trajectory for the adjoint
-  int traj_steps = 0, traj_every = 0;
+  int traj_steps = 0, traj_every = 0, traj_slots = 3;
   float traj_dt = 0.0f;
   size_t lds_vjp = 0;
   void* ck = nullptr;    // checkpoints [(steps/every + 1) * 3 * NVT] reals
@@ -116,6 +116,7 @@ struct dpm_tissue {
   int* rec_overflow = nullptr;
   size_t ck_cap = 0, seg_cap = 0, segw_cap = 0, lam_cap = 0, segwp_cap = 0, segsh_cap = 0, cross_cap = 0,
          segcell_cap = 0;
+  void* lo = nullptr;  // compensated positions: residuals [3 * NVT] (nullptr: off)
 
   // ------------------------------------------------------------------ 2D
   int V = 0, NT2 = 0, NTr = 0;  // Euler threads, relax threads per tissue
@@ -493,6 +494,14 @@ static int set_positions3(dpm_tissue* h, const In* xyz, size_t n) {
     return DPM_OK;
   };
   TRY(h->rs == 8 ? put(0.0) : put(0.0f));
+  if (h->lo) {  // residuals: what the float positions did not keep
+    std::vector<float> r(3 * n);
+    for (size_t i = 0; i < n; i++)
+      for (int k = 0; k < 3; k++)
+        r[k * n + i] = (float)((double)xyz[3 * i + k] - (double)(float)xyz[3 * i + k]);
+    TRY(upload((float*)h->lo, r.data(), 3 * n, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+  }
   h->stats_valid = false;
   return DPM_OK;
 }
@@ -521,7 +530,36 @@ static int get_positions3(dpm_tissue* h, Out* xyz, size_t n) {
   if (!h || h->dim != 3 || !xyz) return fail(DPM_EINVAL, "bad arguments");
   if (n != (size_t)h->C * h->nv) return fail(DPM_EINVAL, "position count does not match the tissue");
   const int k = h->cur;
-  return download3(h, h->x[k], h->y[k], h->z[k], xyz, n);
+  TRY(download3(h, h->x[k], h->y[k], h->z[k], xyz, n));
+  if (h->lo && sizeof(Out) == 8) {  // compensated: x + residual
+    std::vector<Out> r(3 * n);
+    TRY(download3(h, h->lo, (float*)h->lo + n, (float*)h->lo + 2 * n, r.data(), n));
+    for (size_t i = 0; i < 3 * n; i++) xyz[i] += r[i];
+  }
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_set_compensated(dpm_tissue* h, int on) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  if (h->rs != 4) return fail(DPM_EINVAL, "compensated positions apply to float tissues");
+  const size_t n = 3 * (size_t)h->C * h->nv;
+  if (on && !h->lo) {
+    float* p = nullptr;
+    TRY(h->alloc(&p, n));
+    h->lo = p;
+    HIPCHECK(hipMemsetAsync(h->lo, 0, n * sizeof(float), h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+  } else if (!on && h->lo) {
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(h->lo);
+    h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), h->lo), h->allocs.end());
+    h->lo = nullptr;
+  }
+  if (h->graph) {  // captured with the old arguments
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return DPM_OK;
 }
 
 template <typename Out>
@@ -596,6 +634,11 @@ static Step3D<R> step_args(dpm_tissue* h, int in, float dt, int flags) {
   a.counters = h->counters;
   a.ncmax = h->ncmax;
   a.energy = h->energy;
+  if (h->lo) {
+    a.xlo = (R*)h->lo;
+    a.ylo = (R*)h->lo + (size_t)h->C * h->nv;
+    a.zlo = (R*)h->lo + 2 * (size_t)h->C * h->nv;
+  }
   return a;
 }
 
@@ -818,14 +861,18 @@ extern "C" int dpm3d_trajectory(dpm_tissue* h, int nsteps, float dt, int every) 
     return fail(DPM_ERANGE, "cell mesh does not fit the LDS of the adjoint kernel");
   const size_t NVT = (size_t)h->C * h->nv;
   const int nck = nsteps / every + 1;
-  TRY(ensure_real(h, &h->ck, h->ck_cap, (size_t)nck * 3 * NVT));
+  const int ns = h->lo ? 6 : 3;  // x, y, z (+ their residuals when compensated)
+  TRY(ensure_real(h, &h->ck, h->ck_cap, (size_t)nck * ns * NVT));
   TRY(ensure_stats(h));
   for (int k = 0; k < nck; k++) {
-    void* c = slot(h, h->ck, 3 * (size_t)k);
+    void* c = slot(h, h->ck, (size_t)ns * k);
     TRY(copy_positions_dd(h, c, slot(h, c, 1), slot(h, c, 2), h->x[h->cur], h->y[h->cur], h->z[h->cur]));
+    if (h->lo)
+      HIPCHECK(hipMemcpyAsync(slot(h, c, 3), h->lo, 3 * NVT * h->rs, hipMemcpyDeviceToDevice, h->stream));
     if (k < nck - 1) TRY(euler3d(h, every, dt));
   }
   HIPCHECK(hipStreamSynchronize(h->stream));
+  h->traj_slots = ns;
   h->traj_steps = nsteps;
   h->traj_every = every;
   h->traj_dt = dt;
@@ -876,6 +923,9 @@ template <typename In, typename Out>
 static int adjoint3d(dpm_tissue* h, const In* lambda_final, double* dtheta, Out* lambda0) {
   if (!h || h->dim != 3 || !lambda_final) return fail(DPM_EINVAL, "bad arguments");
   if (!h->traj_steps) return fail(DPM_EINVAL, "no recorded trajectory: call dpm3d_trajectory first");
+  if ((h->lo ? 6 : 3) != h->traj_slots)
+    return fail(DPM_EINVAL, "compensation changed since the trajectory was recorded");
+  const int ns = h->traj_slots;
   if (h->mask & DPM3D_ADHESION)
     return fail(DPM_EINVAL, "the adjoint does not include vertex adhesion: clear DPM3D_ADHESION");
   const size_t NVT = (size_t)h->C * h->nv;
@@ -915,9 +965,11 @@ static int adjoint3d(dpm_tissue* h, const In* lambda_final, double* dtheta, Out*
   const float dt = h->traj_dt;
   for (int sg = nseg - 1; sg >= 0; sg--) {
     // recompute the segment from its checkpoint, keeping x_n and W_n
-    void* c = slot(h, h->ck, 3 * (size_t)sg);
+    void* c = slot(h, h->ck, (size_t)ns * sg);
     h->cur = 0;
     TRY(copy_positions_dd(h, h->x[0], h->y[0], h->z[0], c, slot(h, c, 1), slot(h, c, 2)));
+    if (h->lo)
+      HIPCHECK(hipMemcpyAsync(h->lo, slot(h, c, 3), 3 * NVT * h->rs, hipMemcpyDeviceToDevice, h->stream));
     h->stats_valid = false;
     TRY(ensure_stats(h));
     for (int s = 0; s < every; s++) {
@@ -953,9 +1005,11 @@ static int adjoint3d(dpm_tissue* h, const In* lambda_final, double* dtheta, Out*
     }
   }
   // back to the final state of the trajectory
-  void* last = slot(h, h->ck, 3 * (size_t)nseg);
+  void* last = slot(h, h->ck, (size_t)ns * nseg);
   h->cur = 0;
   TRY(copy_positions_dd(h, h->x[0], h->y[0], h->z[0], last, slot(h, last, 1), slot(h, last, 2)));
+  if (h->lo)
+    HIPCHECK(hipMemcpyAsync(h->lo, slot(h, last, 3), 3 * NVT * h->rs, hipMemcpyDeviceToDevice, h->stream));
   h->stats_valid = false;
   if (dtheta)
     HIPCHECK(hipMemcpyAsync(dtheta, h->gtheta, sizeof(double) * h->C * 4, hipMemcpyDeviceToHost, h->stream));

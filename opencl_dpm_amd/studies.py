@@ -75,11 +75,13 @@ def precision(steps: int, every: int, dt: float = 0.002, seed: int = 0, config: 
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     log(f"[precision] {m.C} cells x {m.mesh.nv} vertices = {nvt} vertices, {steps} steps, dt={dt}, "
         f"terms {mask}")
-    t32, t64, probe64 = [_tissue(m, p, MASKS[mask]) for p in (32, 64, 64)]
+    t32, t64, probe64, t32c = [_tissue(m, p, MASKS[mask]) for p in (32, 64, 64, 32)]
+    t32c.set_compensated(True)
     t32.set_positions(m.X)
+    t32c.set_positions(m.X)
     t64.set_positions(m.X.astype(np.float64))
     U0 = float(t64.energy().sum())
-    rows, ms32, ms64 = [], 0.0, 0.0
+    rows, ms32, ms64, ms32c = [], 0.0, 0.0, 0.0
     done = 0
     while True:
         probe64.set_positions(t32.positions(np.float64))
@@ -89,17 +91,24 @@ def precision(steps: int, every: int, dt: float = 0.002, seed: int = 0, config: 
         r["flipped_down_faces"] = int((g32["down"] != g64["down"]).sum())
         r["flipped_adhesion_vertices"] = int((g32["adhesion"] != g64["adhesion"]).sum())
         U32, U64 = float(t32.energy().sum()), float(t64.energy().sum())
-        r.update(step=done, U32=U32, U64=U64, drift=abs(U32 - U64) / abs(U64))
+        U32c = float(t32c.energy().sum())
+        X64 = t64.positions()
+        r.update(step=done, U32=U32, U64=U64, drift=abs(U32 - U64) / abs(U64), U32c=U32c,
+                 drift_compensated=abs(U32c - U64) / abs(U64),
+                 dx_plain=float(np.abs(t32.positions(np.float64) - X64).max()),
+                 dx_compensated=float(np.abs(t32c.positions(np.float64) - X64).max()))
         rows.append(r)
         log(f"[precision] step {done:5d}: force rel {r['force_rel_l2']:.3e}; per vertex median "
             f"{r['vertex_median']:.2e} p99.99 {r['vertex_p9999']:.2e} max {r['vertex_max']:.2e} "
             f"(>1e-3: {r['vertices_above_1e-3']}); flips: down {r['flipped_down_faces']} "
             f"adhesion {r['flipped_adhesion_vertices']};  U32 {U32:.10e} U64 {U64:.10e} "
-            f"drift {r['drift']:.3e}")
+            f"drift {r['drift']:.3e} (compensated {r['drift_compensated']:.3e}); max|x32 - x64| "
+            f"{r['dx_plain']:.2e} (compensated {r['dx_compensated']:.2e})")
         if done >= steps:
             break
         n = min(every, steps - done)
         ms32 += _timed(t32, n, dt)
+        ms32c += _timed(t32c, n, dt)
         ms64 += _timed(t64, n, dt)
         done += n
     out = {
@@ -116,6 +125,9 @@ def precision(steps: int, every: int, dt: float = 0.002, seed: int = 0, config: 
                                          for r in rows),
         "energy_drift_final": rows[-1]["drift"],
         "energy_drift_max": max(r["drift"] for r in rows),
+        "energy_drift_compensated_final": rows[-1]["drift_compensated"],
+        "energy_drift_compensated_max": max(r["drift_compensated"] for r in rows),
+        "f32_compensated_ms_per_step": ms32c / steps,
         "f32_vertex_steps_per_s": nvt * steps / (ms32 / 1e3),
         "f64_vertex_steps_per_s": nvt * steps / (ms64 / 1e3),
         "f32_ms_per_step": ms32 / steps,
@@ -241,8 +253,10 @@ def adjoint(nsteps: int, every: int, fd_cells: int, dt: float = 0.002, eps: floa
     X0 = m.X.astype(np.float64)
     out = {"study": "adjoint", "cells": C, "verts_per_cell": m.mesh.nv, "steps": nsteps, "every": every}
     grads = {}
-    for prec in (32, 64):
-        t = _adj_tissue(m, prec, th)
+    for prec in (32, 64, "32c"):
+        t = _adj_tissue(m, 64 if prec == 64 else 32, th)
+        if prec == "32c":
+            t.set_compensated(True)
         t.set_positions(X0 if prec == 64 else m.X)
         t.counters(reset=True)
         (_, ms_f) = _timed_call(t, lambda: t.trajectory(nsteps, dt, every))
@@ -290,7 +304,7 @@ def adjoint(nsteps: int, every: int, fd_cells: int, dt: float = 0.002, eps: floa
         log(f"[adjoint] differences: {c + 1}/{fd_cells} cells")
     out["jumps_in_ulp_scans"] = jumps
     log(f"[adjoint] quotients flagged as jumps in the 1-ulp scans: {jumps} of {fd.size * 16}")
-    for prec in (32, 64):
+    for prec in (32, "32c", 64):
         g = grads[prec][:fd_cells]
         for label, ref in (("central differences", fd), ("median 1-ulp quotients", med)):
             err = float(np.linalg.norm(g - ref) / np.linalg.norm(ref))
@@ -301,8 +315,10 @@ def adjoint(nsteps: int, every: int, fd_cells: int, dt: float = 0.002, eps: floa
             out[f"f{prec}"][f"rel_error_{key}"] = err
             out[f"f{prec}"][f"rel_error_{key}_per_kind"] = dict(zip(ADJ_NAMES, per))
     gd = float(np.linalg.norm(grads[32] - grads[64]) / np.linalg.norm(grads[64]))
-    log(f"[adjoint] f32 vs f64 adjoint: relative difference {gd:.3e}")
+    gdc = float(np.linalg.norm(grads["32c"] - grads[64]) / np.linalg.norm(grads[64]))
+    log(f"[adjoint] f32 vs f64 adjoint: relative difference {gd:.3e}; compensated f32 vs f64 {gdc:.3e}")
     out["f32_vs_f64_adjoint"] = gd
+    out["f32c_vs_f64_adjoint"] = gdc
     out["fd_cells"] = fd_cells
     print(json.dumps(out), flush=True)
     return out

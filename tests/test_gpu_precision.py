@@ -151,3 +151,30 @@ def test_energy_drift_f32_vs_f64():
         worst = max(worst, abs(e32 - e64) / abs(E0))
     print(f"1000-step energy drift |E32 - E64|/E0 <= {worst:.3e} (E0={E0:.4e})", flush=True)
     assert worst <= 1e-4
+
+
+def test_compensated_positions_track_the_double_run():
+    # smooth terms only: the float run with compensated positions stays far
+    # closer to the double run than the plain float run, and its update is
+    # the plain one where no rounding occurs (same forces from the same x)
+    m = W.monolayer3d(4, 2.0, seed=7)
+    mask = capi.DPM3D_VOLUME | capi.DPM3D_SURFACE | capi.DPM3D_BENDING
+    t32, t32c, t64 = tissue3d(m, mask, 32), tissue3d(m, mask, 32), tissue3d(m, mask, 64)
+    t32c.set_compensated(True)
+    t32.set_positions(m.X)
+    t32c.set_positions(m.X)
+    t64.set_positions(m.X.astype(np.float64))
+    for t in (t32, t32c, t64):
+        t.euler(1000, 0.002)
+    X64 = t64.positions()
+    e_plain = float(np.abs(t32.positions(np.float64) - X64).max())
+    e_comp = float(np.abs(t32c.positions(np.float64) - X64).max())
+    E64 = t64.energy().sum()
+    d_plain = abs(t32.energy().sum() - E64) / E64
+    d_comp = abs(t32c.energy().sum() - E64) / E64
+    print(f"1000 steps: max|x - x64| plain {e_plain:.3e} compensated {e_comp:.3e}; energy drift plain "
+          f"{d_plain:.3e} compensated {d_comp:.3e}", flush=True)
+    assert e_comp < 0.2 * e_plain
+    assert d_comp < d_plain
+    # the float accessor returns the float part, the double one x + residual
+    np.testing.assert_array_equal(t32c.positions(), t32c.positions(np.float64).astype(np.float32))
