@@ -14,7 +14,7 @@ Workload (one "step" = one overdamped Euler step of every vertex):
 `value` is the batched whole-job rate in vertex-steps/s with all inputs
 resident in HBM; the plain rate is reported beside it.
 
-roofline: the dominant (only) kernel of a step is k3d_step.  Algorithmic
+roofline: the dominant (only) kernel of a step is k3d_step_f32.  Algorithmic
 bytes per vertex-step = 24 (position read + write, float xyz) + 24/162
 (per-cell state amortised) = 24.15 B; `achieved` = that x vertices per
 launch / the average launch duration measured here with HIP events
@@ -124,7 +124,19 @@ def main():
     ap.add_argument("--launches", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device 5 times (the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copied 5 x 256 MiB")
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -190,7 +202,7 @@ def main():
         "winding_evals_per_step": windings,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "k3d_step", "launch_ms": b_launch, "bytes_per_vertex_step": B_ALG},
+                     "kernel": "k3d_step_f32", "launch_ms": b_launch, "bytes_per_vertex_step": B_ALG},
     }
     if not args.no_cpu:
         log(f"[bench] cpu baseline: {args.cpu_steps} step(s)")
