@@ -159,6 +159,15 @@ int dpm3d_get_forces_f64(dpm_tissue* h, double* xyz, size_t n);
  * it is the drift diagnostic of the precision study.  energy[ncells]. */
 int dpm3d_energy(dpm_tissue* h, double* energy);
 
+/* Candidate cells of the contact/adhesion search: 0 scans every cell of the
+ * tissue (the reference's all-pairs loops, Cell3D_Kernel.cl:273-310,
+ * 325-364, with exact culling); 1 bins the centres of mass each step
+ * (counting sort into hashed periodic bins no narrower than the step's
+ * largest candidate reach) and visits the 27 surrounding bins; -1 (default)
+ * chooses 1 for tissues of more than 8 x threads-per-cell cells.  Both give
+ * bit-identical results. */
+int dpm3d_set_cell_list(dpm_tissue* h, int mode);
+
 /* Differentiation form (no reference counterpart).  dpm3d_trajectory runs
  * nsteps Euler steps (as dpm_euler) and keeps the positions every `every`
  * steps (nsteps a multiple of every) on the device.  dpm3d_adjoint then

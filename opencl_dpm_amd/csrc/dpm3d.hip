@@ -120,6 +120,12 @@ struct Step3D {
   R* xlo;
   R* ylo;
   R* zlo;
+  // cell list of this step (nullptr: scan every cell of the tissue)
+  const int* bin_start;  // per tissue: [nbuck + 1] bucket offsets at tbin_hoff
+  const int* bin_list;   // cells of each tissue ordered by bucket
+  const int* tbin_nb;    // per tissue: bins per axis (0: scan)
+  const int* tbin_nbuck;
+  const int* tbin_hoff;
 };
 
 // a + b = s + e exactly (Knuth's TwoSum, any magnitudes).
@@ -141,17 +147,19 @@ __device__ __forceinline__ R cell_l0(const float* a0, int c) {
 
 // Culling reach added to a cell's bounding radius for the winding-number
 // test.  Every face of cell j lies inside its bounding sphere R_j, so a
-// point outside R_j + m_j is at least m_j from every face.  A triangle whose
-// edges are all <= e fits in a disc of radius e/sqrt(3); seen from distance
-// >= e it subtends at most 2*pi*(1 - sqrt(3/4)) ~ 0.84 sr, far from the
-// more-than-hemisphere (> pi) solid angle the reference skips
-// (denom < 1e-8).  With m_j = e_max(j) no face is skipped, the exact winding
-// number of such a point is 0, and its float evaluation (|Omega_f| small,
-// about 1e-7 relative rounding each) stays orders of magnitude below the
-// 1e-6 pair threshold — the pair is absent in the reference arithmetic too.
+// point p outside R_j + m_j is at least m_j from every face and edge of j,
+// and outside j's convex hull: its exact winding number is 0.  An edge of
+// length <= e seen from distance >= d subtends at most 2 atan(e / 2d); with
+// d >= 0.4 e that is <= 102.7 degrees, so every pair of the unit vectors
+// u, v, w toward a face's corners has a dot product >= -0.2195 and the
+// reference's denominator 1 + u.v + v.w + w.u (Cell3D_Kernel.cl:293) is
+// >= 0.34: no face is skipped (denom < 1e-8), and the float sum of the
+// face angles is rounding noise about a true 0, far below the 1e-6 pair
+// threshold (Cell3D_Kernel.cl:303) — the pair is absent in the reference
+// arithmetic too.  m_j = 0.4 e_max(j) (with slack for float rounding).
 template <typename R>
 __device__ __forceinline__ R contact_margin(R Rj, R emax_j) {
-  return emax_j * R(1.001f) + R(1e-6f) * Rj;
+  return emax_j * R(0.4004f) + R(1e-6f) * Rj;
 }
 
 // LDS layout (byte offsets), shared by host (sizing) and device (carving).
@@ -159,7 +167,7 @@ __device__ __forceinline__ R contact_margin(R Rj, R emax_j) {
 // them, so they share their bytes with the tables of the later passes
 // (bending, neighbour cell, winding directions).
 struct Lds3DLayout {
-  size_t own, faces, members, clist, plist, cnt, crad, red, nbr, bend, wnum, utab, fc, total;
+  size_t own, faces, members, clist, plist, glist, gath, cnt, crad, red, nbr, bend, wnum, utab, fc, total;
 };
 __host__ __device__ inline Lds3DLayout lds3d_layout(int nv, int nf, int nt, int ww, int rsize) {
   Lds3DLayout L;
@@ -172,10 +180,12 @@ __host__ __device__ inline Lds3DLayout lds3d_layout(int nv, int nf, int nt, int 
   o = (o + 3u) & ~(size_t)3u;
   L.clist = o;   o += 4u * (size_t)nt;
   L.plist = o;   o += 4u * (size_t)nt;
+  L.glist = o;   o += 4u * (size_t)nt;  // candidate cells gathered from the cell list
+  L.gath = o;    o += 4u * 96u;         // 27 buckets: ids, starts, prefix
   L.cnt = o;     o += 4u * 32u;
   L.crad = o;    o += 4u * (3u * DPM3D_MAXCLUS + 1u);  // radii (uint bits), member offsets, active list
   o = (o + 7u) & ~(size_t)7u;
-  L.red = o;     o += 8u * 32u;
+  L.red = o;     o += 8u * 64u;
   o = (o + 31u) & ~(size_t)31u;
   L.fc = o;  // union: shape pass | later passes
   const size_t fc_end = o + 9u * rs * (size_t)nf;
@@ -194,7 +204,7 @@ struct Lds3D {
   Q4<R>* utab;
   uint2* sf;
   unsigned short* smem_members;
-  int *clist, *plist, *cnt, *cbeg, *cact;
+  int *clist, *plist, *glist, *gath, *cnt, *cbeg, *cact;
   unsigned* crad;
   double* red;
 };
@@ -216,6 +226,8 @@ __device__ __forceinline__ Lds3D<R> carve3d(unsigned char* smem, int nv, int nf,
   l.smem_members = reinterpret_cast<unsigned short*>(smem + L.members);
   l.clist = reinterpret_cast<int*>(smem + L.clist);
   l.plist = reinterpret_cast<int*>(smem + L.plist);
+  l.glist = reinterpret_cast<int*>(smem + L.glist);
+  l.gath = reinterpret_cast<int*>(smem + L.gath);
   l.cnt = reinterpret_cast<int*>(smem + L.cnt);
   l.crad = reinterpret_cast<unsigned*>(smem + L.crad);
   l.cbeg = reinterpret_cast<int*>(smem + L.crad) + DPM3D_MAXCLUS;
@@ -266,16 +278,13 @@ __device__ void cell_stats(const Lds3D<R>& l, int nv, int nf, CellStat<R>* st_ou
     py += (double)l.sy[v];
     pz += (double)l.sz[v];
   }
-  px = block_sum(px, l.red);
-  py = block_sum(py, l.red);
-  pz = block_sum(pz, l.red);
+  block_sum3(px, py, pz, l.red);
   const V3<R> C = {(R)(px / nv), (R)(py / nv), (R)(pz / nv)};
   R r = R(0);
   for (int v = tid; v < nv; v += NT) {
     const V3<R> d = vsub(lds_v(l.sx, l.sy, l.sz, v), C);
     r = fmax(r, sqrt(vdot(d, d)));
   }
-  r = block_max(r, reinterpret_cast<R*>(l.red));
   double vol = 0.0;
   R e2 = R(0);
   for (int f = tid; f < nf; f += NT) {
@@ -288,8 +297,13 @@ __device__ void cell_stats(const Lds3D<R>& l, int nv, int nf, CellStat<R>* st_ou
     const V3<R> e0 = vsub(P1, P0), e1 = vsub(P2, P1), ee = vsub(P0, P2);
     e2 = fmax(e2, fmax(vdot(e0, e0), fmax(vdot(e1, e1), vdot(ee, ee))));
   }
-  vol = block_sum(vol, l.red);
-  e2 = block_max(e2, reinterpret_cast<R*>(l.red));
+  {  // the radius, volume and longest edge in one reduction (max of
+     // non-negative floats is exact in double)
+    double rd = (double)r, e2d = (double)e2;
+    block_max_sum_max(rd, vol, e2d, l.red);
+    r = (R)rd;
+    e2 = (R)e2d;
+  }
   if (tid == 0) {
     // Slack of a few ulp so that the culling tests are never tighter than the
     // largest distance a vertex can have from C, or the longest edge.
@@ -352,6 +366,115 @@ __device__ __forceinline__ V3<R> substrate_corner(V3<R> pos, V3<R> C, R Ks, R ks
     c = vadd(c, vscale(vnormalize(c2v), ftmp));
   }
   return c;
+}
+
+// ---------------------------------------------------------------- cell list
+//
+// Per step and tissue, the cells' centres of mass are binned on a periodic
+// grid of nb^3 bins of width L/nb >= the largest candidate reach of the
+// step, bound from the current radii, longest edges and l0:
+//   2 R_max + max(contact margin, 2 l0_max), with the step's own slack.
+// Any candidate pair then lies in neighbouring bins (minimum image), so the
+// 27 buckets around a cell hold every candidate; the exact predicate of the
+// scan decides, on the candidates sorted by cell index, so the step is
+// bit-identical to scanning every cell.  Bins are hashed into nbuck buckets
+// (a power of two; collisions only add candidates).  Tissues whose reach is
+// not small against L (nb < 3, or the all-images adhesion rule) or without
+// periodic boundaries scan every cell.
+__device__ __forceinline__ unsigned bin_bucket(int bx, int by, int bz, int nb, int nbuck) {
+  const unsigned key = (unsigned)bx + (unsigned)nb * ((unsigned)by + (unsigned)nb * (unsigned)bz);
+  if ((unsigned long long)nb * nb * nb <= (unsigned long long)nbuck) return key;
+  unsigned h = key * 2654435761u;
+  h ^= h >> 16;
+  return h & (unsigned)(nbuck - 1);
+}
+
+template <typename R>
+__device__ __forceinline__ int bin_coord(R x, R L, R wb, int nb) {
+  const R xw = x - L * floor(x / L);
+  const int b = (int)floor(xw / wb);
+  return b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+}
+
+template <typename R>
+struct Bin3D {
+  const int* tissue_begin;
+  const CellStat<R>* st_in;
+  const R* em_in;
+  const float* a0;
+  const float* tL;
+  const int* tpbc;
+  const float* tKre;
+  const float* tKat;
+  unsigned mask;
+  int* tbin_nb;
+  const int* tbin_nbuck;
+  const int* tbin_hoff;
+  int* bin_start;
+  int* bin_list;
+};
+
+// One workgroup per tissue; LDS: nbuck counters + bucket and rank per cell.
+template <typename R>
+__device__ void k3d_bin_body(const Bin3D<R>& a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int t = blockIdx.x, tid = threadIdx.x, NT = blockDim.x;
+  const int cb = a.tissue_begin[t], ce = a.tissue_begin[t + 1], nc = ce - cb;
+  const int nbuck = a.tbin_nbuck[t];
+  int* cnt = reinterpret_cast<int*>(smem);
+  int* cbk = cnt + nbuck + 1;
+  int* crk = cbk + nc;
+  double* red = reinterpret_cast<double*>(smem + (((size_t)(nbuck + 1 + 2 * nc) * 4 + 7) & ~(size_t)7));
+  const R L = (R)a.tL[t];
+  const bool do_contact = (a.mask & M_CONTACT) && a.tKre[t] != 0.0f;
+  const bool do_adh = (a.mask & M_ADHESION) && a.tKat[t] != 0.0f;
+  R rmax = R(0), emax = R(0), l0max = R(0);
+  for (int k = tid; k < nc; k += NT) {
+    rmax = fmax(rmax, a.st_in[cb + k].r);
+    emax = fmax(emax, a.em_in[cb + k]);
+    l0max = fmax(l0max, fmax(cell_l0<R>(a.a0, cb + k), R(1e-12f)));
+  }
+  rmax = block_max(rmax, reinterpret_cast<R*>(red));
+  emax = block_max(emax, reinterpret_cast<R*>(red));
+  l0max = block_max(l0max, reinterpret_cast<R*>(red));
+  R reach = R(0);
+  if (do_contact) reach = R(2) * rmax + contact_margin(rmax, emax);
+  bool all_images = false;
+  if (do_adh) {
+    const R r2 = R(2) * rmax + R(2) * l0max;
+    reach = fmax(reach, r2);
+    all_images = R(2) * r2 >= R(0.999f) * L;
+  }
+  reach = (reach * R(1.0001f) + R(1e-6f)) * R(1.001f);  // the step's slack, and more
+  int nb = (int)floor(L / reach);
+  if (!a.tpbc[t] || all_images || nb < 3 || !(do_contact || do_adh) || nbuck == 0) nb = 0;
+  if (nb > 1024) nb = 1024;
+  if (tid == 0) a.tbin_nb[t] = nb;
+  if (nb == 0) return;  // block-uniform
+  const R wb = L / (R)nb;
+  for (int b = tid; b <= nbuck; b += NT) cnt[b] = 0;
+  __syncthreads();
+  for (int k = tid; k < nc; k += NT) {
+    const CellStat<R> s = a.st_in[cb + k];
+    const unsigned b = bin_bucket(bin_coord(s.cx, L, wb, nb), bin_coord(s.cy, L, wb, nb),
+                                  bin_coord(s.cz, L, wb, nb), nb, nbuck);
+    cbk[k] = (int)b;
+    crk[k] = atomicAdd(&cnt[b], 1);
+  }
+  __syncthreads();
+  if (tid == 0) {  // exclusive scan (nbuck <= 8192)
+    int run = 0;
+    for (int b = 0; b < nbuck; b++) {
+      const int c = cnt[b];
+      cnt[b] = run;
+      run += c;
+    }
+    cnt[nbuck] = run;
+  }
+  __syncthreads();
+  int* bs = a.bin_start + a.tbin_hoff[t];
+  for (int b = tid; b <= nbuck; b += NT) bs[b] = cnt[b];
+  for (int k = tid; k < nc; k += NT) a.bin_list[cb + cnt[cbk[k]] + crk[k]] = cb + k;
 }
 
 template <typename R>
@@ -548,8 +671,70 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
   if (do_contact || do_adh) {  // block-uniform
     const R L0i = fmax(l0i, R(1e-12f));
     Q4<R>* ut = l.utab + (size_t)wave * nv;
-    for (int c0 = cb; c0 < ce; c0 += NT) {
-      const int cj = c0 + tid;
+    // candidate source: every cell of the tissue in order, or the cells of
+    // the 27 buckets around this cell, sorted by index (block-uniform)
+    const int* src = nullptr;
+    int nsrc = ce - cb;
+    if (a.bin_list && a.tbin_nb[t] > 0) {
+      const int nb = a.tbin_nb[t], nbuck = a.tbin_nbuck[t];
+      const int* bs = a.bin_start + a.tbin_hoff[t];
+      const R wb = L / (R)nb;
+      int* gid = l.gath;        // [27] bucket ids
+      int* gst = l.gath + 32;   // [27] bucket starts
+      int* gof = l.gath + 64;   // [28] prefix of bucket sizes
+      if (tid < 27) {
+        const int bx = bin_coord(Ci.x, L, wb, nb), by = bin_coord(Ci.y, L, wb, nb),
+                  bz = bin_coord(Ci.z, L, wb, nb);
+        const int dx = tid % 3 - 1, dy = (tid / 3) % 3 - 1, dz = tid / 9 - 1;
+        gid[tid] = (int)bin_bucket((bx + dx + nb) % nb, (by + dy + nb) % nb, (bz + dz + nb) % nb, nb, nbuck);
+      }
+      __syncthreads();
+      if (tid == 0) {  // distinct buckets, their cells' offsets
+        for (int i = 1; i < 27; i++) {
+          const int v = gid[i];
+          int j = i - 1;
+          while (j >= 0 && gid[j] > v) {
+            gid[j + 1] = gid[j];
+            j--;
+          }
+          gid[j + 1] = v;
+        }
+        int nu = 0, run = 0;
+        for (int i = 0; i < 27; i++) {
+          if (i > 0 && gid[i] == gid[i - 1]) continue;
+          gst[nu] = bs[gid[i]];
+          gof[nu] = run;
+          run += bs[gid[i] + 1] - bs[gid[i]];
+          nu++;
+        }
+        gof[nu] = run;
+        gid[31] = nu;
+      }
+      __syncthreads();
+      const int nu = gid[31], n = gof[nu];
+      if (n <= NT) {  // block-uniform; otherwise scan every cell
+        if (tid < n) {
+          int u = 0;
+          while (gof[u + 1] <= tid) u++;
+          l.glist[tid] = a.bin_list[cb + gst[u] + (tid - gof[u])];
+        }
+        __syncthreads();
+        if (tid < n) {  // rank sort (indices are distinct)
+          const int v = l.glist[tid];
+          int r = 0;
+          for (int u = 0; u < n; u++) r += l.glist[u] < v;
+          l.clist[r] = v;
+        }
+        __syncthreads();
+        if (tid < n) l.glist[tid] = l.clist[tid];
+        __syncthreads();
+        src = l.glist;
+        nsrc = n;
+      }
+    }
+    for (int c0 = 0; c0 < nsrc; c0 += NT) {
+      const int k = c0 + tid;
+      const int cj = k < nsrc ? (src ? src[k] : cb + k) : ce;
       bool cand = false;
       if (cj < ce && cj != ci) {
         const CellStat<R> stj = a.st_in[cj];
@@ -576,6 +761,13 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
       }
       const int ncand = block_compact(cand, cj, l.clist, l.cnt);
       n_cand += ncand;
+      // each thread stages one vertex (NT >= nv) of a candidate; the next
+      // candidate's vertex is loaded while the current one is processed
+      V3<R> pre = zero;
+      if (ncand > 0 && own) {
+        const size_t b0 = (size_t)l.clist[0] * nv + tid;
+        pre = v3(a.xin[b0], a.yin[b0], a.zin[b0]);
+      }
       for (int q = 0; q < ncand; q++) {
         const int cj2 = l.clist[q];
         const CellStat<R> stj = a.st_in[cj2];
@@ -585,29 +777,35 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
           sh.y = L * round((Ci.y - stj.cy) / L);
           sh.z = L * round((Ci.z - stj.cz) / L);
         }
-        const size_t bj = (size_t)cj2 * nv;
-        for (int v = tid; v < nv; v += NT) {
-          l.nx[v] = a.xin[bj + v];
-          l.ny[v] = a.yin[bj + v];
-          l.nz[v] = a.zin[bj + v];
+        if (own) {
+          l.nx[tid] = pre.x;
+          l.ny[tid] = pre.y;
+          l.nz[tid] = pre.z;
         }
-        if (tid < DPM3D_MAXCLUS) l.crad[tid] = 0u;
+        if (do_adh && tid < DPM3D_MAXCLUS) l.crad[tid] = 0u;
         __syncthreads();
-        if (do_adh && own) {
-          // radius of each vertex cluster about its seed vertex (as a float:
-          // the test below is a cull with slack, not a pair decision)
-          const int k = a.clus_of[tid];
-          const V3<R> d = vsub(lds_v(l.nx, l.ny, l.nz, tid), lds_v(l.nx, l.ny, l.nz, k));
-          atomicMax(&l.crad[k], __float_as_uint((float)sqrt(vdot(d, d)) * 1.000001f));
+        if (q + 1 < ncand && own) {
+          const size_t b1 = (size_t)l.clist[q + 1] * nv + tid;
+          pre = v3(a.xin[b1], a.yin[b1], a.zin[b1]);
         }
-        __syncthreads();
+        if (do_adh) {
+          if (own) {
+            // radius of each vertex cluster about its seed vertex (as a
+            // float: the adhesion culls below have slack, they decide no pair)
+            const int k = a.clus_of[tid];
+            const V3<R> d = vsub(lds_v(l.nx, l.ny, l.nz, tid), lds_v(l.nx, l.ny, l.nz, k));
+            atomicMax(&l.crad[k], __float_as_uint((float)sqrt(vdot(d, d)) * 1.000001f));
+          }
+          __syncthreads();
+        }
         if (do_contact) {
           const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
           const R rj = (stj.r + contact_margin(stj.r, a.em_in[cj2])) * R(1.0001f) + R(1e-6f);
           bool need = false;
           if (own) {
             const V3<R> d = vsub(P, Cj);
-            need = vdot(d, d) <= rj * rj;
+            const R d2 = vdot(d, d);
+            need = d2 <= rj * rj;
           }
           const int np = block_compact(need, tid, l.plist, l.cnt);
           n_wind += np;
@@ -1260,3 +1458,5 @@ extern "C" __global__ void k3d_vjp_cross_f64(double* lx, double* ly, double* lz,
 }
 extern "C" __global__ void __launch_bounds__(1024) k3d_vjp_f32(const Vjp3D<float> a) { k3d_vjp_body(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_vjp_f64(const Vjp3D<double> a) { k3d_vjp_body(a); }
+extern "C" __global__ void __launch_bounds__(1024) k3d_bin_f32(const Bin3D<float> a) { k3d_bin_body(a); }
+extern "C" __global__ void __launch_bounds__(1024) k3d_bin_f64(const Bin3D<double> a) { k3d_bin_body(a); }

@@ -117,6 +117,12 @@ struct dpm_tissue {
   size_t ck_cap = 0, seg_cap = 0, segw_cap = 0, lam_cap = 0, segwp_cap = 0, segsh_cap = 0, cross_cap = 0,
          segcell_cap = 0;
   void* lo = nullptr;  // compensated positions: residuals [3 * NVT] (nullptr: off)
+  // cell list (counting sort of the centres of mass into hashed bins)
+  int cell_list_mode = -1;  // -1 automatic, 0 off, 1 on
+  bool cells_on = false;
+  int *bin_start = nullptr, *bin_list = nullptr, *tbin_nb = nullptr, *tbin_nbuck = nullptr,
+      *tbin_hoff = nullptr;
+  size_t bin_lds = 0;
 
   // ------------------------------------------------------------------ 2D
   int V = 0, NT2 = 0, NTr = 0;  // Euler threads, relax threads per tissue
@@ -380,6 +386,28 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
   rc |= A(&h->clus_begin, (size_t)K + 1);
   rc |= A(&h->clus_members, (size_t)nv);
   h->nclus = K;
+  // cell-list buckets per tissue: a power of two >= 2 cells (<= 8192; larger
+  // tissues scan every cell)
+  std::vector<int> nbuck(T), hoff(T);
+  size_t nbtot = 0, maxb = 0;
+  int ncmax_t = 0;
+  for (int t = 0; t < T; t++) {
+    const int nc = d->cell_begin[t + 1] - d->cell_begin[t];
+    ncmax_t = std::max(ncmax_t, nc);
+    int nbk = 64;
+    while (nbk < 2 * nc) nbk *= 2;
+    if (nbk > 8192) nbk = 0;
+    nbuck[t] = nbk;
+    hoff[t] = (int)nbtot;
+    nbtot += (size_t)nbk + 1;
+    maxb = std::max(maxb, (((size_t)nbk + 1 + 2 * (size_t)nc) * 4 + 7) / 8 * 8);
+  }
+  h->bin_lds = maxb + 32 * 8;
+  rc |= A(&h->bin_start, nbtot);
+  rc |= A(&h->bin_list, (size_t)C);
+  rc |= A(&h->tbin_nb, (size_t)T);
+  rc |= A(&h->tbin_nbuck, (size_t)T);
+  rc |= A(&h->tbin_hoff, (size_t)T);
   if (rc) { delete h; return DPM_ENOMEM; }
   h->tissue_begin_h.assign(d->cell_begin, d->cell_begin + T + 1);
   h->faces_h.assign(d->faces, d->faces + 3 * nf);
@@ -396,6 +424,12 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
   rc |= upload(h->clus_of, clus_of.data(), (size_t)nv, s);
   rc |= upload(h->clus_begin, clus_begin.data(), (size_t)K + 1, s);
   rc |= upload(h->clus_members, clus_members.data(), (size_t)nv, s);
+  rc |= upload(h->tbin_nbuck, nbuck.data(), (size_t)T, s);
+  rc |= upload(h->tbin_hoff, hoff.data(), (size_t)T, s);
+  // automatic: the cell list pays only where a cell would scan many chunks
+  // (at 4096 cells x 642 vertices it measured 2.57 ms against the scan's
+  // 2.43 ms per step)
+  h->cells_on = ncmax_t > 8 * h->NT;
   HIPCHECK(hipMemsetAsync(h->fx, 0, NVT * rs, s));
   HIPCHECK(hipMemsetAsync(h->fy, 0, NVT * rs, s));
   HIPCHECK(hipMemsetAsync(h->fz, 0, NVT * rs, s));
@@ -415,10 +449,26 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
                             {(const void*)k3d_step_f64, (const void*)k3d_stats_f64, (const void*)k3d_energy_f64}};
   for (const void* k : kern[rs == 8 ? 1 : 0])
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
+  (void)hipFuncSetAttribute(rs == 8 ? (const void*)k3d_bin_f64 : (const void*)k3d_bin_f32,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->bin_lds);
   h->lds_vjp = ldsvjp_layout(nv, nf, rs).total;
   (void)hipFuncSetAttribute(rs == 8 ? (const void*)k3d_vjp_f64 : (const void*)k3d_vjp_f32,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_vjp);
   *out = h;
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_set_cell_list(dpm_tissue* h, int mode) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  if (mode < -1 || mode > 1) return fail(DPM_EINVAL, "mode must be -1 (automatic), 0 or 1");
+  int ncmax_t = 0;
+  for (int t = 0; t < h->T; t++) ncmax_t = std::max(ncmax_t, h->tissue_begin_h[t + 1] - h->tissue_begin_h[t]);
+  h->cell_list_mode = mode;
+  h->cells_on = mode == 1 || (mode == -1 && ncmax_t > 8 * h->NT);
+  if (h->graph) {  // captured with the old arguments
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
   return DPM_OK;
 }
 
@@ -639,6 +689,13 @@ static Step3D<R> step_args(dpm_tissue* h, int in, float dt, int flags) {
     a.ylo = (R*)h->lo + (size_t)h->C * h->nv;
     a.zlo = (R*)h->lo + 2 * (size_t)h->C * h->nv;
   }
+  if (h->cells_on) {
+    a.bin_start = h->bin_start;
+    a.bin_list = h->bin_list;
+    a.tbin_nb = h->tbin_nb;
+    a.tbin_nbuck = h->tbin_nbuck;
+    a.tbin_hoff = h->tbin_hoff;
+  }
   return a;
 }
 
@@ -657,6 +714,27 @@ static int launch3d(dpm_tissue* h, Kind3D kind, int in, float dt, int flags, Edi
       k = kind == K_STEP ? k3d_step_f64 : kind == K_STATS ? k3d_stats_f64 : k3d_energy_f64;
     else
       k = kind == K_STEP ? k3d_step_f32 : kind == K_STATS ? k3d_stats_f32 : k3d_energy_f32;
+    if (kind == K_STEP && a.bin_list) {  // this step's cell list, from its input statistics
+      Bin3D<R> b{};
+      b.tissue_begin = h->tissue_begin;
+      b.st_in = a.st_in;
+      b.em_in = a.em_in;
+      b.a0 = h->a0;
+      b.tL = h->tL;
+      b.tpbc = h->tpbc;
+      b.tKre = h->tKre;
+      b.tKat = h->tKat;
+      b.mask = h->mask;
+      b.tbin_nb = h->tbin_nb;
+      b.tbin_nbuck = h->tbin_nbuck;
+      b.tbin_hoff = h->tbin_hoff;
+      b.bin_start = h->bin_start;
+      b.bin_list = h->bin_list;
+      void (*kb)(Bin3D<R>) = nullptr;
+      if constexpr (sizeof(R) == 8) kb = k3d_bin_f64; else kb = k3d_bin_f32;
+      hipLaunchKernelGGL(kb, dim3(h->T), dim3(256), h->bin_lds, h->stream, b);
+      HIPCHECK(hipGetLastError());
+    }
     hipLaunchKernelGGL(k, dim3(h->C), dim3(h->NT), h->lds, h->stream, a);
     HIPCHECK(hipGetLastError());
     return DPM_OK;

@@ -102,6 +102,72 @@ __device__ __forceinline__ double block_sum(double v, double* scratch) {
   __syncthreads();
   return r;
 }
+// Three block-wide sums in one pass (every thread gets them); scratch: >= 52
+// doubles of LDS.  Contains __syncthreads.
+__device__ __forceinline__ void block_sum3(double& a, double& b, double& c, double* scratch) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
+  const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  if (lane == 0) {
+    scratch[wave] = a;
+    scratch[16 + wave] = b;
+    scratch[32 + wave] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0.0, sb = 0.0, sc = 0.0;
+    for (int w = 0; w < nw; w++) {
+      sa += scratch[w];
+      sb += scratch[16 + w];
+      sc += scratch[32 + w];
+    }
+    scratch[48] = sa;
+    scratch[49] = sb;
+    scratch[50] = sc;
+  }
+  __syncthreads();
+  a = scratch[48];
+  b = scratch[49];
+  c = scratch[50];
+  __syncthreads();
+}
+
+// max, sum, max of three per-thread values in one pass (every thread gets
+// them); scratch: >= 52 doubles of LDS.  Contains __syncthreads.
+__device__ __forceinline__ void block_max_sum_max(double& m1, double& s, double& m2, double* scratch) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
+  const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
+  for (int o = 32; o > 0; o >>= 1) {
+    m1 = fmax(m1, __shfl_down(m1, o, DPM_WAVE));
+    s += __shfl_down(s, o, DPM_WAVE);
+    m2 = fmax(m2, __shfl_down(m2, o, DPM_WAVE));
+  }
+  if (lane == 0) {
+    scratch[wave] = m1;
+    scratch[16 + wave] = s;
+    scratch[32 + wave] = m2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = scratch[0], b = 0.0, c = scratch[32];
+    for (int w = 0; w < nw; w++) {
+      a = fmax(a, scratch[w]);
+      b += scratch[16 + w];
+      c = fmax(c, scratch[32 + w]);
+    }
+    scratch[48] = a;
+    scratch[49] = b;
+    scratch[50] = c;
+  }
+  __syncthreads();
+  m1 = scratch[48];
+  s = scratch[49];
+  m2 = scratch[50];
+  __syncthreads();
+}
+
 template <typename R>
 __device__ __forceinline__ R block_max(R v, R* scratch) {
   const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;

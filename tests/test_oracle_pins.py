@@ -399,6 +399,44 @@ def test_winding_matches_independent_solid_angles():
         assert abs(oracle.winding(X, mesh.faces, mesh.nv, 0, p) - w) < 1e-12
 
 
+@pytest.mark.parametrize("subdiv,noise", [(1, 0.03), (2, 0.02), (2, 0.05), (3, 0.01)])
+def test_contact_culling_margin(subdiv, noise):
+    # The step skips the winding number of a point farther than
+    # R_j + 0.4 e_max from a cell's centre.  Lemma behind it: every point at
+    # least 0.4 e_max from the convex hull (here: beyond the support plane in
+    # its own direction) sees each face with denominator >= 0.34, so no face
+    # is skipped and the float winding number is noise far below the 1e-6
+    # pair threshold.  Points on the boundary of the culled region, in float.
+    mesh, X = ico(subdiv, 1.0, (0.3, -0.2, 1.1), seed=subdiv + 17, noise=noise)
+    C = X.mean(0)
+    F = mesh.faces
+    e = max(np.linalg.norm(X[F[:, a]] - X[F[:, b]], axis=1).max() for a, b in ((0, 1), (1, 2), (2, 0)))
+    rng = np.random.Generator(np.random.PCG64(subdiv))
+    u = rng.normal(size=(300, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    h = ((X - C) @ u.T).max(0)  # support of the vertex set
+    def worst(f):
+        dmin, wmax = np.inf, 0.0
+        for k in range(len(u)):
+            p = C + u[k] * (h[k] + f * e)
+            a, b, c = (X[F[:, i]] - p for i in range(3))
+            ua, ub, uc = (x / np.linalg.norm(x, axis=1, keepdims=True) for x in (a, b, c))
+            den = 1 + np.einsum("ij,ij->i", ua, ub) + np.einsum("ij,ij->i", ub, uc) + \
+                np.einsum("ij,ij->i", uc, ua)
+            dmin = min(dmin, den.min())
+            w = oracle.winding(X.astype(np.float32), F, mesh.nv, 0, p.astype(np.float32), prec=32)
+            wmax = max(wmax, abs(w))
+        return dmin, wmax
+
+    dmin, wmax = worst(0.4)
+    print(f"subdiv {subdiv}: at 0.4 e_max min denominator {dmin:.3f}, max |float winding| {wmax:.2e}")
+    assert dmin >= 0.34
+    assert wmax < 1e-6
+    # closer than that, faces are skipped and the winding number is not 0
+    dmin, wmax = worst(0.05)
+    assert dmin < 1e-8 and wmax > 1e-2
+
+
 def test_contact_pairs_of_overlapping_cells():
     mesh = W.icosphere(2)
     X = np.concatenate([mesh.unit * 1.0, mesh.unit * 1.0 + [1.6, 0.0, 0.0]]).astype(np.float64)
