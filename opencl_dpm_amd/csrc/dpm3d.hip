@@ -32,7 +32,6 @@
 using namespace dpm;
 
 #define DPM3D_MAXDEG 8
-#define DPM3D_MAXCLUS 64
 #define DPM3D_MAXPART 4
 
 enum : unsigned {
@@ -51,6 +50,7 @@ enum : int {
 template <typename R>
 struct CellStat {
   R cx, cy, cz, r;  // centre of mass, bounding radius
+  R em, l0, pad0, pad1;  // longest edge, target edge length l0
 };
 
 template <typename R>
@@ -91,10 +91,6 @@ struct Step3D {
   const int* faces;         // [nf*3]
   const int* vf;            // [nv*MAXDEG] incident (face*3 + corner), ascending, -1 pad
   const int* adj;           // [nv*MAXDEG] neighbour vertices, ascending, -1 pad
-  const int* clus_of;       // [nv] cluster of each vertex (cluster k is seeded at vertex k)
-  const int* clus_begin;    // [nclus+1]
-  const int* clus_members;  // [nv] vertices grouped by cluster
-  int nclus;
   int nv, nf, ncells;
   int ww;  // waves that evaluate winding numbers (LDS tables)
   R dt;
@@ -145,6 +141,19 @@ __device__ __forceinline__ R cell_l0(const float* a0, int c) {
   return sqrt(R(4) * (R)a0[c]) / sqrt(R(3));
 }
 
+// Periodic shift of cell j toward cell i along one axis, L round(d / L)
+// with d = C_i - C_j (Cell3D_Kernel.cl:279-281).  The quotient is taken as
+// a product with 1/L (within two ulp of the quotient); only where that
+// product lies within 1e-5 (relative) of a half-integer can the two round
+// differently, and there the division itself decides.
+template <typename R>
+__device__ __forceinline__ R image_shift(R d, R L, R invL) {
+  const R q = d * invL;
+  R n = round(q);
+  if (fabs(fabs(q - n) - R(0.5f)) < R(1e-5f) * (R(1) + fabs(q))) n = round(d / L);
+  return L * n;
+}
+
 // Culling reach added to a cell's bounding radius for the winding-number
 // test.  Every face of cell j lies inside its bounding sphere R_j, so a
 // point p outside R_j + m_j is at least m_j from every face and edge of j,
@@ -167,7 +176,7 @@ __device__ __forceinline__ R contact_margin(R Rj, R emax_j) {
 // them, so they share their bytes with the tables of the later passes
 // (bending, neighbour cell, winding directions).
 struct Lds3DLayout {
-  size_t own, faces, members, clist, plist, glist, gath, cnt, crad, red, nbr, bend, wnum, utab, fc, total;
+  size_t own, faces, clist, plist, glist, gath, cnt, red, nbr, bend, wnum, utab, fc, total;
 };
 __host__ __device__ inline Lds3DLayout lds3d_layout(int nv, int nf, int nt, int ww, int rsize) {
   Lds3DLayout L;
@@ -176,14 +185,12 @@ __host__ __device__ inline Lds3DLayout lds3d_layout(int nv, int nf, int nt, int 
   L.own = o;     o += 3u * rs * (size_t)nv;
   o = (o + 7u) & ~(size_t)7u;
   L.faces = o;   o += 8u * (size_t)nf;  // (i0 | i1 << 16, i2) per face
-  L.members = o; o += 2u * (size_t)nv;
   o = (o + 3u) & ~(size_t)3u;
   L.clist = o;   o += 4u * (size_t)nt;
-  L.plist = o;   o += 4u * (size_t)nt;
+  L.plist = o;   o += 8u * (size_t)nt;  // two lists of nt
   L.glist = o;   o += 4u * (size_t)nt;  // candidate cells gathered from the cell list
   L.gath = o;    o += 4u * 96u;         // 27 buckets: ids, starts, prefix
-  L.cnt = o;     o += 4u * 32u;
-  L.crad = o;    o += 4u * (3u * DPM3D_MAXCLUS + 1u);  // radii (uint bits), member offsets, active list
+  L.cnt = o;     o += 4u * 48u;
   o = (o + 7u) & ~(size_t)7u;
   L.red = o;     o += 8u * 64u;
   o = (o + 31u) & ~(size_t)31u;
@@ -194,6 +201,10 @@ __host__ __device__ inline Lds3DLayout lds3d_layout(int nv, int nf, int nt, int 
   L.wnum = o;    o += rs * (size_t)nv;
   o = (o + 31u) & ~(size_t)31u;
   L.utab = o;    o += 4u * rs * (size_t)nv * (size_t)ww;
+  // the adhesion pass's per-(slice, vertex) partial sums share the winding
+  // directions' bytes (the two passes are separated by barriers)
+  const size_t adh_end = L.utab + 4u * rs * (size_t)nt;
+  if (o < adh_end) o = adh_end;
   L.total = o > fc_end ? o : fc_end;
   return L;
 }
@@ -203,9 +214,7 @@ struct Lds3D {
   R *sx, *sy, *sz, *nx, *ny, *nz, *sb, *wb, *fcx, *fcy, *fcz;
   Q4<R>* utab;
   uint2* sf;
-  unsigned short* smem_members;
-  int *clist, *plist, *glist, *gath, *cnt, *cbeg, *cact;
-  unsigned* crad;
+  int *clist, *plist, *glist, *gath, *cnt;
   double* red;
 };
 template <typename R>
@@ -223,15 +232,11 @@ __device__ __forceinline__ Lds3D<R> carve3d(unsigned char* smem, int nv, int nf,
   l.sb = reinterpret_cast<R*>(smem + L.bend);
   l.wb = reinterpret_cast<R*>(smem + L.wnum);
   l.sf = reinterpret_cast<uint2*>(smem + L.faces);
-  l.smem_members = reinterpret_cast<unsigned short*>(smem + L.members);
   l.clist = reinterpret_cast<int*>(smem + L.clist);
   l.plist = reinterpret_cast<int*>(smem + L.plist);
   l.glist = reinterpret_cast<int*>(smem + L.glist);
   l.gath = reinterpret_cast<int*>(smem + L.gath);
   l.cnt = reinterpret_cast<int*>(smem + L.cnt);
-  l.crad = reinterpret_cast<unsigned*>(smem + L.crad);
-  l.cbeg = reinterpret_cast<int*>(smem + L.crad) + DPM3D_MAXCLUS;
-  l.cact = l.cbeg + DPM3D_MAXCLUS + 1;
   l.utab = reinterpret_cast<Q4<R>*>(smem + L.utab);
   l.fcx = reinterpret_cast<R*>(smem + L.fc);
   l.fcy = l.fcx + 3 * nf;
@@ -270,7 +275,7 @@ __device__ __forceinline__ void face_of(const Lds3D<R>& l, int f, int& i0, int& 
 // origin.
 template <typename R>
 __device__ void cell_stats(const Lds3D<R>& l, int nv, int nf, CellStat<R>* st_out,
-                           double* vol_out, R* em_out, int ci) {
+                           double* vol_out, R* em_out, int ci, R l0c) {
   const int tid = threadIdx.x, NT = blockDim.x;
   double px = 0.0, py = 0.0, pz = 0.0;
   for (int v = tid; v < nv; v += NT) {
@@ -307,9 +312,10 @@ __device__ void cell_stats(const Lds3D<R>& l, int nv, int nf, CellStat<R>* st_ou
   if (tid == 0) {
     // Slack of a few ulp so that the culling tests are never tighter than the
     // largest distance a vertex can have from C, or the longest edge.
-    st_out[ci] = CellStat<R>{C.x, C.y, C.z, r * R(1.0000005f) + R(1e-30f)};
+    const R em = sqrt(e2) * R(1.000001f);
+    st_out[ci] = CellStat<R>{C.x, C.y, C.z, r * R(1.0000005f) + R(1e-30f), em, l0c, R(0), R(0)};
     vol_out[ci] = fabs(vol / 6.0);
-    em_out[ci] = sqrt(e2) * R(1.000001f);
+    em_out[ci] = em;
   }
 }
 
@@ -319,8 +325,6 @@ __device__ __forceinline__ void stage_topology(const Step3D<R>& a, const Lds3D<R
   for (int f = tid; f < a.nf; f += NT)
     l.sf[f] = make_uint2((unsigned)a.faces[3 * f] | ((unsigned)a.faces[3 * f + 1] << 16),
                          (unsigned)a.faces[3 * f + 2]);
-  for (int i = tid; i < a.nv; i += NT) l.smem_members[i] = (unsigned short)a.clus_members[i];
-  if (tid <= a.nclus) l.cbeg[tid] = a.clus_begin[tid];
 }
 
 template <typename R>
@@ -485,7 +489,7 @@ __device__ void k3d_stats_body(const Step3D<R>& a) {
   stage_topology(a, l);
   load_own(a, l, (size_t)ci * a.nv);
   __syncthreads();
-  cell_stats(l, a.nv, a.nf, a.st_out, a.vol_out, a.em_out, ci);
+  cell_stats(l, a.nv, a.nf, a.st_out, a.vol_out, a.em_out, ci, cell_l0<R>(a.a0, ci));
 }
 
 // Winding number of point p about the staged cell (l.nx/ny/nz shifted by sh),
@@ -670,6 +674,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
   int npart = 0;
   if (do_contact || do_adh) {  // block-uniform
     const R L0i = fmax(l0i, R(1e-12f));
+    const R invL = R(1) / L;
     Q4<R>* ut = l.utab + (size_t)wave * nv;
     // candidate source: every cell of the tissue in order, or the cells of
     // the 27 buckets around this cell, sorted by index (block-uniform)
@@ -739,17 +744,14 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
       if (cj < ce && cj != ci) {
         const CellStat<R> stj = a.st_in[cj];
         V3<R> sh = zero;
-        if (pbc) {
-          sh.x = L * round((Ci.x - stj.cx) / L);
-          sh.y = L * round((Ci.y - stj.cy) / L);
-          sh.z = L * round((Ci.z - stj.cz) / L);
-        }
+        if (pbc)
+          sh = v3(image_shift(Ci.x - stj.cx, L, invL), image_shift(Ci.y - stj.cy, L, invL),
+                  image_shift(Ci.z - stj.cz, L, invL));
         const V3<R> d = {stj.cx + sh.x - Ci.x, stj.cy + sh.y - Ci.y, stj.cz + sh.z - Ci.z};
-        const R dist = sqrt(vdot(d, d));
         R reach = R(0);
-        if (do_contact) reach = Ri + stj.r + contact_margin(stj.r, a.em_in[cj]);
+        if (do_contact) reach = Ri + stj.r + contact_margin(stj.r, stj.em);
         if (do_adh) {
-          const R l0m = fmax(L0i, fmax(cell_l0<R>(a.a0, cj), R(1e-12f)));
+          const R l0m = fmax(L0i, fmax(stj.l0, R(1e-12f)));
           const R r2 = Ri + stj.r + R(2) * l0m;
           reach = fmax(reach, r2);
           // Adhesion takes the minimum image per vertex pair
@@ -757,7 +759,8 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
           // centre-of-mass image when the box is small: then no culling.
           if (pbc && R(2) * r2 >= R(0.999f) * L) cand = true;
         }
-        cand = cand || dist <= reach * R(1.0001f) + R(1e-6f);
+        const R rr = reach * R(1.0001f) + R(1e-6f);
+        cand = cand || vdot(d, d) <= rr * rr;
       }
       const int ncand = block_compact(cand, cj, l.clist, l.cnt);
       n_cand += ncand;
@@ -772,35 +775,22 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
         const int cj2 = l.clist[q];
         const CellStat<R> stj = a.st_in[cj2];
         V3<R> sh = zero;
-        if (pbc) {
-          sh.x = L * round((Ci.x - stj.cx) / L);
-          sh.y = L * round((Ci.y - stj.cy) / L);
-          sh.z = L * round((Ci.z - stj.cz) / L);
-        }
+        if (pbc)
+          sh = v3(image_shift(Ci.x - stj.cx, L, invL), image_shift(Ci.y - stj.cy, L, invL),
+                  image_shift(Ci.z - stj.cz, L, invL));
         if (own) {
           l.nx[tid] = pre.x;
           l.ny[tid] = pre.y;
           l.nz[tid] = pre.z;
         }
-        if (do_adh && tid < DPM3D_MAXCLUS) l.crad[tid] = 0u;
         __syncthreads();
         if (q + 1 < ncand && own) {
           const size_t b1 = (size_t)l.clist[q + 1] * nv + tid;
           pre = v3(a.xin[b1], a.yin[b1], a.zin[b1]);
         }
-        if (do_adh) {
-          if (own) {
-            // radius of each vertex cluster about its seed vertex (as a
-            // float: the adhesion culls below have slack, they decide no pair)
-            const int k = a.clus_of[tid];
-            const V3<R> d = vsub(lds_v(l.nx, l.ny, l.nz, tid), lds_v(l.nx, l.ny, l.nz, k));
-            atomicMax(&l.crad[k], __float_as_uint((float)sqrt(vdot(d, d)) * 1.000001f));
-          }
-          __syncthreads();
-        }
         if (do_contact) {
           const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
-          const R rj = (stj.r + contact_margin(stj.r, a.em_in[cj2])) * R(1.0001f) + R(1e-6f);
+          const R rj = (stj.r + contact_margin(stj.r, stj.em)) * R(1.0001f) + R(1e-6f);
           bool need = false;
           if (own) {
             const V3<R> d = vsub(P, Cj);
@@ -844,47 +834,50 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
             }
           }
         }
-        int nact = 0;
-        const R L0j = fmax(cell_l0<R>(a.a0, cj2), R(1e-12f));
+        const R L0j = fmax(stj.l0, R(1e-12f));
         const R l0m = fmax(L0i, L0j);
         const bool all_images = pbc && R(2) * (Ri + stj.r + R(2) * l0m) >= R(0.999f) * L;
         if (do_adh) {  // block-uniform
-          // clusters of cell j that can reach this cell at all
-          bool act = false;
-          if (tid < a.nclus) {
-            const V3<R> s = {l.nx[tid] + sh.x, l.ny[tid] + sh.y, l.nz[tid] + sh.z};
-            const V3<R> ds = vsub(s, Ci);
-            const R rk = (Ri + (R)__uint_as_float(l.crad[tid]) + R(2) * l0m) * R(1.0001f) + R(1e-6f);
-            act = all_images || vdot(ds, ds) <= rk * rk;
+          // Vertices of cell j within reach of this cell (a pair closer than
+          // 2 l0 has |q - Ci| <= |q - p| + |p - Ci| < 2 l0 + Ri) and vertices
+          // of this cell within reach of cell j, the same bound reversed.
+          bool near = false, mine = false;
+          if (own) {
+            const V3<R> ds = {l.nx[tid] + sh.x - Ci.x, l.ny[tid] + sh.y - Ci.y, l.nz[tid] + sh.z - Ci.z};
+            const R rk = (Ri + R(2) * l0m) * R(1.0001f) + R(1e-6f);
+            near = all_images || vdot(ds, ds) <= rk * rk;
+            const V3<R> dp = {P.x - stj.cx - sh.x, P.y - stj.cy - sh.y, P.z - stj.cz - sh.z};
+            const R r2 = (stj.r + R(2) * l0m) * R(1.0001f) + R(1e-6f);
+            mine = all_images || vdot(dp, dp) <= r2 * r2;
           }
-          nact = block_compact(act, tid, l.cact, l.cnt);
-        }
-        if (do_adh && own && nact > 0) {
-          const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
-          const V3<R> d = vsub(P, Cj);
-          const R r2 = (stj.r + R(2) * l0m) * R(1.0001f) + R(1e-6f);
-          if (all_images || vdot(d, d) <= r2 * r2) {
-            int cnt_own = 0;
-            const R half_l = R(0.49f) * L;
-            const R lim_own = (L0i * R(2)) * (L0i * R(2)) * R(1.00001f);
-            const R lim_rev = (L0j * R(2)) * (L0j * R(2)) * R(1.00001f);
-            for (int q2 = 0; q2 < nact; q2++) {
-              const int k = l.cact[q2];
-              if (!all_images) {
-                // the cluster's vertices lie within crad[k] of its seed
-                const V3<R> s = {l.nx[k] + sh.x, l.ny[k] + sh.y, l.nz[k] + sh.z};
-                const V3<R> ds = vsub(P, s);
-                const R rk = ((R)__uint_as_float(l.crad[k]) + R(2) * l0m) * R(1.0001f) + R(1e-6f);
-                if (vdot(ds, ds) > rk * rk) continue;
-              }
-              for (int m = l.cbeg[k]; m < l.cbeg[k + 1]; m++) {
-                const int vj = l.smem_members[m];
+          int nnear = 0, nmine = 0, slot = 0;
+          int* jl = l.plist;
+          int* il = l.plist + NT;
+          block_compact2(near, tid, jl, mine, tid, il, l.cnt, nnear, nmine, slot);
+          if (nnear > 0 && nmine > 0) {  // block-uniform
+            // (own vertex, slice of near vertices) per thread; each thread
+            // sums its slice in order into LDS, then every own vertex adds
+            // its slices in order: deterministic, every lane busy.
+            const int S = max(1, min(nnear, NT / nmine));
+            const int per = (nnear + S - 1) / S;
+            const int iv = tid % nmine, sl = tid / nmine;
+            Q4<R>* acc = l.utab;
+            if (sl < S) {
+              const V3<R> Pv = lds_v(l.sx, l.sy, l.sz, il[iv]);
+              V3<R> Fa = zero;
+              int cnt_own = 0;
+              const R half_l = R(0.49f) * L;
+              const R lim_own = (L0i * R(2)) * (L0i * R(2)) * R(1.00001f);
+              const R lim_rev = (L0j * R(2)) * (L0j * R(2)) * R(1.00001f);
+              const int m1 = min(nnear, (sl + 1) * per);
+              for (int m = sl * per; m < m1; m++) {
+                const int vj = jl[m];
                 const V3<R> pj = lds_v(l.nx, l.ny, l.nz, vj);
-                // this vertex's own loop: delta = pj - p, minimum image per
+                // the own vertex's loop: delta = pj - p, minimum image per
                 // component (Cell3D_Kernel.cl:341-346); a component below
                 // 0.49 L rounds to image 0, which leaves it unchanged.
-                V3<R> dl = vsub(pj, P);
-                V3<R> dr = vsub(P, pj);  // the partner's loop: delta = p - pj
+                V3<R> dl = vsub(pj, Pv);
+                V3<R> dr = vsub(Pv, pj);  // the partner's loop: delta = p - pj
                 V3<R> nl = zero, nr = zero;  // image indices
                 if (pbc && !(fabs(dl.x) < half_l && fabs(dl.y) < half_l && fabs(dl.z) < half_l)) {
                   nl = v3(round(dl.x / L), round(dl.y / L), round(dl.z / L));
@@ -902,7 +895,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
                 if (d2 < lim_own) {
                   const R ds = sqrt(d2);
                   if (ds < L0i * R(2) && ds > R(1e-12f)) {
-                    F = vsub(F, adhesion_term(pj, P, nl, L, Kat, L0i));
+                    Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i));
                     cnt_own++;
                   }
                 }
@@ -910,11 +903,21 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
                 if (d2r < lim_rev) {
                   const R dsr = sqrt(d2r);
                   if (dsr < L0j * R(2) && dsr > R(1e-12f))
-                    F = vadd(F, adhesion_term(P, pj, nr, L, Kat, L0j));
+                    Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j));
                 }
               }
+              acc[sl * nmine + iv] = Q4<R>{Fa.x, Fa.y, Fa.z, (R)cnt_own};
             }
-            if (a.p_adh) a.p_adh[base + tid] += cnt_own;
+            __syncthreads();
+            if (mine) {
+              int cnt_own = 0;
+              for (int s2 = 0; s2 < S; s2++) {
+                const Q4<R> q4 = acc[s2 * nmine + slot];
+                F = vadd(F, V3<R>{q4.x, q4.y, q4.z});
+                cnt_own += (int)q4.w;
+              }
+              if (a.p_adh) a.p_adh[base + tid] += cnt_own;
+            }
           }
         }
         __syncthreads();
@@ -957,7 +960,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
       l.sz[tid] = Pn.z;
     }
     __syncthreads();
-    cell_stats(l, nv, nf, a.st_out, a.vol_out, a.em_out, ci);
+    cell_stats(l, nv, nf, a.st_out, a.vol_out, a.em_out, ci, l0i);
   }
   if (a.counters) {
     const double c = block_sum((double)n_contact, l.red);

@@ -63,6 +63,9 @@ int max_lds_per_block(int dev) {
 // Steps replayed per hipGraph launch (even, so the position ping-pong ends
 // where it started).
 constexpr int kGraphSteps = 64;
+// reals per cell of the cached cell statistics (CellStat)
+constexpr int kStatWords = (int)(sizeof(CellStat<float>) / sizeof(float));
+static_assert(sizeof(CellStat<double>) == kStatWords * sizeof(double), "CellStat layout");
 
 }  // namespace
 
@@ -88,8 +91,7 @@ struct dpm_tissue {
   float *tL = nullptr, *tKre = nullptr, *tKat = nullptr, *tksg = nullptr;
   int* tpbc = nullptr;
   int *faces = nullptr, *vf = nullptr, *adj = nullptr;
-  int *clus_of = nullptr, *clus_begin = nullptr, *clus_members = nullptr;
-  int nclus = 0, ww = 1;
+  int ww = 1;
   unsigned mask = 0;
   int cur = 0;
   bool stats_valid = false;
@@ -289,38 +291,6 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
       nb[v].push_back(a);
       nb[a].push_back(v);
     }
-  // Vertex clusters for adhesion culling: graph-distance Voronoi regions of
-  // seed vertices 0..K-1 (in an icosphere, the 42 vertices of the once
-  // subdivided icosahedron come first), multi-source breadth-first, first
-  // visit wins (deterministic).
-  const int K = nv >= 162 ? 42 : std::min(12, nv);
-  std::vector<int> clus_of(nv, -1), bfs;
-  for (int k = 0; k < K; k++) {
-    clus_of[k] = k;
-    bfs.push_back(k);
-  }
-  std::vector<std::vector<int>> nbs(nv);
-  for (int f = 0; f < nf; f++)
-    for (int k = 0; k < 3; k++) {
-      const int a = d->faces[3 * f + k], b = d->faces[3 * f + (k + 1) % 3];
-      nbs[a].push_back(b);
-      nbs[b].push_back(a);
-    }
-  for (auto& v : nbs) std::sort(v.begin(), v.end());
-  for (size_t q = 0; q < bfs.size(); q++)
-    for (int u : nbs[bfs[q]])
-      if (clus_of[u] < 0) {
-        clus_of[u] = clus_of[bfs[q]];
-        bfs.push_back(u);
-      }
-  for (int v = 0; v < nv; v++)
-    if (clus_of[v] < 0) clus_of[v] = 0;  // disconnected vertex: any cluster is conservative
-  std::vector<int> clus_begin(K + 1, 0), clus_members;
-  for (int k = 0; k < K; k++) {
-    for (int v = 0; v < nv; v++)
-      if (clus_of[v] == k) clus_members.push_back(v);
-    clus_begin[k + 1] = (int)clus_members.size();
-  }
   std::vector<int> vf((size_t)nv * DPM3D_MAXDEG, -1), adj((size_t)nv * DPM3D_MAXDEG, -1);
   for (int v = 0; v < nv; v++) {
     std::sort(nb[v].begin(), nb[v].end());
@@ -358,7 +328,7 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
     rc |= AR(&h->x[b], NVT);
     rc |= AR(&h->y[b], NVT);
     rc |= AR(&h->z[b], NVT);
-    rc |= AR(&h->st[b], 4 * (size_t)C);
+    rc |= AR(&h->st[b], kStatWords * (size_t)C);
     rc |= A(&h->vol[b], (size_t)C);
     rc |= AR(&h->em[b], (size_t)C);
   }
@@ -382,10 +352,6 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
   rc |= A(&h->faces, (size_t)nf * 3);
   rc |= A(&h->vf, vf.size());
   rc |= A(&h->adj, adj.size());
-  rc |= A(&h->clus_of, (size_t)nv);
-  rc |= A(&h->clus_begin, (size_t)K + 1);
-  rc |= A(&h->clus_members, (size_t)nv);
-  h->nclus = K;
   // cell-list buckets per tissue: a power of two >= 2 cells (<= 8192; larger
   // tissues scan every cell)
   std::vector<int> nbuck(T), hoff(T);
@@ -421,9 +387,6 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
   rc |= upload(h->faces, d->faces, (size_t)nf * 3, s);
   rc |= upload(h->vf, vf.data(), vf.size(), s);
   rc |= upload(h->adj, adj.data(), adj.size(), s);
-  rc |= upload(h->clus_of, clus_of.data(), (size_t)nv, s);
-  rc |= upload(h->clus_begin, clus_begin.data(), (size_t)K + 1, s);
-  rc |= upload(h->clus_members, clus_members.data(), (size_t)nv, s);
   rc |= upload(h->tbin_nbuck, nbuck.data(), (size_t)T, s);
   rc |= upload(h->tbin_hoff, hoff.data(), (size_t)T, s);
   // automatic: the cell list pays only where a cell would scan many chunks
@@ -492,7 +455,10 @@ extern "C" int dpm3d_set_cell_params(dpm_tissue* h, const float* Kv, const float
   if (Ks) TRY(upload(h->Ks, Ks, C, s));
   if (Kb) TRY(upload(h->Kb, Kb, C, s));
   if (v0) TRY(upload(h->v0, v0, C, s));
-  if (a0) TRY(upload(h->a0, a0, C, s));
+  if (a0) {
+    TRY(upload(h->a0, a0, C, s));
+    h->stats_valid = false;  // the cached statistics carry each cell's l0
+  }
   HIPCHECK(hipStreamSynchronize(s));
   return DPM_OK;
 }
@@ -670,10 +636,6 @@ static Step3D<R> step_args(dpm_tissue* h, int in, float dt, int flags) {
   a.faces = h->faces;
   a.vf = h->vf;
   a.adj = h->adj;
-  a.clus_of = h->clus_of;
-  a.clus_begin = h->clus_begin;
-  a.clus_members = h->clus_members;
-  a.nclus = h->nclus;
   a.ww = h->ww;
   a.nv = h->nv;
   a.nf = h->nf;
@@ -821,7 +783,7 @@ extern "C" int dpm3d_cell_stats(dpm_tissue* h, double* volume, float* com) {
   if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
   TRY(ensure_stats(h));
   const int C = h->C;
-  std::vector<unsigned char> st((size_t)C * 4 * h->rs);
+  std::vector<unsigned char> st((size_t)C * kStatWords * h->rs);
   HIPCHECK(hipMemcpyAsync(st.data(), h->st[h->cur], st.size(), hipMemcpyDeviceToHost, h->stream));
   if (volume)
     HIPCHECK(hipMemcpyAsync(volume, h->vol[h->cur], C * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -829,8 +791,8 @@ extern "C" int dpm3d_cell_stats(dpm_tissue* h, double* volume, float* com) {
   if (com)
     for (int c = 0; c < C; c++)
       for (int k = 0; k < 3; k++)
-        com[3 * c + k] = h->rs == 8 ? (float)((const double*)st.data())[4 * c + k]
-                                    : ((const float*)st.data())[4 * c + k];
+        com[3 * c + k] = h->rs == 8 ? (float)((const double*)st.data())[kStatWords * c + k]
+                                    : ((const float*)st.data())[kStatWords * c + k];
   return DPM_OK;
 }
 

@@ -212,6 +212,42 @@ __device__ __forceinline__ int block_compact(bool flag, int value, int* list, in
   return total;
 }
 
+// Two stream compactions in one pass: threads with fa append va to la,
+// threads with fb append vb to lb, in thread order; slot_b is the calling
+// thread's index in lb (meaningful where fb).  counts holds >= 34 ints.
+__device__ __forceinline__ void block_compact2(bool fa, int va, int* la, bool fb, int vb, int* lb, int* counts,
+                                               int& na, int& nb, int& slot_b) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
+  const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
+  const unsigned long long ma = __ballot(fa), mb = __ballot(fb);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  if (lane == 0) {
+    counts[wave] = __popcll(ma);
+    counts[16 + wave] = __popcll(mb);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int sa = 0, sb = 0;
+    for (int w = 0; w < nw; w++) {
+      const int ca = counts[w], cb = counts[16 + w];
+      counts[w] = sa;
+      counts[16 + w] = sb;
+      sa += ca;
+      sb += cb;
+    }
+    counts[32] = sa;
+    counts[33] = sb;
+  }
+  __syncthreads();
+  const int pa = counts[wave] + __popcll(ma & below);
+  slot_b = counts[16 + wave] + __popcll(mb & below);
+  if (fa) la[pa] = va;
+  if (fb) lb[slot_b] = vb;
+  na = counts[32];
+  nb = counts[33];
+  __syncthreads();
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X
 // (workgroup b runs on XCD b % 8).  This bijection hands each XCD a
 // contiguous range of work items, so spatially neighbouring cells — which
