@@ -168,6 +168,15 @@ int dpm3d_energy(dpm_tissue* h, double* energy);
  * bit-identical results. */
 int dpm3d_set_cell_list(dpm_tissue* h, int mode);
 
+/* Kernel form of the 3D step (no reference counterpart; the reference
+ * launches one work-item per vertex per force term, Tissue3D.cpp:372-445):
+ * 1 one workgroup per cell; 2 one 64-lane wave per cell (meshes of at most
+ * 256 vertices, several cells per workgroup, no workgroup barriers after
+ * set-up); 0 (default) chooses 2 where it applies.  Volume, edge-spring,
+ * substrate, bending and contact forces are bit-identical between the two;
+ * adhesion sums are grouped differently (float rounding). */
+int dpm3d_set_step_kernel(dpm_tissue* h, int kind);
+
 /* Differentiation form (no reference counterpart).  dpm3d_trajectory runs
  * nsteps Euler steps (as dpm_euler) and keeps the positions every `every`
  * steps (nsteps a multiple of every) on the device.  dpm3d_adjoint then

@@ -35,7 +35,7 @@ EXPORTS = [
     "dpm3d_set_cell_params", "dpm3d_set_tissue_params", "dpm3d_cell_stats", "dpm3d_probe",
     "dpm3d_create_f64", "dpm3d_set_positions_f64", "dpm3d_get_positions_f64",
     "dpm3d_get_forces_f64", "dpm3d_energy", "dpm3d_trajectory", "dpm3d_adjoint", "dpm3d_adjoint_f64",
-    "dpm3d_set_compensated", "dpm3d_set_cell_list",
+    "dpm3d_set_compensated", "dpm3d_set_cell_list", "dpm3d_set_step_kernel",
     "dpm2d_create", "dpm2d_set_positions", "dpm2d_get_positions", "dpm2d_get_forces",
     "dpm2d_set_cell_params", "dpm2d_set_tissue_params", "dpm2d_probe",
     "dpm2d_create_f64", "dpm2d_set_positions_f64", "dpm2d_get_positions_f64", "dpm2d_get_forces_f64",
@@ -82,6 +82,7 @@ def load() -> ctypes.CDLL:
     lib.dpm3d_trajectory.argtypes = [vp, i, fp, i]
     lib.dpm3d_set_compensated.argtypes = [vp, i]
     lib.dpm3d_set_cell_list.argtypes = [vp, i]
+    lib.dpm3d_set_step_kernel.argtypes = [vp, i]
     lib.dpm3d_adjoint.argtypes = [vp, vp, vp, vp]
     lib.dpm3d_adjoint_f64.argtypes = [vp, vp, vp, vp]
     lib.dpm2d_create.argtypes = [vp, vp]
@@ -291,6 +292,11 @@ class Tissues3D(_Handle):
     def set_cell_list(self, mode: int = -1) -> None:
         """Candidate search: 0 scan, 1 binned cell list, -1 automatic."""
         _check(load().dpm3d_set_cell_list(self._h, int(mode)))
+
+    def set_step_kernel(self, kind: int = 0) -> None:
+        """0 automatic, 1 one workgroup per cell, 2 one wave per cell
+        (dpm3d_set_step_kernel)."""
+        _check(load().dpm3d_set_step_kernel(self._h, int(kind)))
 
     def set_compensated(self, on: bool = True) -> None:
         """Compensated (x + residual) positions for a float tissue."""

@@ -1,0 +1,659 @@
+// Wave-per-cell form of the 3D step (one 64-lane wave owns one cell).
+//
+// The same step as k3d_step (dpm3d.hip) for meshes of at most 256
+// vertices: lane l handles vertices l, l + 64, ..., the per-vertex state
+// (positions, force accumulators) lives in the wave's LDS slice, every
+// per-cell reduction is a wave shuffle and every list a wave ballot, and the
+// waves of a workgroup never wait for each other after the faces are
+// staged.  Shape forces are gathered per corner: each vertex evaluates its
+// incident (face, corner) terms with the arithmetic of the face pass of
+// k3d_step, in the same order, so volume, edge-spring, substrate and bending
+// forces are bit-identical to it; contact uses the same winding evaluation;
+// the adhesion sums are grouped differently (slices of one wave).
+#pragma once
+
+constexpr int kWaveGlist = 128;  // gathered cell-list candidates per wave
+constexpr int kWaveMaxNv = 256;  // meshes the wave-per-cell step takes
+constexpr int kWaveCand = 128;   // candidate cells per batch
+
+// LDS: packed faces shared by the workgroup, then one slice per wave.  The
+// winding directions (or adhesion slices) share their bytes with the
+// cell-list gather, which is finished before the first candidate.
+struct Lds3DWave {
+  size_t vf, adj, fend, slice, total;
+  size_t sx, nx, fx, ws, np, ms, jl, il, down, cl, ut, gl, gath;
+};
+__host__ __device__ inline Lds3DWave ldsw_layout(int nv, int nf, int cpw, int rsize) {
+  Lds3DWave L;
+  const size_t rs = (size_t)rsize, n = (size_t)nv;
+  auto al = [](size_t o) { return (o + 31u) & ~(size_t)31u; };
+  L.vf = al(4u * (size_t)nf);                   // faces: i0 | i1 << 10 | i2 << 20
+  L.adj = L.vf + 2u * n * DPM3D_MAXDEG;          // incident (face*3 + corner) rows
+  L.fend = al(L.adj + 2u * n * DPM3D_MAXDEG);    // neighbour rows (0xffff: none)
+  size_t s = 0;
+  L.sx = s;   s = al(s + 3u * rs * n);  // own positions (x | y | z)
+  L.nx = s;   s = al(s + 3u * rs * n);  // candidate positions; bending scratch
+  L.fx = s;   s = al(s + 3u * rs * n);  // force accumulators
+  L.ws = s;   s = al(s + rs * n);       // sum of |w| over contact partners
+  L.np = s;   s += 2u * n;              // contact partners recorded
+  L.ms = s;   s += 2u * n;              // slot in the own-vertex list (adhesion)
+  L.jl = s;   s += 2u * n;              // candidate vertices in reach
+  L.il = s;   s = al(s + 2u * n);       // own vertices in reach
+  L.down = s; s = al(s + 4u * (size_t)((nf + 31) / 32));  // facing-down faces
+  L.cl = s;   s = al(s + 4u * (size_t)kWaveCand);  // candidate cells of a batch
+  L.ut = s;
+  const size_t ut_end = al(s + 4u * rs * (n > 64 ? n : 64));  // winding directions | adhesion slices
+  L.gl = s;   s += 4u * (size_t)kWaveGlist;  // cell-list candidates
+  L.gath = s; s += 4u * 96u;                  // 27 buckets: ids, starts, prefix
+  s = al(s);
+  L.slice = s > ut_end ? s : ut_end;
+  L.total = L.fend + (size_t)cpw * L.slice;
+  return L;
+}
+
+__device__ __forceinline__ void wface(const unsigned* sf, int f, int& i0, int& i1, int& i2) {
+  const unsigned w = sf[f];
+  i0 = (int)(w & 1023u);
+  i1 = (int)((w >> 10) & 1023u);
+  i2 = (int)(w >> 20);
+}
+
+// Butterfly reductions: every lane receives the result.
+__device__ __forceinline__ double wsum_all(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, DPM_WAVE);
+  return v;
+}
+__device__ __forceinline__ double wmax_all(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, DPM_WAVE));
+  return v;
+}
+
+// winding_wave over per-wave arrays and packed faces (same arithmetic).
+template <typename R>
+__device__ __forceinline__ R winding_w(const R* nx, const R* ny, const R* nz, const unsigned* sf, Q4<R>* ut,
+                                       int nv, int nf, V3<R> sh, V3<R> p) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1);
+  for (int v = lane; v < nv; v += DPM_WAVE) {
+    const V3<R> A = {(nx[v] + sh.x) - p.x, (ny[v] + sh.y) - p.y, (nz[v] + sh.z) - p.z};
+    const V3<R> u = vnormalize(A);
+    ut[v] = Q4<R>{u.x, u.y, u.z, R(0)};
+  }
+  wave_lds_sync();
+  R part = R(0);
+  for (int f = lane; f < nf; f += DPM_WAVE) {
+    int ia, ib, ic;
+    wface(sf, f, ia, ib, ic);
+    const Q4<R> ua = ut[ia], ub = ut[ib], uc = ut[ic];
+    const V3<R> u = {ua.x, ua.y, ua.z};
+    const V3<R> v = {ub.x, ub.y, ub.z};
+    const V3<R> w = {uc.x, uc.y, uc.z};
+    const R denom = ((R(1) + vdot(u, v)) + vdot(v, w)) + vdot(w, u);
+    if (denom < R(1e-8f)) continue;
+    const R num = vdot(u, vcross(v, w));
+    part += R(2) * atan2(num, denom);
+  }
+  const double tot = wave_sum((double)part);
+  wave_lds_sync();
+  return (R)(__shfl(tot, 0, DPM_WAVE) / (double)(R(4) * R(3.14159265358979323846)));
+}
+
+template <typename R>
+__device__ void k3d_wstep_body(const Step3D<R>& a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nv = a.nv, nf = a.nf;
+  const int lane = threadIdx.x & (DPM_WAVE - 1), wv = threadIdx.x / DPM_WAVE;
+  const int cpw = blockDim.x / DPM_WAVE;
+  const Lds3DWave LW = ldsw_layout(nv, nf, cpw, (int)sizeof(R));
+  unsigned* sf = reinterpret_cast<unsigned*>(smem);
+  unsigned short* svf = reinterpret_cast<unsigned short*>(smem + LW.vf);
+  unsigned short* sadj = reinterpret_cast<unsigned short*>(smem + LW.adj);
+  for (int f = threadIdx.x; f < nf; f += blockDim.x)
+    sf[f] = (unsigned)a.faces[3 * f] | ((unsigned)a.faces[3 * f + 1] << 10) | ((unsigned)a.faces[3 * f + 2] << 20);
+  for (int i = threadIdx.x; i < nv * DPM3D_MAXDEG; i += blockDim.x) {
+    svf[i] = (unsigned short)a.vf[i];    // -1 pads become 0xffff
+    sadj[i] = (unsigned short)a.adj[i];
+  }
+  __syncthreads();  // the only workgroup barrier
+  const int ci = xcd_remap(blockIdx.x, gridDim.x) * cpw + wv;
+  if (ci >= a.ncells) return;  // whole wave
+
+  unsigned char* slb = smem + LW.fend + (size_t)wv * LW.slice;
+  R* sx = reinterpret_cast<R*>(slb + LW.sx);
+  R* sy = sx + nv;
+  R* sz = sy + nv;
+  R* nx = reinterpret_cast<R*>(slb + LW.nx);
+  R* ny = nx + nv;
+  R* nz = ny + nv;
+  R* fx = reinterpret_cast<R*>(slb + LW.fx);
+  R* fy = fx + nv;
+  R* fz = fy + nv;
+  R* ws = reinterpret_cast<R*>(slb + LW.ws);
+  short* np = reinterpret_cast<short*>(slb + LW.np);
+  short* ms = reinterpret_cast<short*>(slb + LW.ms);
+  unsigned short* jl = reinterpret_cast<unsigned short*>(slb + LW.jl);
+  unsigned short* il = reinterpret_cast<unsigned short*>(slb + LW.il);
+  Q4<R>* ut = reinterpret_cast<Q4<R>*>(slb + LW.ut);
+  int* gl = reinterpret_cast<int*>(slb + LW.gl);
+  int* gath = reinterpret_cast<int*>(slb + LW.gath);
+  int* cl = reinterpret_cast<int*>(slb + LW.cl);
+  unsigned* down = reinterpret_cast<unsigned*>(slb + LW.down);
+
+  const int t = a.cell_tissue[ci];
+  const int cb = a.tissue_begin[t], ce = a.tissue_begin[t + 1];
+  const R L = (R)a.tL[t];
+  const int pbc = a.tpbc[t];
+  const R Kre = (R)a.tKre[t], Kat = (R)a.tKat[t], ksg = (R)a.tksg[t];
+  const CellStat<R> sti = a.st_in[ci];
+  const V3<R> Ci = {sti.cx, sti.cy, sti.cz};
+  const R Ri = sti.r;
+  const R Kvi = (R)a.Kv[ci], Kai = (R)a.Ka[ci], Ksi = (R)a.Ks[ci], Kbi = (R)a.Kb[ci];
+  const R v0i = (R)a.v0[ci], a0i = (R)a.a0[ci], l0i = cell_l0<R>(a.a0, ci);
+  const unsigned mask = a.mask;
+  const size_t base = (size_t)ci * nv;
+  const V3<R> zero = {R(0), R(0), R(0)};
+
+  for (int v = lane; v < nv; v += DPM_WAVE) {
+    sx[v] = a.xin[base + v];
+    sy[v] = a.yin[base + v];
+    sz[v] = a.zin[base + v];
+    ws[v] = R(0);
+    np[v] = 0;
+  }
+  wave_lds_sync();
+
+  // ---------------------------------------------------------- shape forces
+  const bool shape_on = (mask & (M_VOLUME | M_SURFACE | M_SUBSTRATE)) != 0;
+  const R strain = (R)(a.vol_in[ci] / (double)v0i - 1.0);
+  const R kvol = -Kvi * strain / R(6);
+  const bool vol_on = (mask & M_VOLUME) && Kvi != R(0);
+  const bool surf_on = (mask & M_SURFACE) && !(Kai < R(1e-8f));
+  const bool sub_on = (mask & M_SUBSTRATE) && !(Ksi < R(1e-12f));
+  const R sc = Kai * sqrt(a0i) / l0i * R(0.3f);
+  const R inv_l0 = R(1) / l0i;
+  if (shape_on && sub_on) {  // facing-down faces (Cell3D_Kernel.cl:206-246), one bit each
+    for (int f0 = 0; f0 < nf; f0 += DPM_WAVE) {
+      const int f = f0 + lane;
+      bool dn = false;
+      if (f < nf) {
+        int i0, i1, i2;
+        wface(sf, f, i0, i1, i2);
+        const V3<R> P0 = lds_v(sx, sy, sz, i0), P1 = lds_v(sx, sy, sz, i1), P2 = lds_v(sx, sy, sz, i2);
+        const V3<R> un = vnormalize(vcross(vsub(P1, P0), vsub(P2, P0)));
+        dn = un.z < R(-0.1f);
+        if (dn && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
+      }
+      const unsigned long long m = __ballot(dn);
+      if (lane == 0) {
+        down[f0 / 32] = (unsigned)m;
+        if (f0 / 32 + 1 < (nf + 31) / 32) down[f0 / 32 + 1] = (unsigned)(m >> 32);
+      }
+    }
+    wave_lds_sync();
+  }
+  for (int v = lane; v < nv; v += DPM_WAVE) {
+    const V3<R> P = lds_v(sx, sy, sz, v);
+    V3<R> F = zero;
+    if (shape_on) {
+      for (int q = 0; q < DPM3D_MAXDEG; q++) {
+        const int e = svf[v * DPM3D_MAXDEG + q];  // face*3 + corner, ascending
+        if (e == 0xffff) break;
+        const int f = e / 3, c = e - 3 * f;
+        int i0, i1, i2;
+        wface(sf, f, i0, i1, i2);
+        const V3<R> P0 = lds_v(sx, sy, sz, i0), P1 = lds_v(sx, sy, sz, i1), P2 = lds_v(sx, sy, sz, i2);
+        // the face pass's corner term for this corner (k3d_step)
+        V3<R> cc = zero;
+        if (vol_on) {  // Cell3D_Kernel.cl:95-111
+          const V3<R> A = vsub(P1, Ci), B = vsub(P2, Ci), Cc = vsub(P0, Ci);
+          cc = c == 0 ? vscale(vcross(A, B), kvol) : c == 1 ? vscale(vcross(B, Cc), kvol)
+                                                            : vscale(vcross(Cc, A), kvol);
+        }
+        if (surf_on) {  // Cell3D_Kernel.cl:140-176
+          const V3<R> lv0 = vsub(P1, P0), lv1 = vsub(P2, P1), lv2 = vsub(P0, P2);
+          const R len0 = sqrt(vdot(lv0, lv0)), len1 = sqrt(vdot(lv1, lv1)), len2 = sqrt(vdot(lv2, lv2));
+          if (!(len0 < R(1e-12f) || len1 < R(1e-12f) || len2 < R(1e-12f))) {
+            // corner c takes e_c - e_(c-1)
+            const V3<R> la = c == 0 ? lv0 : c == 1 ? lv1 : lv2;
+            const V3<R> lb = c == 0 ? lv2 : c == 1 ? lv0 : lv1;
+            const R na = c == 0 ? len0 : c == 1 ? len1 : len2;
+            const R nb = c == 0 ? len2 : c == 1 ? len0 : len1;
+            const V3<R> ea = vscale(la, (na * inv_l0 - R(1)) / na);
+            const V3<R> eb = vscale(lb, (nb * inv_l0 - R(1)) / nb);
+            cc = vadd(cc, vscale(vsub(ea, eb), sc));
+          }
+        }
+        if (sub_on && ((down[f >> 5] >> (f & 31)) & 1u)) cc = vadd(cc, substrate_corner(P, Ci, Ksi, ksg, L, l0i));
+        F = vadd(F, cc);
+      }
+    }
+    fx[v] = F.x;
+    fy[v] = F.y;
+    fz[v] = F.z;
+  }
+
+  // --------------------------------------------------------------- bending
+  if ((mask & M_BENDING) && Kbi != R(0)) {  // wave-uniform; scratch in nx
+    for (int v = lane; v < nv; v += DPM_WAVE) {
+      const V3<R> P = lds_v(sx, sy, sz, v);
+      V3<R> s = zero;
+      for (int q = 0; q < DPM3D_MAXDEG; q++) {
+        const int u = sadj[v * DPM3D_MAXDEG + q];
+        if (u == 0xffff) break;
+        s = vadd(s, vsub(lds_v(sx, sy, sz, u), P));
+      }
+      nx[v] = s.x;
+      ny[v] = s.y;
+      nz[v] = s.z;
+    }
+    wave_lds_sync();
+    for (int v = lane; v < nv; v += DPM_WAVE) {
+      V3<R> acc = zero;
+      int deg = 0;
+      for (int q = 0; q < DPM3D_MAXDEG; q++) {
+        const int u = sadj[v * DPM3D_MAXDEG + q];
+        if (u == 0xffff) break;
+        deg++;
+        acc = vadd(acc, lds_v(nx, ny, nz, u));
+      }
+      const V3<R> lap = vsub(acc, vscale(lds_v(nx, ny, nz, v), (R)deg));
+      const V3<R> F = vadd(lds_v(fx, fy, fz, v), vscale(lap, -Kbi));
+      fx[v] = F.x;
+      fy[v] = F.y;
+      fz[v] = F.z;
+    }
+  }
+  wave_lds_sync();  // forces and scratch settled before the candidates
+
+  // ------------------------------------------- contact and vertex adhesion
+  const bool do_contact = (mask & M_CONTACT) && Kre != R(0);
+  const bool do_adh = (mask & M_ADHESION) && Kat != R(0);
+  unsigned long long n_wind = 0, n_contact = 0, n_cand = 0;
+  if (do_contact || do_adh) {  // wave-uniform
+    const R L0i = fmax(l0i, R(1e-12f));
+    const R invL = R(1) / L;
+    // candidate source: every cell of the tissue, or the cells of the 27
+    // buckets around this cell sorted by index (held two per lane)
+    bool gathered = false;
+    int g0 = 0, g1 = 0;
+    int nsrc = ce - cb;
+    if (a.bin_list && a.tbin_nb[t] > 0) {
+      const int nb = a.tbin_nb[t], nbuck = a.tbin_nbuck[t];
+      const int* bs = a.bin_start + a.tbin_hoff[t];
+      const R wb = L / (R)nb;
+      int* gid = gath;
+      int* gst = gath + 32;
+      int* gof = gath + 64;
+      if (lane < 27) {
+        const int bx = bin_coord(Ci.x, L, wb, nb), by = bin_coord(Ci.y, L, wb, nb), bz = bin_coord(Ci.z, L, wb, nb);
+        const int dx = lane % 3 - 1, dy = (lane / 3) % 3 - 1, dz = lane / 9 - 1;
+        gid[lane] = (int)bin_bucket((bx + dx + nb) % nb, (by + dy + nb) % nb, (bz + dz + nb) % nb, nb, nbuck);
+      }
+      wave_lds_sync();
+      if (lane == 0) {  // distinct buckets, their cells' offsets
+        for (int i = 1; i < 27; i++) {
+          const int v = gid[i];
+          int j = i - 1;
+          while (j >= 0 && gid[j] > v) {
+            gid[j + 1] = gid[j];
+            j--;
+          }
+          gid[j + 1] = v;
+        }
+        int nu = 0, run = 0;
+        for (int i = 0; i < 27; i++) {
+          if (i > 0 && gid[i] == gid[i - 1]) continue;
+          gst[nu] = bs[gid[i]];
+          gof[nu] = run;
+          run += bs[gid[i] + 1] - bs[gid[i]];
+          nu++;
+        }
+        gof[nu] = run;
+        gid[31] = nu;
+      }
+      wave_lds_sync();
+      const int nu = gid[31], n = gof[nu];
+      if (n <= kWaveGlist) {  // wave-uniform; otherwise scan every cell
+        int val0 = -1, val1 = -1;
+        if (lane < n) {
+          int u = 0;
+          while (gof[u + 1] <= lane) u++;
+          val0 = a.bin_list[cb + gst[u] + (lane - gof[u])];
+          gl[lane] = val0;
+        }
+        if (lane + DPM_WAVE < n) {
+          const int i = lane + DPM_WAVE;
+          int u = 0;
+          while (gof[u + 1] <= i) u++;
+          val1 = a.bin_list[cb + gst[u] + (i - gof[u])];
+          gl[i] = val1;
+        }
+        wave_lds_sync();
+        int rk0 = 0, rk1 = 0;  // rank sort (indices are distinct)
+        for (int u = 0; u < n; u++) {
+          const int g = gl[u];
+          rk0 += g < val0;
+          rk1 += g < val1;
+        }
+        wave_lds_sync();
+        if (lane < n) gl[rk0] = val0;
+        if (lane + DPM_WAVE < n) gl[rk1] = val1;
+        wave_lds_sync();
+        if (lane < n) g0 = gl[lane];
+        if (lane + DPM_WAVE < n) g1 = gl[lane + DPM_WAVE];
+        wave_lds_sync();  // the gather's bytes are the winding table's
+        gathered = true;
+        nsrc = n;
+      }
+    }
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int c0 = 0;
+    while (c0 < nsrc) {  // wave-uniform: candidates in batches of at most kWaveCand
+      int ncl = 0;
+      while (c0 < nsrc && ncl + DPM_WAVE <= kWaveCand) {
+        const int k0 = c0 + lane;
+        const int cj = k0 < nsrc ? (gathered ? (c0 == 0 ? g0 : g1) : cb + k0) : ce;
+        bool cand = false;
+        if (cj < ce && cj != ci) {
+          const CellStat<R> stj = a.st_in[cj];
+          V3<R> sh = zero;
+          if (pbc)
+            sh = v3(image_shift(Ci.x - stj.cx, L, invL), image_shift(Ci.y - stj.cy, L, invL),
+                    image_shift(Ci.z - stj.cz, L, invL));
+          const V3<R> d = {stj.cx + sh.x - Ci.x, stj.cy + sh.y - Ci.y, stj.cz + sh.z - Ci.z};
+          R reach = R(0);
+          if (do_contact) reach = Ri + stj.r + contact_margin(stj.r, stj.em);
+          if (do_adh) {
+            const R l0m = fmax(L0i, fmax(stj.l0, R(1e-12f)));
+            const R r2 = Ri + stj.r + R(2) * l0m;
+            reach = fmax(reach, r2);
+            if (pbc && R(2) * r2 >= R(0.999f) * L) cand = true;  // minimum image per pair
+          }
+          const R rr = reach * R(1.0001f) + R(1e-6f);
+          cand = cand || vdot(d, d) <= rr * rr;
+        }
+        const unsigned long long cmask = __ballot(cand);
+        if (cand) cl[ncl + __popcll(cmask & below)] = cj;  // ascending
+        ncl += __popcll(cmask);
+        c0 += DPM_WAVE;
+      }
+      n_cand += (unsigned long long)ncl;
+      wave_lds_sync();
+      // each candidate's statistics and vertices are loaded while the
+      // previous candidate is processed
+      int cjn = 0;
+      CellStat<R> stn = {};
+      R pfx[kWaveMaxNv / DPM_WAVE], pfy[kWaveMaxNv / DPM_WAVE], pfz[kWaveMaxNv / DPM_WAVE];
+      if (ncl > 0) {
+        cjn = cl[0];
+        stn = a.st_in[cjn];
+#pragma unroll
+        for (int r = 0; r < kWaveMaxNv / DPM_WAVE; r++) {
+          const int v = lane + DPM_WAVE * r;
+          if (v < nv) {
+            pfx[r] = a.xin[(size_t)cjn * nv + v];
+            pfy[r] = a.yin[(size_t)cjn * nv + v];
+            pfz[r] = a.zin[(size_t)cjn * nv + v];
+          }
+        }
+      }
+      for (int qc = 0; qc < ncl; qc++) {
+        const int cj2 = cjn;
+        const CellStat<R> stj = stn;
+#pragma unroll
+        for (int r = 0; r < kWaveMaxNv / DPM_WAVE; r++) {
+          const int v = lane + DPM_WAVE * r;
+          if (v < nv) {
+            nx[v] = pfx[r];
+            ny[v] = pfy[r];
+            nz[v] = pfz[r];
+          }
+        }
+        if (qc + 1 < ncl) {
+          cjn = cl[qc + 1];
+          stn = a.st_in[cjn];
+#pragma unroll
+          for (int r = 0; r < kWaveMaxNv / DPM_WAVE; r++) {
+            const int v = lane + DPM_WAVE * r;
+            if (v < nv) {
+              pfx[r] = a.xin[(size_t)cjn * nv + v];
+              pfy[r] = a.yin[(size_t)cjn * nv + v];
+              pfz[r] = a.zin[(size_t)cjn * nv + v];
+            }
+          }
+        }
+        V3<R> sh = zero;
+        if (pbc)
+          sh = v3(image_shift(Ci.x - stj.cx, L, invL), image_shift(Ci.y - stj.cy, L, invL),
+                  image_shift(Ci.z - stj.cz, L, invL));
+        wave_lds_sync();
+        if (do_contact) {
+          const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
+          const R rj = (stj.r + contact_margin(stj.r, stj.em)) * R(1.0001f) + R(1e-6f);
+          for (int k0v = 0; k0v < nv; k0v += DPM_WAVE) {
+            const int v = k0v + lane;
+            bool need = false;
+            V3<R> P = zero;
+            if (v < nv) {
+              P = lds_v(sx, sy, sz, v);
+              const V3<R> d = vsub(P, Cj);
+              need = vdot(d, d) <= rj * rj;
+            }
+            unsigned long long nm = __ballot(need);
+            n_wind += (unsigned long long)__popcll(nm);
+            R wk = R(0);
+            while (nm) {
+              const int bb = __ffsll((long long)nm) - 1;
+              nm &= nm - 1ull;
+              const R w = winding_w(nx, ny, nz, sf, ut, nv, nf, sh, lds_v(sx, sy, sz, k0v + bb));
+              if (lane == bb) wk = w;
+            }
+            if (need) {
+              const size_t pidx = (base + v) * (size_t)a.ncmax + (size_t)(cj2 - cb);
+              if (a.p_winding) a.p_winding[pidx] = (float)wk;
+              if (!(fabs(wk) < R(1e-6f))) {
+                const V3<R> dir = vnormalize(vsub(Ci, P));
+                const R mag = fabs(wk) * R(0.5f) * Kre;
+                const V3<R> F = vadd(lds_v(fx, fy, fz, v), vscale(dir, mag));
+                fx[v] = F.x;
+                fy[v] = F.y;
+                fz[v] = F.z;
+                n_contact++;
+                ws[v] += fabs(wk);
+                if (a.rec_cell) {
+                  const int npart = np[v];
+                  if (npart < DPM3D_MAXPART) {
+                    const size_t r = (base + v) * DPM3D_MAXPART + npart;
+                    a.rec_cell[r] = cj2;
+                    a.rec_w[r] = wk;
+                    a.rec_shift[3 * r] = sh.x;
+                    a.rec_shift[3 * r + 1] = sh.y;
+                    a.rec_shift[3 * r + 2] = sh.z;
+                  } else {
+                    atomicOr(a.rec_overflow, 1);
+                  }
+                  np[v] = (short)(npart + 1);
+                }
+                if (a.p_contact) a.p_contact[pidx] = 1;
+              }
+            }
+          }
+        }
+        if (do_adh) {
+          const R L0j = fmax(stj.l0, R(1e-12f));
+          const R l0m = fmax(L0i, L0j);
+          const bool all_images = pbc && R(2) * (Ri + stj.r + R(2) * l0m) >= R(0.999f) * L;
+          // vertices of cell j within reach of this cell, and of this cell
+          // within reach of cell j (|q - Ci| < 2 l0 + Ri, and reversed)
+          const R rk = (Ri + R(2) * l0m) * R(1.0001f) + R(1e-6f);
+          const R r2 = (stj.r + R(2) * l0m) * R(1.0001f) + R(1e-6f);
+          int nnear = 0, nmine = 0;
+          for (int k0v = 0; k0v < nv; k0v += DPM_WAVE) {
+            const int v = k0v + lane;
+            bool near = false, mine = false;
+            if (v < nv) {
+              const V3<R> ds = {nx[v] + sh.x - Ci.x, ny[v] + sh.y - Ci.y, nz[v] + sh.z - Ci.z};
+              near = all_images || vdot(ds, ds) <= rk * rk;
+              const V3<R> dp = {sx[v] - stj.cx - sh.x, sy[v] - stj.cy - sh.y, sz[v] - stj.cz - sh.z};
+              mine = all_images || vdot(dp, dp) <= r2 * r2;
+            }
+            const unsigned long long mn = __ballot(near), mm = __ballot(mine);
+            if (near) jl[nnear + __popcll(mn & below)] = (unsigned short)v;
+            const int slot = nmine + __popcll(mm & below);
+            if (mine) il[slot] = (unsigned short)v;
+            if (v < nv) ms[v] = (short)(mine ? slot : -1);
+            nnear += __popcll(mn);
+            nmine += __popcll(mm);
+          }
+          if (nnear > 0 && nmine > 0) {  // wave-uniform
+            wave_lds_sync();
+            // (own vertex, slice of near vertices) per lane; the slices are
+            // summed in order afterwards
+            const int S = nmine >= DPM_WAVE ? 1 : max(1, min(nnear, DPM_WAVE / nmine));
+            const int per = (nnear + S - 1) / S;
+            const R half_l = R(0.49f) * L;
+            const R lim_own = (L0i * R(2)) * (L0i * R(2)) * R(1.00001f);
+            const R lim_rev = (L0j * R(2)) * (L0j * R(2)) * R(1.00001f);
+            for (int tw = lane; tw < S * nmine; tw += DPM_WAVE) {
+              const int iv = tw % nmine, sl = tw / nmine;
+              const V3<R> Pv = lds_v(sx, sy, sz, (int)il[iv]);
+              V3<R> Fa = zero;
+              int cnt_own = 0;
+              const int m1 = min(nnear, (sl + 1) * per);
+              for (int m = sl * per; m < m1; m++) {
+                const V3<R> pj = lds_v(nx, ny, nz, (int)jl[m]);
+                // delta = pj - p and p - pj, minimum image per component
+                // (Cell3D_Kernel.cl:341-346); decisions in the reference's
+                // arithmetic, accepted forces in double on the same image
+                V3<R> dl = vsub(pj, Pv);
+                V3<R> dr = vsub(Pv, pj);
+                V3<R> nl = zero, nr = zero;
+                if (pbc && !(fabs(dl.x) < half_l && fabs(dl.y) < half_l && fabs(dl.z) < half_l)) {
+                  nl = v3(round(dl.x / L), round(dl.y / L), round(dl.z / L));
+                  nr = v3(round(dr.x / L), round(dr.y / L), round(dr.z / L));
+                  dl = vsub(dl, v3(L * nl.x, L * nl.y, L * nl.z));
+                  dr = vsub(dr, v3(L * nr.x, L * nr.y, L * nr.z));
+                }
+                const R d2 = vdot(dl, dl);
+                if (d2 < lim_own) {
+                  const R dsq = sqrt(d2);
+                  if (dsq < L0i * R(2) && dsq > R(1e-12f)) {
+                    Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i));
+                    cnt_own++;
+                  }
+                }
+                const R d2r = vdot(dr, dr);
+                if (d2r < lim_rev) {
+                  const R dsr = sqrt(d2r);
+                  if (dsr < L0j * R(2) && dsr > R(1e-12f)) Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j));
+                }
+              }
+              ut[sl * nmine + iv] = Q4<R>{Fa.x, Fa.y, Fa.z, (R)cnt_own};
+            }
+            wave_lds_sync();
+            for (int v = lane; v < nv; v += DPM_WAVE) {
+              const int slot = ms[v];
+              if (slot < 0) continue;
+              V3<R> F = lds_v(fx, fy, fz, v);
+              int cnt_own = 0;
+              for (int s2 = 0; s2 < S; s2++) {
+                const Q4<R> q4 = ut[s2 * nmine + slot];
+                F = vadd(F, V3<R>{q4.x, q4.y, q4.z});
+                cnt_own += (int)q4.w;
+              }
+              fx[v] = F.x;
+              fy[v] = F.y;
+              fz[v] = F.z;
+              if (a.p_adh) a.p_adh[base + v] += cnt_own;
+            }
+          }
+        }
+        wave_lds_sync();  // the candidate arrays are rewritten next
+      }
+    }
+  }
+
+  // ------------------------------------------------------- Euler and stats
+  double px = 0.0, py = 0.0, pz = 0.0;
+  for (int v = lane; v < nv; v += DPM_WAVE) {
+    const V3<R> F = lds_v(fx, fy, fz, v);
+    if (a.w_out) a.w_out[base + v] = ws[v];
+    if (a.rec_cell)
+      for (int q = np[v]; q < DPM3D_MAXPART; q++) a.rec_cell[(base + v) * DPM3D_MAXPART + q] = -1;
+    if (a.flags & F_WRITE_FORCES) {
+      a.fx[base + v] = F.x;
+      a.fy[base + v] = F.y;
+      a.fz[base + v] = F.z;
+    }
+    if (a.flags & F_MOVE) {
+      const V3<R> P = lds_v(sx, sy, sz, v);
+      V3<R> Pn;
+      if (a.xlo) {
+        // x + lo + F dt, the displacement rounded once onto the residual
+        R ex, ey, ez;
+        Pn = v3(two_sum(P.x, fma(F.x, a.dt, a.xlo[base + v]), ex),
+                two_sum(P.y, fma(F.y, a.dt, a.ylo[base + v]), ey),
+                two_sum(P.z, fma(F.z, a.dt, a.zlo[base + v]), ez));
+        a.xlo[base + v] = ex;
+        a.ylo[base + v] = ey;
+        a.zlo[base + v] = ez;
+      } else {
+        Pn = v3(fma(F.x, a.dt, P.x), fma(F.y, a.dt, P.y), fma(F.z, a.dt, P.z));
+      }
+      a.xout[base + v] = Pn.x;
+      a.yout[base + v] = Pn.y;
+      a.zout[base + v] = Pn.z;
+      sx[v] = Pn.x;
+      sy[v] = Pn.y;
+      sz[v] = Pn.z;
+      px += (double)Pn.x;
+      py += (double)Pn.y;
+      pz += (double)Pn.z;
+    }
+  }
+  if (a.flags & F_MOVE) {  // wave-uniform: statistics for the next step (cell_stats)
+    wave_lds_sync();
+    px = wsum_all(px);
+    py = wsum_all(py);
+    pz = wsum_all(pz);
+    const V3<R> C = {(R)(px / nv), (R)(py / nv), (R)(pz / nv)};
+    R r = R(0);
+    for (int v = lane; v < nv; v += DPM_WAVE) {
+      const V3<R> d = vsub(lds_v(sx, sy, sz, v), C);
+      r = fmax(r, sqrt(vdot(d, d)));
+    }
+    double vol = 0.0;
+    R e2 = R(0);
+    for (int f = lane; f < nf; f += DPM_WAVE) {
+      int i0, i1, i2;
+      wface(sf, f, i0, i1, i2);
+      const V3<R> P0 = vsub(lds_v(sx, sy, sz, i0), C);
+      const V3<R> P1 = vsub(lds_v(sx, sy, sz, i1), C);
+      const V3<R> P2 = vsub(lds_v(sx, sy, sz, i2), C);
+      vol += (double)vdot(vcross(P0, P1), P2);
+      const V3<R> e0 = vsub(P1, P0), e1 = vsub(P2, P1), ee = vsub(P0, P2);
+      e2 = fmax(e2, fmax(vdot(e0, e0), fmax(vdot(e1, e1), vdot(ee, ee))));
+    }
+    const double rd = wmax_all((double)r), e2d = wmax_all((double)e2);
+    vol = wsum_all(vol);
+    if (lane == 0) {
+      const R em = sqrt((R)e2d) * R(1.000001f);
+      a.st_out[ci] = CellStat<R>{C.x, C.y, C.z, (R)rd * R(1.0000005f) + R(1e-30f), em, l0i, R(0), R(0)};
+      a.vol_out[ci] = fabs(vol / 6.0);
+      a.em_out[ci] = em;
+    }
+  }
+  if (a.counters) {
+    // n_contact counts this lane's contacts; n_wind and n_cand are wave totals
+    const unsigned long long c = (unsigned long long)wsum_all((double)n_contact);
+    if (lane == 0) {
+      unsigned long long* slot = a.counters + 4 * ((blockIdx.x * cpw + wv) % DPM_COUNTER_SLOTS);
+      if (ci == 0) atomicAdd(&slot[0], 1ull);
+      if (n_wind) atomicAdd(&slot[1], n_wind);
+      if (c) atomicAdd(&slot[2], c);
+      if (n_cand) atomicAdd(&slot[3], n_cand);
+    }
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(256) k3d_wstep_f32(const Step3D<float> a) { k3d_wstep_body(a); }
+extern "C" __global__ void __launch_bounds__(256) k3d_wstep_f64(const Step3D<double> a) { k3d_wstep_body(a); }

@@ -15,6 +15,7 @@
 
 #include "dpm2d.hip"
 #include "dpm3d.hip"
+#include "dpm3d_wave.hip"
 
 namespace {
 
@@ -92,6 +93,10 @@ struct dpm_tissue {
   int* tpbc = nullptr;
   int *faces = nullptr, *vf = nullptr, *adj = nullptr;
   int ww = 1;
+  // wave-per-cell step (meshes of <= 256 vertices): cells per workgroup
+  // (0: not available), LDS; step_kernel 0 automatic, 1 workgroup, 2 wave
+  int wave_cpw = 0, step_kernel = 0;
+  size_t lds_w = 0;
   unsigned mask = 0;
   int cur = 0;
   bool stats_valid = false;
@@ -246,6 +251,11 @@ static bool all_finite(const R* p, size_t n) {
   for (size_t i = 0; i < n; i++)
     if (!std::isfinite(p[i])) return false;
   return true;
+}
+
+// The wave-per-cell step kernel of a precision.
+static const void* wstep_kernel(int rs) {
+  return rs == 8 ? (const void*)k3d_wstep_f64 : (const void*)k3d_wstep_f32;
 }
 
 static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
@@ -414,6 +424,15 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
   (void)hipFuncSetAttribute(rs == 8 ? (const void*)k3d_bin_f64 : (const void*)k3d_bin_f32,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->bin_lds);
+  if (nv <= kWaveMaxNv) {  // cells per workgroup: as many as fit 64 KiB of LDS, at most 4
+    int cpw = 4;
+    while (cpw > 1 && ldsw_layout(nv, nf, cpw, rs).total > 65536) cpw /= 2;
+    if (ldsw_layout(nv, nf, cpw, rs).total <= 65536) {
+      h->wave_cpw = cpw;
+      h->lds_w = ldsw_layout(nv, nf, cpw, rs).total;
+      (void)hipFuncSetAttribute(wstep_kernel(rs), hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_w);
+    }
+  }
   h->lds_vjp = ldsvjp_layout(nv, nf, rs).total;
   (void)hipFuncSetAttribute(rs == 8 ? (const void*)k3d_vjp_f64 : (const void*)k3d_vjp_f32,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_vjp);
@@ -429,6 +448,18 @@ extern "C" int dpm3d_set_cell_list(dpm_tissue* h, int mode) {
   h->cell_list_mode = mode;
   h->cells_on = mode == 1 || (mode == -1 && ncmax_t > 8 * h->NT);
   if (h->graph) {  // captured with the old arguments
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_set_step_kernel(dpm_tissue* h, int kind) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  if (kind < 0 || kind > 2) return fail(DPM_EINVAL, "kind must be 0 (automatic), 1 (workgroup) or 2 (wave)");
+  if (kind == 2 && !h->wave_cpw) return fail(DPM_EINVAL, "the wave-per-cell step needs at most 256 vertices per cell");
+  h->step_kernel = kind;
+  if (h->graph) {  // captured with the other kernel
     (void)hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
   }
@@ -697,7 +728,14 @@ static int launch3d(dpm_tissue* h, Kind3D kind, int in, float dt, int flags, Edi
       hipLaunchKernelGGL(kb, dim3(h->T), dim3(256), h->bin_lds, h->stream, b);
       HIPCHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k, dim3(h->C), dim3(h->NT), h->lds, h->stream, a);
+    if (kind == K_STEP && h->wave_cpw && h->step_kernel != 1) {
+      void (*kw)(Step3D<R>) = nullptr;
+      if constexpr (sizeof(R) == 8) kw = k3d_wstep_f64; else kw = k3d_wstep_f32;
+      hipLaunchKernelGGL(kw, dim3((h->C + h->wave_cpw - 1) / h->wave_cpw), dim3(64 * h->wave_cpw), h->lds_w,
+                         h->stream, a);
+    } else {
+      hipLaunchKernelGGL(k, dim3(h->C), dim3(h->NT), h->lds, h->stream, a);
+    }
     HIPCHECK(hipGetLastError());
     return DPM_OK;
   };

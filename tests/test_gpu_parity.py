@@ -55,17 +55,24 @@ CASES_3D = {
 }
 
 
+@pytest.mark.parametrize("kernel", [1, 2])
 @pytest.mark.parametrize("case", list(CASES_3D))
 @pytest.mark.parametrize("mask", [capi.DPM3D_REFERENCE, capi.DPM3D_ALL])
-def test_3d_forces_and_pairs(case, mask):
+def test_3d_forces_and_pairs(case, mask, kernel):
+    # kernel 1: one workgroup per cell; 2: one wave per cell (<= 256 vertices)
     n, sp, sd, seed = CASES_3D[case]
     m = W.monolayer3d(n, sp, subdiv=sd, seed=seed)
     t = tissue3d(m, mask)
+    if kernel == 2 and m.mesh.nv > 256:
+        with pytest.raises(ValueError):
+            t.set_step_kernel(2)
+        return
+    t.set_step_kernel(kernel)
     t.set_positions(m.X)
     g = t.probe()
     r64 = ref3d(m, m.X, mask, 64)
     r32 = ref3d(m, m.X, mask, 32, pairs=True)
-    assert_forces(g["forces"], r64["forces"], f"3d {case} mask={mask}")
+    assert_forces(g["forces"], r64["forces"], f"3d {case} mask={mask} kernel={kernel}")
     np.testing.assert_array_equal(g["contact"], r32["contact"])
     np.testing.assert_array_equal(g["down"], r32["down"])
     np.testing.assert_array_equal(g["adhesion"], r32["adhesion"])
@@ -92,6 +99,33 @@ def test_3d_pairs_after_deformation(spacing, seed):
     np.testing.assert_array_equal(g["adhesion"], r32["adhesion"])
     print(f"deformed pairs: contact={int(g['contact'].sum())}", flush=True)
     assert g["contact"].sum() > 0
+
+
+@pytest.mark.parametrize("mask,exact", [(capi.DPM3D_ALL & ~capi.DPM3D_ADHESION, True), (capi.DPM3D_ALL, False)])
+def test_3d_wave_and_workgroup_kernels_agree(mask, exact):
+    # the two kernel forms of the step along a 60-step trajectory of squeezed
+    # cells: bit-identical without adhesion (same arithmetic, same order);
+    # with adhesion the pair sums are grouped differently (float rounding)
+    m = W.monolayer3d(4, 1.8, seed=2)
+    out = []
+    for kind in (1, 2):
+        t = tissue3d(m, mask)
+        t.set_step_kernel(kind)
+        t.set_positions(m.X)
+        t.counters(reset=True)
+        t.euler(60, 0.002)
+        out.append((t.positions(), t.forces(), t.counters()))
+    (Xa, Fa, ca), (Xb, Fb, cb) = out
+    print(f"kernels: mask={mask} position diff {np.abs(Xa - Xb).max():.3e} force rel {rel(Fb, Fa):.3e} "
+          f"counters {ca} {cb}", flush=True)
+    assert ca[0] == cb[0]  # steps
+    if exact:
+        np.testing.assert_array_equal(Xa, Xb)
+        np.testing.assert_array_equal(Fa, Fb)
+        assert (ca[1:] == cb[1:]).all()  # windings, contacts, candidates
+    else:
+        assert np.abs(Xa - Xb).max() <= 1e-5 * np.abs(Xa).max()
+        assert rel(Fb, Fa) <= 1e-5
 
 
 def test_3d_substrate_gradient_forces():
