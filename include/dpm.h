@@ -172,9 +172,9 @@ int dpm3d_set_cell_list(dpm_tissue* h, int mode);
  * launches one work-item per vertex per force term, Tissue3D.cpp:372-445):
  * 1 one workgroup per cell; 2 one 64-lane wave per cell (meshes of at most
  * 256 vertices, several cells per workgroup, no workgroup barriers after
- * set-up); 0 (default) chooses 2 where it applies.  Volume, edge-spring,
- * substrate, bending and contact forces are bit-identical between the two;
- * adhesion sums are grouped differently (float rounding). */
+ * set-up); 0 (default) chooses 2 where it applies.  Both make the same
+ * pair decisions; the wave form sums each vertex's edge springs once per ring
+ * edge and its adhesion terms in slices, so forces agree to float rounding. */
 int dpm3d_set_step_kernel(dpm_tissue* h, int kind);
 
 /* Differentiation form (no reference counterpart).  dpm3d_trajectory runs

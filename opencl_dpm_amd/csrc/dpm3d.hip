@@ -92,6 +92,7 @@ struct Step3D {
   const int* vf;            // [nv*MAXDEG] incident (face*3 + corner), ascending, -1 pad
   const int* adj;           // [nv*MAXDEG] neighbour vertices, ascending, -1 pad
   int nv, nf, ncells;
+  int closed;  // every edge of the mesh lies on exactly two faces
   int ww;  // waves that evaluate winding numbers (LDS tables)
   R dt;
   unsigned mask;

@@ -5,11 +5,11 @@
 // (positions, force accumulators) lives in the wave's LDS slice, every
 // per-cell reduction is a wave shuffle and every list a wave ballot, and the
 // waves of a workgroup never wait for each other after the faces are
-// staged.  Shape forces are gathered per corner: each vertex evaluates its
-// incident (face, corner) terms with the arithmetic of the face pass of
-// k3d_step, in the same order, so volume, edge-spring, substrate and bending
-// forces are bit-identical to it; contact uses the same winding evaluation;
-// the adhesion sums are grouped differently (slices of one wave).
+// staged.  Shape forces are gathered per vertex: volume and substrate terms
+// per incident face, edge springs once per ring edge (each lies on two of
+// the vertex's faces), so the sums are grouped differently from the face
+// pass of k3d_step (float rounding); bending and contact use the same
+// arithmetic; the adhesion sums are grouped in slices of one wave.
 #pragma once
 
 constexpr int kWaveGlist = 128;  // gathered cell-list candidates per wave
@@ -61,6 +61,10 @@ __device__ __forceinline__ void wface(const unsigned* sf, int f, int& i0, int& i
 // Butterfly reductions: every lane receives the result.
 __device__ __forceinline__ double wsum_all(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, DPM_WAVE);
+  return v;
+}
+__device__ __forceinline__ double wmin_all(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, DPM_WAVE));
   return v;
 }
 __device__ __forceinline__ double wmax_all(double v) {
@@ -170,7 +174,11 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   const bool sub_on = (mask & M_SUBSTRATE) && !(Ksi < R(1e-12f));
   const R sc = Kai * sqrt(a0i) / l0i * R(0.3f);
   const R inv_l0 = R(1) / l0i;
-  if (shape_on && sub_on) {  // facing-down faces (Cell3D_Kernel.cl:206-246), one bit each
+  // Face pre-pass: the facing-down faces (Cell3D_Kernel.cl:206-246), one bit
+  // each, and the shortest edge of the cell.
+  bool degenerate = false;
+  if (shape_on && (sub_on || surf_on)) {  // wave-uniform
+    R e2min = R(3.0e38f);
     for (int f0 = 0; f0 < nf; f0 += DPM_WAVE) {
       const int f = f0 + lane;
       bool dn = false;
@@ -178,24 +186,65 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         int i0, i1, i2;
         wface(sf, f, i0, i1, i2);
         const V3<R> P0 = lds_v(sx, sy, sz, i0), P1 = lds_v(sx, sy, sz, i1), P2 = lds_v(sx, sy, sz, i2);
-        const V3<R> un = vnormalize(vcross(vsub(P1, P0), vsub(P2, P0)));
-        dn = un.z < R(-0.1f);
-        if (dn && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
+        const V3<R> l0v = vsub(P1, P0), l2v = vsub(P0, P2);
+        if (sub_on) {
+          const V3<R> un = vnormalize(vcross(l0v, vsub(P2, P0)));
+          dn = un.z < R(-0.1f);
+          if (dn && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
+        }
+        const V3<R> l1v = vsub(P2, P1);
+        e2min = fmin(e2min, fmin(vdot(l0v, l0v), fmin(vdot(l1v, l1v), vdot(l2v, l2v))));
       }
-      const unsigned long long m = __ballot(dn);
-      if (lane == 0) {
-        down[f0 / 32] = (unsigned)m;
-        if (f0 / 32 + 1 < (nf + 31) / 32) down[f0 / 32 + 1] = (unsigned)(m >> 32);
+      if (sub_on) {
+        const unsigned long long m = __ballot(dn);
+        if (lane == 0) {
+          down[f0 / 32] = (unsigned)m;
+          if (f0 / 32 + 1 < (nf + 31) / 32) down[f0 / 32 + 1] = (unsigned)(m >> 32);
+        }
       }
     }
+    // an edge shorter than 1e-12 makes the reference skip its faces' edge
+    // springs; such cells take the per-corner form below
+    degenerate = wmin_all((double)e2min) < 4e-24;
     wave_lds_sync();
   }
   for (int v = lane; v < nv; v += DPM_WAVE) {
     const V3<R> P = lds_v(sx, sy, sz, v);
     V3<R> F = zero;
-    if (shape_on) {
+    if (shape_on && !degenerate && a.closed) {
+      // volume and substrate per incident face, edge springs per ring edge:
+      // corner c of a face takes sc (e_c - e_(c-1)) = sc (s(a - p) + s(b - p))
+      // with s(d) = d (|d| / l0 - 1) / |d| odd and a, b its other corners
+      // (Cell3D_Kernel.cl:140-176), and every ring edge lies on two of the
+      // vertex's faces of a closed mesh.
+      int ndown = 0;
       for (int q = 0; q < DPM3D_MAXDEG; q++) {
         const int e = svf[v * DPM3D_MAXDEG + q];  // face*3 + corner, ascending
+        if (e == 0xffff) break;
+        const int f = e / 3, c = e - 3 * f;
+        if (vol_on) {  // Cell3D_Kernel.cl:95-111
+          int i0, i1, i2;
+          wface(sf, f, i0, i1, i2);
+          const int ia = c == 0 ? i1 : c == 1 ? i2 : i0, ib = c == 0 ? i2 : c == 1 ? i0 : i1;
+          F = vadd(F, vscale(vcross(vsub(lds_v(sx, sy, sz, ia), Ci), vsub(lds_v(sx, sy, sz, ib), Ci)), kvol));
+        }
+        if (sub_on) ndown += (down[f >> 5] >> (f & 31)) & 1u;
+      }
+      if (surf_on) {
+        V3<R> Fs = zero;
+        for (int q = 0; q < DPM3D_MAXDEG; q++) {
+          const int u = sadj[v * DPM3D_MAXDEG + q];
+          if (u == 0xffff) break;
+          const V3<R> d = vsub(lds_v(sx, sy, sz, u), P);
+          const R len = sqrt(vdot(d, d));
+          Fs = vadd(Fs, vscale(d, (len * inv_l0 - R(1)) / len));
+        }
+        F = vadd(F, vscale(Fs, R(2) * sc));
+      }
+      if (ndown) F = vadd(F, vscale(substrate_corner(P, Ci, Ksi, ksg, L, l0i), (R)ndown));
+    } else if (shape_on) {
+      for (int q = 0; q < DPM3D_MAXDEG; q++) {
+        const int e = svf[v * DPM3D_MAXDEG + q];
         if (e == 0xffff) break;
         const int f = e / 3, c = e - 3 * f;
         int i0, i1, i2;
@@ -203,16 +252,15 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         const V3<R> P0 = lds_v(sx, sy, sz, i0), P1 = lds_v(sx, sy, sz, i1), P2 = lds_v(sx, sy, sz, i2);
         // the face pass's corner term for this corner (k3d_step)
         V3<R> cc = zero;
-        if (vol_on) {  // Cell3D_Kernel.cl:95-111
+        if (vol_on) {
           const V3<R> A = vsub(P1, Ci), B = vsub(P2, Ci), Cc = vsub(P0, Ci);
           cc = c == 0 ? vscale(vcross(A, B), kvol) : c == 1 ? vscale(vcross(B, Cc), kvol)
                                                             : vscale(vcross(Cc, A), kvol);
         }
-        if (surf_on) {  // Cell3D_Kernel.cl:140-176
+        if (surf_on) {
           const V3<R> lv0 = vsub(P1, P0), lv1 = vsub(P2, P1), lv2 = vsub(P0, P2);
           const R len0 = sqrt(vdot(lv0, lv0)), len1 = sqrt(vdot(lv1, lv1)), len2 = sqrt(vdot(lv2, lv2));
           if (!(len0 < R(1e-12f) || len1 < R(1e-12f) || len2 < R(1e-12f))) {
-            // corner c takes e_c - e_(c-1)
             const V3<R> la = c == 0 ? lv0 : c == 1 ? lv1 : lv2;
             const V3<R> lb = c == 0 ? lv2 : c == 1 ? lv0 : lv1;
             const R na = c == 0 ? len0 : c == 1 ? len1 : len2;

@@ -93,6 +93,7 @@ struct dpm_tissue {
   int* tpbc = nullptr;
   int *faces = nullptr, *vf = nullptr, *adj = nullptr;
   int ww = 1;
+  int closed = 0;
   // wave-per-cell step (meshes of <= 256 vertices): cells per workgroup
   // (0: not available), LDS; step_kernel 0 automatic, 1 workgroup, 2 wave
   int wave_cpw = 0, step_kernel = 0;
@@ -291,6 +292,15 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
       return fail(DPM_ERANGE, "Invalid spring constants");
     if (!(d->v0[c] > 0) || !(d->a0[c] > 0)) return fail(DPM_ERANGE, "Invalid reference values");
   }
+  // closed two-manifold mesh: every undirected edge on exactly two faces
+  std::vector<int> ecount((size_t)nv * nv, 0);
+  for (int f = 0; f < nf; f++)
+    for (int k = 0; k < 3; k++) {
+      const int u = d->faces[3 * f + k], w = d->faces[3 * f + (k + 1) % 3];
+      ecount[(size_t)std::min(u, w) * nv + std::max(u, w)]++;
+    }
+  bool closed = true;
+  for (int c : ecount) closed = closed && (c == 0 || c == 2);
   // incidence (face*3 + corner) and adjacency, ascending
   std::vector<std::vector<int>> inc(nv), nb(nv);
   for (int f = 0; f < nf; f++)
@@ -320,6 +330,7 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
   h->C = C;
   h->nv = nv;
   h->nf = nf;
+  h->closed = closed ? 1 : 0;
   h->ncmax = ncmax;
   h->NT = std::max(256, (nv + 63) / 64 * 64);
   // Waves that evaluate winding numbers, each with an nv-entry direction
@@ -671,6 +682,7 @@ static Step3D<R> step_args(dpm_tissue* h, int in, float dt, int flags) {
   a.nv = h->nv;
   a.nf = h->nf;
   a.ncells = h->C;
+  a.closed = h->closed;
   a.dt = (R)dt;
   a.mask = h->mask;
   a.flags = flags;
