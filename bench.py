@@ -124,6 +124,8 @@ def main():
     ap.add_argument("--launches", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--step-kernel", type=int, default=0,
+                    help="0 the library's choice, 1 one workgroup per cell, 2 one wave per cell")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device 5 times (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -154,6 +156,7 @@ def main():
 
     log(f"[bench] plain: {m.C} cells x {m.mesh.nv} vertices, {args.plain_steps} steps")
     plain = build_tissue(capi, [m])
+    plain.set_step_kernel(args.step_kernel)
     p_ms, p_wall, p_cnt = time_steps(plain, hiptime, args.plain_steps, args.warmup, dt)
     p_launch, p_launch_min = time_launches(plain, hiptime, args.launches, dt)
     plain_rate = nvt * args.plain_steps / (p_ms / 1e3)
@@ -165,6 +168,7 @@ def main():
     log(f"[bench] batched: {B} tissues, {args.steps} steps")
     ms = [W.monolayer_config("cfg2", seed=s) for s in range(B)]
     bt = build_tissue(capi, ms)
+    bt.set_step_kernel(args.step_kernel)
     b_ms, b_wall, b_cnt = time_steps(bt, hiptime, args.steps, args.warmup, dt)
     b_launch, b_launch_min = time_launches(bt, hiptime, args.launches, dt)
     nvb = nvt * B

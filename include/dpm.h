@@ -172,7 +172,8 @@ int dpm3d_set_cell_list(dpm_tissue* h, int mode);
  * launches one work-item per vertex per force term, Tissue3D.cpp:372-445):
  * 1 one workgroup per cell; 2 one 64-lane wave per cell (meshes of at most
  * 256 vertices, several cells per workgroup, no workgroup barriers after
- * set-up); 0 (default) chooses 2 where it applies.  Both make the same
+ * set-up); 0 (default) chooses 2 for meshes it takes once the handle holds
+ * at least 24 cells per compute unit, else 1.  Both make the same
  * pair decisions; the wave form sums each vertex's edge springs once per ring
  * edge and its adhesion terms in slices, so forces agree to float rounding. */
 int dpm3d_set_step_kernel(dpm_tissue* h, int kind);

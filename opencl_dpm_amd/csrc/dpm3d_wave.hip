@@ -21,7 +21,7 @@ constexpr int kWaveCand = 128;   // candidate cells per batch
 // cell-list gather, which is finished before the first candidate.
 struct Lds3DWave {
   size_t vf, adj, fend, slice, total;
-  size_t sx, nx, fx, ws, np, ms, jl, il, down, cl, ut, gl, gath;
+  size_t sx, nx, fx, ws, np, jl, il, down, cl, ut, gl, gath;
 };
 __host__ __device__ inline Lds3DWave ldsw_layout(int nv, int nf, int cpw, int rsize) {
   Lds3DWave L;
@@ -36,7 +36,6 @@ __host__ __device__ inline Lds3DWave ldsw_layout(int nv, int nf, int cpw, int rs
   L.fx = s;   s = al(s + 3u * rs * n);  // force accumulators
   L.ws = s;   s = al(s + rs * n);       // sum of |w| over contact partners
   L.np = s;   s += 2u * n;              // contact partners recorded
-  L.ms = s;   s += 2u * n;              // slot in the own-vertex list (adhesion)
   L.jl = s;   s += 2u * n;              // candidate vertices in reach
   L.il = s;   s = al(s + 2u * n);       // own vertices in reach
   L.down = s; s = al(s + 4u * (size_t)((nf + 31) / 32));  // facing-down faces
@@ -133,7 +132,6 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   R* fz = fy + nv;
   R* ws = reinterpret_cast<R*>(slb + LW.ws);
   short* np = reinterpret_cast<short*>(slb + LW.np);
-  short* ms = reinterpret_cast<short*>(slb + LW.ms);
   unsigned short* jl = reinterpret_cast<unsigned short*>(slb + LW.jl);
   unsigned short* il = reinterpret_cast<unsigned short*>(slb + LW.il);
   Q4<R>* ut = reinterpret_cast<Q4<R>*>(slb + LW.ut);
@@ -548,7 +546,6 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
             if (near) jl[nnear + __popcll(mn & below)] = (unsigned short)v;
             const int slot = nmine + __popcll(mm & below);
             if (mine) il[slot] = (unsigned short)v;
-            if (v < nv) ms[v] = (short)(mine ? slot : -1);
             nnear += __popcll(mn);
             nmine += __popcll(mm);
           }
@@ -598,9 +595,8 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
               ut[sl * nmine + iv] = Q4<R>{Fa.x, Fa.y, Fa.z, (R)cnt_own};
             }
             wave_lds_sync();
-            for (int v = lane; v < nv; v += DPM_WAVE) {
-              const int slot = ms[v];
-              if (slot < 0) continue;
+            for (int slot = lane; slot < nmine; slot += DPM_WAVE) {
+              const int v = il[slot];
               V3<R> F = lds_v(fx, fy, fz, v);
               int cnt_own = 0;
               for (int s2 = 0; s2 < S; s2++) {

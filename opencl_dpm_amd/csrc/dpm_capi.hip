@@ -96,7 +96,7 @@ struct dpm_tissue {
   int closed = 0;
   // wave-per-cell step (meshes of <= 256 vertices): cells per workgroup
   // (0: not available), LDS; step_kernel 0 automatic, 1 workgroup, 2 wave
-  int wave_cpw = 0, step_kernel = 0;
+  int wave_cpw = 0, step_kernel = 0, wave_min_cells = 0;
   size_t lds_w = 0;
   unsigned mask = 0;
   int cur = 0;
@@ -441,6 +441,12 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
     if (ldsw_layout(nv, nf, cpw, rs).total <= 65536) {
       h->wave_cpw = cpw;
       h->lds_w = ldsw_layout(nv, nf, cpw, rs).total;
+      // automatic choice: the wave form once the batch gives every CU about
+      // 24 cells (measured crossover between 16 and 32 cells per CU on the
+      // 256-cell monolayer); below, the workgroup form fills the device better
+      int dev = 0, cus = 256;
+      if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      h->wave_min_cells = 24 * std::max(1, cus);
       (void)hipFuncSetAttribute(wstep_kernel(rs), hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_w);
     }
   }
@@ -740,7 +746,8 @@ static int launch3d(dpm_tissue* h, Kind3D kind, int in, float dt, int flags, Edi
       hipLaunchKernelGGL(kb, dim3(h->T), dim3(256), h->bin_lds, h->stream, b);
       HIPCHECK(hipGetLastError());
     }
-    if (kind == K_STEP && h->wave_cpw && h->step_kernel != 1) {
+    if (kind == K_STEP && h->wave_cpw &&
+        (h->step_kernel == 2 || (h->step_kernel == 0 && h->C >= h->wave_min_cells))) {
       void (*kw)(Step3D<R>) = nullptr;
       if constexpr (sizeof(R) == 8) kw = k3d_wstep_f64; else kw = k3d_wstep_f32;
       hipLaunchKernelGGL(kw, dim3((h->C + h->wave_cpw - 1) / h->wave_cpw), dim3(64 * h->wave_cpw), h->lds_w,

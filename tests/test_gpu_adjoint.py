@@ -150,6 +150,31 @@ def test_trajectory_gradient_vs_differences(precision, tol):
     assert err <= tol
 
 
+@pytest.mark.parametrize("precision", [64, "32c"])
+def test_adjoint_same_with_either_kernel_form(precision):
+    # the adjoint recomputes the forward with the handle's kernel form; the
+    # two forms differ by float rounding only, so do their gradients
+    m = W.monolayer3d(3, 1.75, seed=7, ks_grad=9.0)  # contact events along the way
+    X0 = m.X.astype(np.float64)
+    res = []
+    for kind in (1, 2):
+        t = tissue(m, 64 if precision == 64 else 32)
+        t.set_step_kernel(kind)
+        if precision == "32c":
+            t.set_compensated(True)
+        t.set_positions(X0 if precision == 64 else m.X)
+        t.trajectory(40, 0.002, 10)
+        L, lam = loss_and_lambda(t, m, X0)
+        res.append((L,) + tuple(t.adjoint(lam)))
+    (L1, d1, l1), (L2, d2, l2) = res
+    e = float(np.linalg.norm(d1 - d2) / np.linalg.norm(d1))
+    ex = float(np.linalg.norm(l1 - l2) / np.linalg.norm(l1))
+    print(f"adjoint f{precision}: kernels differ by {e:.3e} (parameters), {ex:.3e} (positions)", flush=True)
+    tol = 1e-9 if precision == 64 else 1e-3
+    assert abs(L1 - L2) <= tol * abs(L1)
+    assert e <= tol and ex <= tol
+
+
 def test_autograd_function():
     m = W.monolayer3d(2, 2.4, seed=3, ks_grad=9.0)
     t = tissue(m, 64)

@@ -56,12 +56,17 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("kernel", [1, 2])
 @pytest.mark.parametrize("case", list(CASES))
 @pytest.mark.parametrize("mask", [capi.DPM3D_REFERENCE, capi.DPM3D_ALL])
-def test_f64_forces_and_pairs_vs_ref64(case, mask):
+def test_f64_forces_and_pairs_vs_ref64(case, mask, kernel):
+    # kernel 1: one workgroup per cell; 2: one wave per cell (<= 256 vertices)
     n, sp, sd, seed = CASES[case]
     m = W.monolayer3d(n, sp, subdiv=sd, seed=seed)
     t = tissue3d(m, mask)
+    if kernel == 2 and m.mesh.nv > 256:
+        pytest.skip("the wave-per-cell step takes meshes of at most 256 vertices")
+    t.set_step_kernel(kernel)
     X = m.X.astype(np.float64)
     t.set_positions(X)
     np.testing.assert_array_equal(t.positions(), X)
@@ -71,7 +76,7 @@ def test_f64_forces_and_pairs_vs_ref64(case, mask):
     r = ref3d(m, X, mask, pairs=True)
     e = rel(Fg, r["forces"])
     vmax = float(np.abs(Fg - r["forces"]).max())
-    print(f"f64 {case} mask={mask}: rel={e:.3e} vmax={vmax:.3e}", flush=True)
+    print(f"f64 {case} mask={mask} kernel={kernel}: rel={e:.3e} vmax={vmax:.3e}", flush=True)
     assert e <= F64_RTOL
     assert vmax <= F64_VTOL * float(np.abs(r["forces"]).max())
     np.testing.assert_array_equal(g["contact"], r["contact"])
@@ -79,18 +84,20 @@ def test_f64_forces_and_pairs_vs_ref64(case, mask):
     np.testing.assert_array_equal(g["adhesion"], r["adhesion"])
 
 
+@pytest.mark.parametrize("kernel", [1, 2])
 @pytest.mark.parametrize("mask", [capi.DPM3D_VOLUME | capi.DPM3D_SURFACE | capi.DPM3D_BENDING,
                                   capi.DPM3D_ALL])
-def test_f64_trajectory_vs_ref64(mask):
+def test_f64_trajectory_vs_ref64(mask, kernel):
     m = W.monolayer3d(3, 1.7, seed=4)
     t = tissue3d(m, mask)
+    t.set_step_kernel(kernel)
     t.set_positions(m.X.astype(np.float64))
     nsteps, dt = 40, 0.002
     t.euler(nsteps, dt)
     X64, _ = oracle.run3d(m.X, m.mesh.faces, m.mesh.nv, f32(m.Kv), f32(m.Ka), f32(m.Ks), f32(m.v0), f32(m.a0),
                           f32(m.L), f32(m.Kre), nsteps, f32(dt), Kat=f32(m.Kat), Kb=f32(m.Kb), mask=mask, prec=64)
     err = float(np.abs(t.positions() - X64).max())
-    print(f"f64 40-step trajectory mask={mask}: max|dx|={err:.3e}", flush=True)
+    print(f"f64 40-step trajectory mask={mask} kernel={kernel}: max|dx|={err:.3e}", flush=True)
     assert err <= 1e-12
 
 
