@@ -14,7 +14,9 @@ Workload (one "step" = one overdamped Euler step of every vertex):
 `value` is the batched whole-job rate in vertex-steps/s with all inputs
 resident in HBM; the plain rate is reported beside it.
 
-roofline: the dominant (only) kernel of a step is k3d_step_f32.  Algorithmic
+roofline: the only kernel of a batched step is k3d_wstep_f32 (one wave per
+cell; the plain 256-cell tissue runs k3d_step_f32, one workgroup per cell,
+which fills the device better at that size).  Algorithmic
 bytes per vertex-step = 24 (position read + write, float xyz) + 24/162
 (per-cell state amortised) = 24.15 B; `achieved` = that x vertices per
 launch / the average launch duration measured here with HIP events
@@ -102,9 +104,9 @@ def cpu_baseline(m, steps):
 
 
 def measured_traffic_per_vertex_step():
-    """HBM bytes per vertex-step of k3d_step from the committed rocprofv3
+    """HBM bytes per vertex-step of the batched step kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (None if absent)."""
-    path = os.path.join(ROOT, "profiles", "k3d_step_monolayer_256x162.md")
+    path = os.path.join(ROOT, "profiles", "k3d_wstep_monolayer_64x256x162.md")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_vertex_step:"):
@@ -124,8 +126,9 @@ def main():
     ap.add_argument("--launches", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--step-kernel", type=int, default=0,
-                    help="0 the library's choice, 1 one workgroup per cell, 2 one wave per cell")
+    ap.add_argument("--step-kernel", type=int, default=None,
+                    help="1 one workgroup per cell, 2 one wave per cell (default: the library's "
+                         "automatic choice, which is 1 for the plain tissue and 2 for the batch)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device 5 times (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -156,7 +159,7 @@ def main():
 
     log(f"[bench] plain: {m.C} cells x {m.mesh.nv} vertices, {args.plain_steps} steps")
     plain = build_tissue(capi, [m])
-    plain.set_step_kernel(args.step_kernel)
+    plain.set_step_kernel(args.step_kernel or 0)
     p_ms, p_wall, p_cnt = time_steps(plain, hiptime, args.plain_steps, args.warmup, dt)
     p_launch, p_launch_min = time_launches(plain, hiptime, args.launches, dt)
     plain_rate = nvt * args.plain_steps / (p_ms / 1e3)
@@ -168,7 +171,8 @@ def main():
     log(f"[bench] batched: {B} tissues, {args.steps} steps")
     ms = [W.monolayer_config("cfg2", seed=s) for s in range(B)]
     bt = build_tissue(capi, ms)
-    bt.set_step_kernel(args.step_kernel)
+    b_kind = args.step_kernel or 2  # the automatic choice at 64 x 256 cells
+    bt.set_step_kernel(b_kind)
     b_ms, b_wall, b_cnt = time_steps(bt, hiptime, args.steps, args.warmup, dt)
     b_launch, b_launch_min = time_launches(bt, hiptime, args.launches, dt)
     nvb = nvt * B
@@ -206,7 +210,8 @@ def main():
         "winding_evals_per_step": windings,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "k3d_step_f32", "launch_ms": b_launch, "bytes_per_vertex_step": B_ALG},
+                     "kernel": "k3d_wstep_f32" if b_kind == 2 else "k3d_step_f32", "launch_ms": b_launch,
+                     "bytes_per_vertex_step": B_ALG},
     }
     if not args.no_cpu:
         log(f"[bench] cpu baseline: {args.cpu_steps} step(s)")
