@@ -93,6 +93,10 @@ struct Step3D {
   const int* adj;           // [nv*MAXDEG] neighbour vertices, ascending, -1 pad
   int nv, nf, ncells;
   int closed;  // every edge of the mesh lies on exactly two faces
+  // the wave form's workgroup-shared topology, byte for byte its LDS image
+  // (packed faces, 16-bit incidence and neighbour rows), in 16-byte words
+  const uint4* wtopo;
+  int wtopo_n16;
   int ww;  // waves that evaluate winding numbers (LDS tables)
   R dt;
   unsigned mask;

@@ -110,12 +110,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   unsigned* sf = reinterpret_cast<unsigned*>(smem);
   unsigned short* svf = reinterpret_cast<unsigned short*>(smem + LW.vf);
   unsigned short* sadj = reinterpret_cast<unsigned short*>(smem + LW.adj);
-  for (int f = threadIdx.x; f < nf; f += blockDim.x)
-    sf[f] = (unsigned)a.faces[3 * f] | ((unsigned)a.faces[3 * f + 1] << 10) | ((unsigned)a.faces[3 * f + 2] << 20);
-  for (int i = threadIdx.x; i < nv * DPM3D_MAXDEG; i += blockDim.x) {
-    svf[i] = (unsigned short)a.vf[i];    // -1 pads become 0xffff
-    sadj[i] = (unsigned short)a.adj[i];
-  }
+  for (int i = threadIdx.x; i < a.wtopo_n16; i += blockDim.x) reinterpret_cast<uint4*>(smem)[i] = a.wtopo[i];
   __syncthreads();  // the only workgroup barrier
   const int ci = xcd_remap(blockIdx.x, gridDim.x) * cpw + wv;
   if (ci >= a.ncells) return;  // whole wave

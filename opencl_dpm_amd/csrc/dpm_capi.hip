@@ -97,6 +97,8 @@ struct dpm_tissue {
   // wave-per-cell step (meshes of <= 256 vertices): cells per workgroup
   // (0: not available), LDS; step_kernel 0 automatic, 1 workgroup, 2 wave
   int wave_cpw = 0, step_kernel = 0, wave_min_cells = 0;
+  void* wtopo = nullptr;  // LDS image of the wave form's shared topology
+  int wtopo_n16 = 0;
   size_t lds_w = 0;
   unsigned mask = 0;
   int cur = 0;
@@ -447,6 +449,27 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
       int dev = 0, cus = 256;
       if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
       h->wave_min_cells = 24 * std::max(1, cus);
+      // the workgroup-shared topology as its LDS image: faces packed 10 bits
+      // per index, incidence and neighbour rows in 16 bits (0xffff: none)
+      const Lds3DWave LW = ldsw_layout(nv, nf, cpw, rs);
+      std::vector<unsigned char> img((LW.fend + 15) / 16 * 16, 0);
+      unsigned* pf = reinterpret_cast<unsigned*>(img.data());
+      for (int f = 0; f < nf; f++)
+        pf[f] = (unsigned)d->faces[3 * f] | ((unsigned)d->faces[3 * f + 1] << 10) |
+                ((unsigned)d->faces[3 * f + 2] << 20);
+      unsigned short* pv = reinterpret_cast<unsigned short*>(img.data() + LW.vf);
+      unsigned short* pa = reinterpret_cast<unsigned short*>(img.data() + LW.adj);
+      for (size_t i = 0; i < vf.size(); i++) {
+        pv[i] = (unsigned short)vf[i];
+        pa[i] = (unsigned short)adj[i];
+      }
+      h->wtopo_n16 = (int)(img.size() / 16);
+      unsigned char* q = nullptr;
+      if (h->alloc(&q, img.size()) || hipMemcpy(q, img.data(), img.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        delete h;
+        return fail(DPM_EDEVICE, "topology upload failed");
+      }
+      h->wtopo = q;
       (void)hipFuncSetAttribute(wstep_kernel(rs), hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_w);
     }
   }
@@ -689,6 +712,8 @@ static Step3D<R> step_args(dpm_tissue* h, int in, float dt, int flags) {
   a.nf = h->nf;
   a.ncells = h->C;
   a.closed = h->closed;
+  a.wtopo = (const uint4*)h->wtopo;
+  a.wtopo_n16 = h->wtopo_n16;
   a.dt = (R)dt;
   a.mask = h->mask;
   a.flags = flags;
