@@ -344,6 +344,18 @@ def test_bending_is_gradient_and_invariant():
     np.testing.assert_allclose(F3(2.0 * X, mesh, 32, Kb=Kb)["forces"], 2.0 * F, rtol=1e-9, atol=1e-9)
 
 
+def test_bending_on_a_ring_is_the_2d_reference_bending(poly):
+    # The 3D bending F = -Kb L^T L x (L the graph Laplacian of the mesh's
+    # edge graph) on a cycle graph is the reference's 2D bending
+    # Kb (2 s_i - s_(i-1) - s_(i+1)), s_i = x_(i+1) - 2 x_i + x_(i-1)
+    # (Cell2D_kernel.cl:44-90): the operator the 3D oracle applies to the
+    # icosphere graph in the test above.
+    Kb, n = 0.4, poly.shape[0]
+    Lc = -2.0 * np.eye(n) + np.roll(np.eye(n), 1, axis=1) + np.roll(np.eye(n), -1, axis=1)
+    F_ring = -Kb * Lc.T @ Lc @ poly
+    np.testing.assert_allclose(f2d(poly, 4, Kb=Kb), F_ring, rtol=1e-9, atol=1e-12)
+
+
 @pytest.mark.parametrize("mask", [1, 2, 32, 35])
 def test_energy_is_potential_of_shape_forces(mask):
     # the oracle's energy (used for the drift study) against the oracle's
