@@ -62,7 +62,7 @@ def build_tissue(capi, ms):
 
 
 def time_steps(t, hiptime, steps, warmup, dt):
-    t.euler_async(warmup, dt)
+    t.euler_async(max(1, warmup), dt)  # also captures the step graph for this dt
     t.synchronize()
     t.counters(reset=True)
     e0, e1 = hiptime.Event(), hiptime.Event()
@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--launches", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the batched handle's autotuner (dpm3d_autotune) and use --step-kernel")
     ap.add_argument("--step-kernel", type=int, default=None,
                     help="1 one workgroup per cell, 2 one wave per cell (default: the library's "
                          "automatic choice, which is 1 for the plain tissue and 2 for the batch)")
@@ -173,6 +175,14 @@ def main():
     bt = build_tissue(capi, ms)
     b_kind = args.step_kernel or 2  # the automatic choice at 64 x 256 cells
     bt.set_step_kernel(b_kind)
+    tuned = None
+    if not args.no_autotune and args.step_kernel is None:
+        # the autotuner on the start state (it restores it), then the
+        # warm-up and the timed steps
+        best, times = bt.autotune(steps=5, dt=dt)
+        b_kind = best["form"]
+        tuned = {"choice": best, "ms_per_step": times}
+        log(f"[bench] autotune: {best}")
     b_ms, b_wall, b_cnt = time_steps(bt, hiptime, args.steps, args.warmup, dt)
     b_launch, b_launch_min = time_launches(bt, hiptime, args.launches, dt)
     nvb = nvt * B
@@ -206,6 +216,7 @@ def main():
         "plain": {"value": plain_rate, "unit": "vertex-steps/s", "steps": args.plain_steps,
                   "ms_per_step": p_ms / args.plain_steps, "launch_ms": p_launch},
         "batched_vs_plain": rate / plain_rate,
+        "autotune": tuned,
         "hbm_gbps_algorithmic": B_ALG * rate / 1e9,
         "winding_evals_per_step": windings,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",

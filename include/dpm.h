@@ -178,6 +178,19 @@ int dpm3d_set_cell_list(dpm_tissue* h, int mode);
  * edge and its adhesion terms in slices, so forces agree to float rounding. */
 int dpm3d_set_step_kernel(dpm_tissue* h, int kind);
 
+/* Autotuner of the 3D step (no reference counterpart): times `steps` steps
+ * (after one warm-up launch) of each configuration on the handle's current
+ * state, restoring that state after every trial, and keeps the fastest.
+ * Configurations, in the order of times_ms[8] (ms per step; -1 where a
+ * configuration does not apply): workgroup form with the scan, workgroup
+ * form with the cell list, then the wave form with 1, 2, 4 cells per
+ * workgroup with the scan, and the same with the cell list.  best[3] (may
+ * be NULL) receives {form, cells per workgroup (0 for the workgroup form),
+ * cell list}; the handle keeps them (dpm3d_set_step_kernel and
+ * dpm3d_set_cell_list override).  Results do not depend on the choice
+ * beyond float rounding of the grouped sums. */
+int dpm3d_autotune(dpm_tissue* h, int steps, float dt, int* best, float* times_ms);
+
 /* Differentiation form (no reference counterpart).  dpm3d_trajectory runs
  * nsteps Euler steps (as dpm_euler) and keeps the positions every `every`
  * steps (nsteps a multiple of every) on the device.  dpm3d_adjoint then

@@ -35,7 +35,7 @@ EXPORTS = [
     "dpm3d_set_cell_params", "dpm3d_set_tissue_params", "dpm3d_cell_stats", "dpm3d_probe",
     "dpm3d_create_f64", "dpm3d_set_positions_f64", "dpm3d_get_positions_f64",
     "dpm3d_get_forces_f64", "dpm3d_energy", "dpm3d_trajectory", "dpm3d_adjoint", "dpm3d_adjoint_f64",
-    "dpm3d_set_compensated", "dpm3d_set_cell_list", "dpm3d_set_step_kernel",
+    "dpm3d_set_compensated", "dpm3d_set_cell_list", "dpm3d_set_step_kernel", "dpm3d_autotune",
     "dpm2d_create", "dpm2d_set_positions", "dpm2d_get_positions", "dpm2d_get_forces",
     "dpm2d_set_cell_params", "dpm2d_set_tissue_params", "dpm2d_probe",
     "dpm2d_create_f64", "dpm2d_set_positions_f64", "dpm2d_get_positions_f64", "dpm2d_get_forces_f64",
@@ -83,6 +83,7 @@ def load() -> ctypes.CDLL:
     lib.dpm3d_set_compensated.argtypes = [vp, i]
     lib.dpm3d_set_cell_list.argtypes = [vp, i]
     lib.dpm3d_set_step_kernel.argtypes = [vp, i]
+    lib.dpm3d_autotune.argtypes = [vp, i, fp, vp, vp]
     lib.dpm3d_adjoint.argtypes = [vp, vp, vp, vp]
     lib.dpm3d_adjoint_f64.argtypes = [vp, vp, vp, vp]
     lib.dpm2d_create.argtypes = [vp, vp]
@@ -297,6 +298,19 @@ class Tissues3D(_Handle):
         """0 automatic, 1 one workgroup per cell, 2 one wave per cell
         (dpm3d_set_step_kernel)."""
         _check(load().dpm3d_set_step_kernel(self._h, int(kind)))
+
+    AUTOTUNE_CONFIGS = ("workgroup/scan", "workgroup/cell list", "wave x1/scan", "wave x2/scan", "wave x4/scan",
+                        "wave x1/cell list", "wave x2/cell list", "wave x4/cell list")
+
+    def autotune(self, steps: int = 10, dt: float = 0.002):
+        """Times the step configurations on the current state and keeps the
+        fastest (dpm3d_autotune).  Returns ({form, cells per workgroup, cell
+        list}, {configuration: ms per step})."""
+        best = np.zeros(3, np.int32)
+        t = np.zeros(8, np.float32)
+        _check(load().dpm3d_autotune(self._h, int(steps), ctypes.c_float(dt), _p(best), _p(t)))
+        return ({"form": int(best[0]), "cells_per_workgroup": int(best[1]), "cell_list": int(best[2])},
+                {k: float(v) for k, v in zip(self.AUTOTUNE_CONFIGS, t) if v >= 0})
 
     def set_compensated(self, on: bool = True) -> None:
         """Compensated (x + residual) positions for a float tissue."""

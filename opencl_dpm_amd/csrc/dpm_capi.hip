@@ -834,6 +834,9 @@ static int euler3d(dpm_tissue* h, int nsteps, float dt) {
   if (nsteps <= 0) return fail(DPM_EINVAL, "nsteps must be positive");
   if (!(dt > 0.0f) || dt > 0.1f) return fail(DPM_EINVAL, "dt must be positive and reasonable");
   TRY(ensure_stats(h));
+  // the step graph for this dt is captured on the first call with it (a
+  // capture launches nothing), so that later calls replay it
+  if (!h->graph || h->graph_dt != dt) TRY(build_graph3d(h, dt));
   int left = nsteps - 1;  // the last step also writes the forces
   if (left >= kGraphSteps) {
     // Bring the ping-pong to buffer 0 so that the graph (captured from 0)
@@ -844,7 +847,6 @@ static int euler3d(dpm_tissue* h, int nsteps, float dt) {
       left--;
     }
     if (left >= kGraphSteps) {
-      if (!h->graph || h->graph_dt != dt) TRY(build_graph3d(h, dt));
       while (left >= kGraphSteps) {
         HIPCHECK(hipGraphLaunch(h->graph, h->stream));
         left -= kGraphSteps;
@@ -858,6 +860,117 @@ static int euler3d(dpm_tissue* h, int nsteps, float dt) {
   TRY(launch_step3d(h, h->cur, dt, F_MOVE | F_WRITE_FORCES));
   h->cur = 1 - h->cur;
   h->forces_valid = true;
+  return DPM_OK;
+}
+
+// Cells per workgroup of the wave form (1, 2 or 4, within 64 KiB of LDS).
+static int set_wave_cpw(dpm_tissue* h, int cpw) {
+  const size_t lds = ldsw_layout(h->nv, h->nf, cpw, h->rs).total;
+  if (lds > 65536) return fail(DPM_ERANGE, "cells per workgroup exceed the LDS of a workgroup");
+  h->wave_cpw = cpw;
+  h->lds_w = lds;
+  HIPCHECK(hipFuncSetAttribute(wstep_kernel(h->rs), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (h->graph) {
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_autotune(dpm_tissue* h, int steps, float dt, int* best, float* times_ms) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  if (steps < 1 || steps > 1000) return fail(DPM_EINVAL, "steps must be in 1..1000");
+  if (!(dt > 0.0f) || dt > 0.1f) return fail(DPM_EINVAL, "dt must be positive and reasonable");
+  TRY(ensure_stats(h));
+  // the state the trials start from, restored after each of them
+  const int c0 = h->cur;
+  const size_t NVT = (size_t)h->C * h->nv, rs = (size_t)h->rs, C = (size_t)h->C;
+  const size_t sz_pos = NVT * rs, sz_lo = h->lo ? 3 * NVT * rs : 0, sz_st = C * kStatWords * rs,
+               sz_vol = C * sizeof(double), sz_em = C * rs, sz_cnt = 4 * DPM_COUNTER_SLOTS * sizeof(unsigned long long);
+  const size_t total = 3 * sz_pos + sz_lo + sz_st + sz_vol + sz_em + sz_cnt;
+  unsigned char* save = nullptr;
+  TRY(dev_alloc(&save, total));
+  struct Piece { void* dev; size_t n; };
+  const Piece pieces[] = {{h->x[c0], sz_pos}, {h->y[c0], sz_pos}, {h->z[c0], sz_pos}, {h->lo, sz_lo},
+                          {h->st[c0], sz_st}, {h->vol[c0], sz_vol}, {h->em[c0], sz_em}, {h->counters, sz_cnt}};
+  auto copy_all = [&](bool to_save) -> int {
+    size_t off = 0;
+    for (const Piece& q : pieces) {
+      if (q.n && hipMemcpyAsync(to_save ? save + off : q.dev, to_save ? q.dev : save + off, q.n,
+                                hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+        return fail(DPM_EDEVICE, "state copy failed");
+      off += q.n;
+    }
+    h->cur = c0;
+    return DPM_OK;
+  };
+  int rc = copy_all(true);
+  // configurations: {form, cells per workgroup, cell list}, in the order of times_ms
+  const int cfg[8][3] = {{1, 0, 0}, {1, 0, 1}, {2, 1, 0}, {2, 2, 0}, {2, 4, 0}, {2, 1, 1}, {2, 2, 1}, {2, 4, 1}};
+  const int kind0 = h->step_kernel, cpw0 = h->wave_cpw, mode0 = h->cell_list_mode;
+  const bool cells0 = h->cells_on;
+  float tbest = 0.0f;
+  int ib = -1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(DPM_EDEVICE, "hipEventCreate failed");
+  for (int i = 0; i < 8 && !rc; i++) {
+    if (times_ms) times_ms[i] = -1.0f;
+    if (cfg[i][0] == 2 && (!h->wave_cpw || ldsw_layout(h->nv, h->nf, cfg[i][1], h->rs).total > 65536)) continue;
+    h->step_kernel = cfg[i][0];
+    if (cfg[i][0] == 2) rc = set_wave_cpw(h, cfg[i][1]);
+    h->cells_on = cfg[i][2] == 1;
+    for (int rep = 0; rep < 2 && !rc; rep++) {  // a warm-up launch, then the timed steps
+      rc = copy_all(false);
+      if (rc) break;
+      if (rep == 0) {
+        rc = launch_step3d(h, h->cur, dt, F_MOVE);
+        continue;
+      }
+      if (hipEventRecord(e0, h->stream) != hipSuccess) rc = fail(DPM_EDEVICE, "hipEventRecord failed");
+      for (int k = 0; k < steps && !rc; k++) {
+        rc = launch_step3d(h, h->cur, dt, F_MOVE);
+        h->cur = 1 - h->cur;
+      }
+      if (!rc && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+        rc = fail(DPM_EDEVICE, "timing failed");
+      float ms = 0.0f;
+      if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(DPM_EDEVICE, "timing failed");
+      if (!rc) {
+        if (times_ms) times_ms[i] = ms / (float)steps;
+        if (ib < 0 || ms < tbest) {
+          tbest = ms;
+          ib = i;
+        }
+      }
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  int rc2 = copy_all(false);
+  if (!rc && rc2) rc = rc2;
+  if (hipStreamSynchronize(h->stream) != hipSuccess && !rc) rc = fail(DPM_EDEVICE, "stream sync failed");
+  (void)hipFree(save);
+  if (rc || ib < 0) {  // leave the handle as it was
+    h->step_kernel = kind0;
+    if (cpw0) (void)set_wave_cpw(h, cpw0);
+    h->cells_on = cells0;
+    h->cell_list_mode = mode0;
+    return rc ? rc : fail(DPM_EDEVICE, "no configuration ran");
+  }
+  h->step_kernel = cfg[ib][0];
+  if (cfg[ib][0] == 2) TRY(set_wave_cpw(h, cfg[ib][1]));
+  h->cells_on = cfg[ib][2] == 1;
+  h->cell_list_mode = cfg[ib][2];
+  if (h->graph) {
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  if (best) {
+    best[0] = cfg[ib][0];
+    best[1] = cfg[ib][0] == 2 ? cfg[ib][1] : 0;
+    best[2] = cfg[ib][2];
+  }
   return DPM_OK;
 }
 

@@ -266,6 +266,38 @@ def test_3d_batch_equals_individual(kernel):
         off += n
 
 
+def test_3d_autotune_restores_state_and_keeps_a_configuration():
+    ms = [W.monolayer3d(6, 1.9, seed=30 + k) for k in range(3)]
+    cb = np.concatenate([[0], np.cumsum([m.C for m in ms])])
+    cat = lambda f: np.concatenate([np.broadcast_to(np.asarray(f(m), np.float32), (m.C,)) for m in ms])
+
+    def handle():
+        return capi.Tissues3D(ms[0].mesh.faces, cb, cat(lambda m: m.Kv), cat(lambda m: m.Ka), cat(lambda m: m.Ks),
+                              cat(lambda m: m.v0), cat(lambda m: m.a0), [m.L for m in ms], [m.Kre for m in ms],
+                              Kat=[m.Kat for m in ms], Kb=cat(lambda m: m.Kb), mask=capi.DPM3D_ALL,
+                              nv=ms[0].mesh.nv)
+
+    X0 = np.concatenate([m.X for m in ms])
+    t = handle()
+    t.set_positions(X0)
+    t.euler(5, 0.002)
+    X5, c5 = t.positions(), t.counters()
+    best, times = t.autotune(steps=3, dt=0.002)
+    print(f"autotune: best {best}; ms per step {times}", flush=True)
+    assert len(times) == 8 and all(v > 0 for v in times.values())
+    np.testing.assert_array_equal(t.positions(), X5)
+    np.testing.assert_array_equal(t.counters(), c5)
+    t.euler(20, 0.002)
+    # the same steps on a handle set to the chosen configuration explicitly
+    u = handle()
+    u.set_step_kernel(best["form"])
+    u.set_cell_list(best["cell_list"])
+    u.set_positions(X0)
+    u.euler(5, 0.002)
+    u.euler(20, 0.002)
+    np.testing.assert_array_equal(t.positions(), u.positions())
+
+
 def test_3d_dropin_class_matches_c_abi():
     m = W.monolayer3d(3, 1.8, seed=6, keep_cells=True)
     for c in m.cells:
