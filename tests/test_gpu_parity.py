@@ -52,6 +52,8 @@ CASES_3D = {
     "compressed3x3": (3, 1.7, 2, 1),
     "compressed4x4": (4, 1.8, 2, 2),
     "subdiv3_2x2": (2, 1.8, 3, 3),
+    "subdiv1_3x3": (3, 1.6, 1, 5),   # 42 vertices: fewer than a wave's lanes
+    "subdiv0_3x3": (3, 1.6, 0, 6),   # 12 vertices (icosahedra)
 }
 
 
@@ -264,6 +266,23 @@ def test_3d_batch_equals_individual(kernel):
         n = m.C * m.mesh.nv
         np.testing.assert_array_equal(Xb[off:off + n], t.positions())
         off += n
+
+
+def test_3d_open_mesh_takes_the_per_corner_path():
+    # a mesh with a face removed is not closed: the wave form's ring-edge
+    # sum does not apply and it evaluates every (face, corner) term, with
+    # the workgroup form's arithmetic
+    m = W.monolayer3d(2, 2.4, seed=8)
+    faces = np.ascontiguousarray(m.mesh.faces[1:])
+    shape = capi.DPM3D_VOLUME | capi.DPM3D_SURFACE | capi.DPM3D_SUBSTRATE | capi.DPM3D_BENDING
+    F = []
+    for kind in (1, 2):
+        t = capi.Tissues3D(faces, [0, m.C], m.Kv, m.Ka, m.Ks, m.v0, m.a0, m.L, m.Kre, Kat=m.Kat, Kb=m.Kb,
+                           mask=shape, nv=m.mesh.nv)
+        t.set_step_kernel(kind)
+        t.set_positions(m.X)
+        F.append(t.probe(contact=False)["forces"])
+    np.testing.assert_array_equal(F[0], F[1])
 
 
 def test_3d_autotune_restores_state_and_keeps_a_configuration():
