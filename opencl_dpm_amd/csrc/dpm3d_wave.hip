@@ -20,22 +20,23 @@ constexpr int kWaveCand = 128;   // candidate cells per batch
 // winding directions (or adhesion slices) share their bytes with the
 // cell-list gather, which is finished before the first candidate.
 struct Lds3DWave {
-  size_t vf, adj, fend, slice, total;
-  size_t sx, nx, fx, ws, np, jl, il, down, cl, ut, gl, gath;
+  size_t vf, adj, img, fend, slice, total;
+  size_t sx, nx, fx, jl, il, down, cl, ut, gl, gath;
 };
 __host__ __device__ inline Lds3DWave ldsw_layout(int nv, int nf, int cpw, int rsize) {
   Lds3DWave L;
   const size_t rs = (size_t)rsize, n = (size_t)nv;
   auto al = [](size_t o) { return (o + 31u) & ~(size_t)31u; };
-  L.vf = al(4u * (size_t)nf);                   // faces: i0 | i1 << 10 | i2 << 20
-  L.adj = L.vf + 2u * n * DPM3D_MAXDEG;          // incident (face*3 + corner) rows
-  L.fend = al(L.adj + 2u * n * DPM3D_MAXDEG);    // neighbour rows (0xffff: none)
+  // the topology image (global, L1/L2-resident): packed faces, then the
+  // incidence and neighbour rows, one 16-byte row of 16-bit entries per vertex
+  L.fend = al(4u * (size_t)nf);          // faces: i0 | i1 << 10 | i2 << 20 (also in LDS)
+  L.vf = L.fend;                          // incident (face*3 + corner) rows (0xffff: none)
+  L.adj = L.vf + 2u * n * DPM3D_MAXDEG;   // neighbour rows
+  L.img = al(L.adj + 2u * n * DPM3D_MAXDEG);
   size_t s = 0;
   L.sx = s;   s = al(s + 3u * rs * n);  // own positions (x | y | z)
   L.nx = s;   s = al(s + 3u * rs * n);  // candidate positions; bending scratch
   L.fx = s;   s = al(s + 3u * rs * n);  // force accumulators
-  L.ws = s;   s = al(s + rs * n);       // sum of |w| over contact partners
-  L.np = s;   s += 2u * n;              // contact partners recorded
   L.jl = s;   s += 2u * n;              // candidate vertices in reach
   L.il = s;   s = al(s + 2u * n);       // own vertices in reach
   L.down = s; s = al(s + 4u * (size_t)((nf + 31) / 32));  // facing-down faces
@@ -48,6 +49,12 @@ __host__ __device__ inline Lds3DWave ldsw_layout(int nv, int nf, int cpw, int rs
   L.slice = s > ut_end ? s : ut_end;
   L.total = L.fend + (size_t)cpw * L.slice;
   return L;
+}
+
+// Entry q of a 16-byte row of eight 16-bit entries.
+__device__ __forceinline__ int row16(uint4 r, int q) {
+  const unsigned w = q < 4 ? (q < 2 ? r.x : r.y) : (q < 6 ? r.z : r.w);
+  return (int)((w >> ((q & 1) * 16)) & 0xffffu);
 }
 
 __device__ __forceinline__ void wface(const unsigned* sf, int f, int& i0, int& i1, int& i2) {
@@ -108,9 +115,8 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   const int cpw = blockDim.x / DPM_WAVE;
   const Lds3DWave LW = ldsw_layout(nv, nf, cpw, (int)sizeof(R));
   unsigned* sf = reinterpret_cast<unsigned*>(smem);
-  unsigned short* svf = reinterpret_cast<unsigned short*>(smem + LW.vf);
-  unsigned short* sadj = reinterpret_cast<unsigned short*>(smem + LW.adj);
-  for (int i = threadIdx.x; i < a.wtopo_n16; i += blockDim.x) reinterpret_cast<uint4*>(smem)[i] = a.wtopo[i];
+  const uint4* rows = a.wtopo;  // 16-byte words of the topology image
+  for (int i = threadIdx.x; i < (int)(LW.fend / 16); i += blockDim.x) reinterpret_cast<uint4*>(smem)[i] = rows[i];
   __syncthreads();  // the only workgroup barrier
   const int ci = xcd_remap(blockIdx.x, gridDim.x) * cpw + wv;
   if (ci >= a.ncells) return;  // whole wave
@@ -125,8 +131,6 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   R* fx = reinterpret_cast<R*>(slb + LW.fx);
   R* fy = fx + nv;
   R* fz = fy + nv;
-  R* ws = reinterpret_cast<R*>(slb + LW.ws);
-  short* np = reinterpret_cast<short*>(slb + LW.np);
   unsigned short* jl = reinterpret_cast<unsigned short*>(slb + LW.jl);
   unsigned short* il = reinterpret_cast<unsigned short*>(slb + LW.il);
   Q4<R>* ut = reinterpret_cast<Q4<R>*>(slb + LW.ut);
@@ -153,8 +157,9 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
     sx[v] = a.xin[base + v];
     sy[v] = a.yin[base + v];
     sz[v] = a.zin[base + v];
-    ws[v] = R(0);
-    np[v] = 0;
+    if (a.w_out) a.w_out[base + v] = R(0);
+    if (a.rec_cell)
+      for (int q = 0; q < DPM3D_MAXPART; q++) a.rec_cell[(base + v) * DPM3D_MAXPART + q] = -1;
   }
   wave_lds_sync();
 
@@ -211,8 +216,9 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       // (Cell3D_Kernel.cl:140-176), and every ring edge lies on two of the
       // vertex's faces of a closed mesh.
       int ndown = 0;
+      const uint4 vrow = rows[LW.vf / 16 + v];
       for (int q = 0; q < DPM3D_MAXDEG; q++) {
-        const int e = svf[v * DPM3D_MAXDEG + q];  // face*3 + corner, ascending
+        const int e = row16(vrow, q);  // face*3 + corner, ascending
         if (e == 0xffff) break;
         const int f = e / 3, c = e - 3 * f;
         if (vol_on) {  // Cell3D_Kernel.cl:95-111
@@ -225,8 +231,9 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       }
       if (surf_on) {
         V3<R> Fs = zero;
+        const uint4 arow = rows[LW.adj / 16 + v];
         for (int q = 0; q < DPM3D_MAXDEG; q++) {
-          const int u = sadj[v * DPM3D_MAXDEG + q];
+          const int u = row16(arow, q);
           if (u == 0xffff) break;
           const V3<R> d = vsub(lds_v(sx, sy, sz, u), P);
           const R len = sqrt(vdot(d, d));
@@ -236,8 +243,9 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       }
       if (ndown) F = vadd(F, vscale(substrate_corner(P, Ci, Ksi, ksg, L, l0i), (R)ndown));
     } else if (shape_on) {
+      const uint4 vrow = rows[LW.vf / 16 + v];
       for (int q = 0; q < DPM3D_MAXDEG; q++) {
-        const int e = svf[v * DPM3D_MAXDEG + q];
+        const int e = row16(vrow, q);
         if (e == 0xffff) break;
         const int f = e / 3, c = e - 3 * f;
         int i0, i1, i2;
@@ -277,8 +285,9 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
     for (int v = lane; v < nv; v += DPM_WAVE) {
       const V3<R> P = lds_v(sx, sy, sz, v);
       V3<R> s = zero;
+      const uint4 arow = rows[LW.adj / 16 + v];
       for (int q = 0; q < DPM3D_MAXDEG; q++) {
-        const int u = sadj[v * DPM3D_MAXDEG + q];
+        const int u = row16(arow, q);
         if (u == 0xffff) break;
         s = vadd(s, vsub(lds_v(sx, sy, sz, u), P));
       }
@@ -290,8 +299,9 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
     for (int v = lane; v < nv; v += DPM_WAVE) {
       V3<R> acc = zero;
       int deg = 0;
+      const uint4 arow = rows[LW.adj / 16 + v];
       for (int q = 0; q < DPM3D_MAXDEG; q++) {
-        const int u = sadj[v * DPM3D_MAXDEG + q];
+        const int u = row16(arow, q);
         if (u == 0xffff) break;
         deg++;
         acc = vadd(acc, lds_v(nx, ny, nz, u));
@@ -499,11 +509,13 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
                 fy[v] = F.y;
                 fz[v] = F.z;
                 n_contact++;
-                ws[v] += fabs(wk);
-                if (a.rec_cell) {
-                  const int npart = np[v];
-                  if (npart < DPM3D_MAXPART) {
-                    const size_t r = (base + v) * DPM3D_MAXPART + npart;
+                if (a.w_out) a.w_out[base + v] += fabs(wk);
+                if (a.rec_cell) {  // the first free of the vertex's partner slots
+                  const size_t r0 = (base + v) * DPM3D_MAXPART;
+                  int q = 0;
+                  while (q < DPM3D_MAXPART && a.rec_cell[r0 + q] >= 0) q++;
+                  if (q < DPM3D_MAXPART) {
+                    const size_t r = r0 + q;
                     a.rec_cell[r] = cj2;
                     a.rec_w[r] = wk;
                     a.rec_shift[3 * r] = sh.x;
@@ -512,7 +524,6 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
                   } else {
                     atomicOr(a.rec_overflow, 1);
                   }
-                  np[v] = (short)(npart + 1);
                 }
                 if (a.p_contact) a.p_contact[pidx] = 1;
               }
@@ -615,9 +626,6 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   double px = 0.0, py = 0.0, pz = 0.0;
   for (int v = lane; v < nv; v += DPM_WAVE) {
     const V3<R> F = lds_v(fx, fy, fz, v);
-    if (a.w_out) a.w_out[base + v] = ws[v];
-    if (a.rec_cell)
-      for (int q = np[v]; q < DPM3D_MAXPART; q++) a.rec_cell[(base + v) * DPM3D_MAXPART + q] = -1;
     if (a.flags & F_WRITE_FORCES) {
       a.fx[base + v] = F.x;
       a.fy[base + v] = F.y;
@@ -694,5 +702,5 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   }
 }
 
-extern "C" __global__ void __launch_bounds__(256) k3d_wstep_f32(const Step3D<float> a) { k3d_wstep_body(a); }
+extern "C" __global__ void __launch_bounds__(256, 4) k3d_wstep_f32(const Step3D<float> a) { k3d_wstep_body(a); }
 extern "C" __global__ void __launch_bounds__(256) k3d_wstep_f64(const Step3D<double> a) { k3d_wstep_body(a); }

@@ -449,10 +449,10 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
       int dev = 0, cus = 256;
       if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
       h->wave_min_cells = 24 * std::max(1, cus);
-      // the workgroup-shared topology as its LDS image: faces packed 10 bits
-      // per index, incidence and neighbour rows in 16 bits (0xffff: none)
+      // the topology image: faces packed 10 bits per index (staged in
+      // LDS), incidence and neighbour rows of 16-bit entries (0xffff: none)
       const Lds3DWave LW = ldsw_layout(nv, nf, cpw, rs);
-      std::vector<unsigned char> img((LW.fend + 15) / 16 * 16, 0);
+      std::vector<unsigned char> img(LW.img, 0);
       unsigned* pf = reinterpret_cast<unsigned*>(img.data());
       for (int f = 0; f < nf; f++)
         pf[f] = (unsigned)d->faces[3 * f] | ((unsigned)d->faces[3 * f + 1] << 10) |
