@@ -429,48 +429,13 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       }
       n_cand += (unsigned long long)ncl;
       wave_lds_sync();
-      // each candidate's statistics and vertices are loaded while the
-      // previous candidate is processed
-      int cjn = 0;
-      CellStat<R> stn = {};
-      R pfx[kWaveMaxNv / DPM_WAVE], pfy[kWaveMaxNv / DPM_WAVE], pfz[kWaveMaxNv / DPM_WAVE];
-      if (ncl > 0) {
-        cjn = cl[0];
-        stn = a.st_in[cjn];
-#pragma unroll
-        for (int r = 0; r < kWaveMaxNv / DPM_WAVE; r++) {
-          const int v = lane + DPM_WAVE * r;
-          if (v < nv) {
-            pfx[r] = a.xin[(size_t)cjn * nv + v];
-            pfy[r] = a.yin[(size_t)cjn * nv + v];
-            pfz[r] = a.zin[(size_t)cjn * nv + v];
-          }
-        }
-      }
       for (int qc = 0; qc < ncl; qc++) {
-        const int cj2 = cjn;
-        const CellStat<R> stj = stn;
-#pragma unroll
-        for (int r = 0; r < kWaveMaxNv / DPM_WAVE; r++) {
-          const int v = lane + DPM_WAVE * r;
-          if (v < nv) {
-            nx[v] = pfx[r];
-            ny[v] = pfy[r];
-            nz[v] = pfz[r];
-          }
-        }
-        if (qc + 1 < ncl) {
-          cjn = cl[qc + 1];
-          stn = a.st_in[cjn];
-#pragma unroll
-          for (int r = 0; r < kWaveMaxNv / DPM_WAVE; r++) {
-            const int v = lane + DPM_WAVE * r;
-            if (v < nv) {
-              pfx[r] = a.xin[(size_t)cjn * nv + v];
-              pfy[r] = a.yin[(size_t)cjn * nv + v];
-              pfz[r] = a.zin[(size_t)cjn * nv + v];
-            }
-          }
+        const int cj2 = cl[qc];
+        const CellStat<R> stj = a.st_in[cj2];
+        for (int v = lane; v < nv; v += DPM_WAVE) {
+          nx[v] = a.xin[(size_t)cj2 * nv + v];
+          ny[v] = a.yin[(size_t)cj2 * nv + v];
+          nz[v] = a.zin[(size_t)cj2 * nv + v];
         }
         V3<R> sh = zero;
         if (pbc)
