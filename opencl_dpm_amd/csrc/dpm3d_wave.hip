@@ -2,10 +2,12 @@
 //
 // The same step as k3d_step (dpm3d.hip) for meshes of at most 256
 // vertices: lane l handles vertices l, l + 64, ..., the per-vertex state
-// (positions, force accumulators) lives in the wave's LDS slice, every
-// per-cell reduction is a wave shuffle and every list a wave ballot, and the
-// waves of a workgroup never wait for each other after the faces are
-// staged.  Shape forces are gathered per vertex: volume and substrate terms
+// (positions, force accumulators) lives in the wave's LDS slice, the
+// incidence and neighbour rows are read as 16-byte words of an L1-resident
+// topology image, every per-cell reduction is a wave shuffle and every list
+// a wave ballot, and the waves of a workgroup never wait for each other
+// after the faces are staged.  The LDS slice and a 128-register budget are
+// sized for four waves per SIMD.  Shape forces are gathered per vertex: volume and substrate terms
 // per incident face, edge springs once per ring edge (each lies on two of
 // the vertex's faces), so the sums are grouped differently from the face
 // pass of k3d_step (float rounding); bending and contact use the same
