@@ -658,9 +658,10 @@ __device__ int fire_min(const Relax2D<R>& a, const Tissue2DCtx<R>& T, const Lds2
       P += (double)fx * vx + (double)fy * vy;
       F2 += (double)fx * fx + (double)fy * fy;
       v2 += vx * vx + vy * vy;
-      fm = fmax(fm, sqrt((double)fx * fx + (double)fy * fy));
+      fm = fmax(fm, (double)fx * fx + (double)fy * fy);  // squared: one square root after the max
     }
     block_reduce5(P, F2, v2, ui, fm, l.red);
+    fm = sqrt(fm);  // correctly rounded and monotone: the max of the roots
     if (u_clear >= 0.0 && ui / T.nc <= u_clear) {
       cleared = true;
       break;
@@ -715,10 +716,11 @@ __device__ void energy_pass(const Relax2D<R>& a, const Tissue2DCtx<R>& T, const 
     force_vertex(a.g, T, l, k, fx, fy, &no, &na, &u, &ui);
     a.fx[T.v0 + k] = fx;
     a.fy[T.v0 + k] = fy;
-    fm = fmax(fm, sqrt((double)fx * fx + (double)fy * fy));
+    fm = fmax(fm, (double)fx * fx + (double)fy * fy);
   }
   double unused2 = 0.0;
   block_reduce5(u, ui, unused, unused2, fm, l.red);
+  fm = sqrt(fm);
   U = u + ui;
   Uint = ui;
   fmax_out = fm;
