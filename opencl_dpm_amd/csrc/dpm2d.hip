@@ -229,6 +229,26 @@ __device__ void cand_pass(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Lds
   }
 }
 
+// True when no vertex of cell cj can lie within `cut` of point p under the
+// per-pair minimum image: per axis the minimum-image gap between p and cj's
+// axis-aligned box exceeds the cut-off, and the box is small against L (then
+// no other image comes closer) — maybe_pair's argument with a point for the
+// first box.  Skipping such a cell leaves every sum unchanged.
+template <typename R>
+__device__ __forceinline__ bool vertex_far_from_box(const Lds2D<R>& l, const Tissue2DCtx<R>& T, int cj, R px, R py,
+                                                    R cut) {
+  const R L = T.L;
+  const R hx = R(0.5f) * (l.ax1[cj] - l.ax0[cj]), hy = R(0.5f) * (l.ay1[cj] - l.ay0[cj]);
+  R dx = R(0.5f) * (l.ax0[cj] + l.ax1[cj]) - px, dy = R(0.5f) * (l.ay0[cj] + l.ay1[cj]) - py;
+  if (T.pbc) {
+    if (!(hx + cut < R(0.45f) * L && hy + cut < R(0.45f) * L)) return false;
+    dx -= L * round(dx / L);
+    dy -= L * round(dy / L);
+  }
+  const R reach = cut * R(1.0001f) + R(1e-5f) * (R(1) + L);
+  return fabs(dx) - hx > reach || fabs(dy) - hy > reach;
+}
+
 // Crossing parity of point p against cell cj, Cell2D_kernel.cl:171-196.
 template <typename R>
 __device__ __forceinline__ bool crossing_odd(const Lds2D<R>& l, int cj, R px, R py, int pbc, R L) {
@@ -334,7 +354,7 @@ __device__ __noinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DCtx<
     int cnt = 0;
     for (int q = 0; q < ncl; q++) {
       const int cj = scan_all ? q : (int)l.cand[ci * DPM2D_MAXCAND + q];
-      if (cj == ci) continue;
+      if (cj == ci || vertex_far_from_box(l, T, cj, px, py, l0)) continue;
       for (int j = l.vb[cj]; j < l.vb[cj + 1]; j++) {
         R rx = l.x[j] - px, ry = l.y[j] - py;
         R nx = R(0), ny = R(0);
@@ -371,6 +391,7 @@ __device__ __noinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DCtx<
       const int cj = scan_all ? q : (int)l.cand[ci * DPM2D_MAXCAND + q];
       if (cj == ci) continue;
       const R sig = R(0.5f) * (l0i + (R)g.l0[T.c0 + cj]);
+      if (vertex_far_from_box(l, T, cj, px, py, sig)) continue;
       for (int j = l.vb[cj]; j < l.vb[cj + 1]; j++) {
         R rx = l.x[j] - px, ry = l.y[j] - py;
         if (T.pbc && !(fabs(rx) < half_l && fabs(ry) < half_l)) {
