@@ -280,7 +280,7 @@ __device__ __forceinline__ bool crossing_odd(const Lds2D<R>& l, int cj, R px, R 
 // k -> k+1, bending at k, the cell's area energy at its first vertex) and
 // with `e_int` half of each vertex-repulsion pair energy.
 template <typename R>
-__device__ __noinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Lds2D<R>& l,
+__device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Lds2D<R>& l,
                                           int k, R& Fx, R& Fy, int* n_over, int* n_attr, double* energy,
                                           double* e_int) {
   const int ci = cell_of(l.vb, T.nc, k);
