@@ -27,6 +27,7 @@ using namespace dpm;
 
 #define DPM2D_MAXCAND 48
 #define DPM2D_RED 88  // doubles of reduction scratch
+#define DPM2D_CSPLIT 8  // threads per cell in the candidate pass (chunks of <= 32 cells)
 
 enum : unsigned {
   M2_AREA = 1u,
@@ -65,6 +66,8 @@ __host__ __device__ inline size_t lds2d_bytes(int nvt, int nc, int rs) {
   b = (b + 7u) & ~(size_t)7u;
   b += (size_t)rs * 8u * (size_t)nc + sizeof(int) * (size_t)nc;
   b += sizeof(short) * (size_t)nc * DPM2D_MAXCAND;
+  b = (b + 3u) & ~(size_t)3u;
+  b += sizeof(unsigned) * (size_t)nc * DPM2D_CSPLIT;
   b = (b + 7u) & ~(size_t)7u;
   return b + sizeof(double) * DPM2D_RED;
 }
@@ -76,6 +79,7 @@ struct Lds2D {
   R *strain, *orient, *comx, *comy, *ax0, *ax1, *ay0, *ay1;
   int* ncand;
   short* cand;
+  unsigned* cbits;  // [nc * CSPLIT] candidate bits of one chunk of cells
   double* red;
 };
 
@@ -102,6 +106,9 @@ __device__ __forceinline__ Lds2D<R> carve2d(unsigned char* smem, int nvt, int nc
   o += sizeof(int) * (size_t)nc;
   l.cand = reinterpret_cast<short*>(smem + o);
   o += sizeof(short) * (size_t)nc * DPM2D_MAXCAND;
+  o = (o + 3u) & ~(size_t)3u;
+  l.cbits = reinterpret_cast<unsigned*>(smem + o);
+  o += sizeof(unsigned) * (size_t)nc * DPM2D_CSPLIT;
   o = (o + 7u) & ~(size_t)7u;
   l.red = reinterpret_cast<double*>(smem + o);
   return l;
@@ -216,9 +223,37 @@ __device__ bool maybe_pair(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Ld
 
 template <typename R>
 __device__ void cand_pass(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Lds2D<R>& l) {
-  for (int ci = threadIdx.x; ci < T.nc; ci += blockDim.x) {
+  const int nc = T.nc, NT = blockDim.x;
+  const int S = max(1, min(DPM2D_CSPLIT, NT / max(1, nc)));  // threads per cell
+  const int chunk = (nc + S - 1) / S;
+  if (chunk <= 32) {  // block-uniform: each thread tests one chunk of <= 32 cells into a bit mask
+    for (int w = threadIdx.x; w < nc * S; w += NT) {
+      const int ci = w % nc, sp = w / nc;
+      unsigned bits = 0u;
+      const int c1 = min(nc, (sp + 1) * chunk);
+      for (int cj = sp * chunk; cj < c1; cj++)
+        if (cj != ci && maybe_pair(g, T, l, ci, cj)) bits |= 1u << (cj - sp * chunk);
+      l.cbits[ci * DPM2D_CSPLIT + sp] = bits;
+    }
+    __syncthreads();
+    for (int ci = threadIdx.x; ci < nc; ci += NT) {  // the list in ascending cell order
+      int n = 0;
+      for (int sp = 0; sp < S; sp++) {
+        unsigned bits = l.cbits[ci * DPM2D_CSPLIT + sp];
+        while (bits) {
+          const int cj = sp * chunk + __ffs((int)bits) - 1;
+          bits &= bits - 1u;
+          if (n < DPM2D_MAXCAND) l.cand[ci * DPM2D_MAXCAND + n] = (short)cj;
+          n++;
+        }
+      }
+      l.ncand[ci] = n;  // > MAXCAND: the vertex loops scan every cell
+    }
+    return;
+  }
+  for (int ci = threadIdx.x; ci < nc; ci += NT) {
     int n = 0;
-    for (int cj = 0; cj < T.nc; cj++) {
+    for (int cj = 0; cj < nc; cj++) {
       if (cj == ci) continue;
       if (maybe_pair(g, T, l, ci, cj)) {
         if (n < DPM2D_MAXCAND) l.cand[ci * DPM2D_MAXCAND + n] = (short)cj;
