@@ -176,6 +176,30 @@ __device__ __forceinline__ R contact_margin(R Rj, R emax_j) {
   return emax_j * R(0.4004f) + R(1e-6f) * Rj;
 }
 
+// Exact outside test ahead of a winding number, evaluated by one whole wave
+// (every lane calls it with the same arguments).  With d = p - C_j and
+// h = max_v (x_v - c_j).d over cell j's vertices (x_v unshifted, c_j its
+// unshifted centre), every point of j's convex hull lies in
+// {x : (x - C_j).d <= h}, so p is at least (|d|^2 - h) / |d| from every
+// face of j.  Where that distance is at least the culling margin m_j, the
+// argument of contact_margin holds (no face is skipped, the point is outside
+// the hull): the winding number is 0 up to rounding and the contact test
+// rejects the point.  The margin is raised by 1e-4 (|d| + R_j) and by 1e-6
+// of the coordinates' magnitude (c_j and the shifted C_j) for the rounding of
+// these float sums.
+template <typename R>
+__device__ __forceinline__ bool outside_by_margin(const R* xs, const R* ys, const R* zs, int nv, R cx, R cy,
+                                                  R cz, V3<R> Cj, V3<R> d, R m, R Rj) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1);
+  R h = R(-3.0e38f);
+  for (int v = lane; v < nv; v += DPM_WAVE)
+    h = fmax(h, ((xs[v] - cx) * d.x + (ys[v] - cy) * d.y) + (zs[v] - cz) * d.z);
+  for (int o = 32; o > 0; o >>= 1) h = fmax(h, __shfl_xor(h, o, DPM_WAVE));
+  const R dn = sqrt(vdot(d, d));
+  const R M = ((fabs(cx) + fabs(cy)) + fabs(cz)) + ((fabs(Cj.x) + fabs(Cj.y)) + fabs(Cj.z));
+  return dn > R(0) && dn * dn - h >= ((m + R(1e-4f) * (dn + Rj)) + R(1e-6f) * M) * dn;
+}
+
 // LDS layout (byte offsets), shared by host (sizing) and device (carving).
 // The per-(face, corner) shape forces are dead once every vertex gathered
 // them, so they share their bytes with the tables of the later passes
@@ -795,7 +819,8 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
         }
         if (do_contact) {
           const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
-          const R rj = (stj.r + contact_margin(stj.r, stj.em)) * R(1.0001f) + R(1e-6f);
+          const R mj = contact_margin(stj.r, stj.em);
+          const R rj = (stj.r + mj) * R(1.0001f) + R(1e-6f);
           bool need = false;
           if (own) {
             const V3<R> d = vsub(P, Cj);
@@ -803,11 +828,16 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
             need = d2 <= rj * rj;
           }
           const int np = block_compact(need, tid, l.plist, l.cnt);
-          n_wind += np;
           if (wave < a.ww) {
-            for (int pi = wave; pi < np; pi += a.ww) {
+            for (int pi = wave; pi < np; pi += a.ww) {  // wave-uniform
               const int v = l.plist[pi];
-              const R wv = winding_wave(l, ut, nv, nf, sh, lds_v(l.sx, l.sy, l.sz, v));
+              const V3<R> Pv = lds_v(l.sx, l.sy, l.sz, v);
+              // a point beyond the margin from j's hull has winding number 0
+              R wv = R(0);
+              if (!outside_by_margin(l.nx, l.ny, l.nz, nv, stj.cx, stj.cy, stj.cz, Cj, vsub(Pv, Cj), mj, stj.r)) {
+                wv = winding_wave(l, ut, nv, nf, sh, Pv);
+                if (lane == 0) n_wind++;
+              }
               if (lane == 0) l.wb[v] = wv;
             }
           }
@@ -969,12 +999,13 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
   }
   if (a.counters) {
     const double c = block_sum((double)n_contact, l.red);
+    const double nw = block_sum((double)n_wind, l.red);  // counted by lane 0 of each wave
     if (tid == 0) {
       // spread over DPM_COUNTER_SLOTS slots so that workgroups do not
       // serialise on one address; the host sums the slots
       unsigned long long* slot = a.counters + 4 * (blockIdx.x % DPM_COUNTER_SLOTS);
       if (blockIdx.x == 0) atomicAdd(&slot[0], 1ull);
-      if (n_wind) atomicAdd(&slot[1], n_wind);
+      if (nw != 0.0) atomicAdd(&slot[1], (unsigned long long)nw);
       if (c != 0.0) atomicAdd(&slot[2], (unsigned long long)c);
       if (n_cand) atomicAdd(&slot[3], n_cand);
     }

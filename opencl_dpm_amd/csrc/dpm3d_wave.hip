@@ -446,7 +446,8 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         wave_lds_sync();
         if (do_contact) {
           const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
-          const R rj = (stj.r + contact_margin(stj.r, stj.em)) * R(1.0001f) + R(1e-6f);
+          const R mj = contact_margin(stj.r, stj.em);
+          const R rj = (stj.r + mj) * R(1.0001f) + R(1e-6f);
           for (int k0v = 0; k0v < nv; k0v += DPM_WAVE) {
             const int v = k0v + lane;
             bool need = false;
@@ -457,12 +458,15 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
               need = vdot(d, d) <= rj * rj;
             }
             unsigned long long nm = __ballot(need);
-            n_wind += (unsigned long long)__popcll(nm);
             R wk = R(0);
-            while (nm) {
+            while (nm) {  // wave-uniform: one point at a time
               const int bb = __ffsll((long long)nm) - 1;
               nm &= nm - 1ull;
-              const R w = winding_w(nx, ny, nz, sf, ut, nv, nf, sh, lds_v(sx, sy, sz, k0v + bb));
+              const V3<R> Pb = lds_v(sx, sy, sz, k0v + bb);
+              // a point beyond the margin from j's hull has winding number 0
+              if (outside_by_margin(nx, ny, nz, nv, stj.cx, stj.cy, stj.cz, Cj, vsub(Pb, Cj), mj, stj.r)) continue;
+              n_wind++;
+              const R w = winding_w(nx, ny, nz, sf, ut, nv, nf, sh, Pb);
               if (lane == bb) wk = w;
             }
             if (need) {

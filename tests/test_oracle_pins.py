@@ -414,7 +414,9 @@ def test_winding_matches_independent_solid_angles():
 @pytest.mark.parametrize("subdiv,noise", [(1, 0.03), (2, 0.02), (2, 0.05), (3, 0.01)])
 def test_contact_culling_margin(subdiv, noise):
     # The step skips the winding number of a point farther than
-    # R_j + 0.4 e_max from a cell's centre.  Lemma behind it: every point at
+    # R_j + 0.4 e_max from a cell's centre, and of a point beyond
+    # h + 0.4 e_max along its own direction d from the centre, h the support
+    # max_v (x_v - C).d / |d| (outside_by_margin).  Lemma behind both: every point at
     # least 0.4 e_max from the convex hull (here: beyond the support plane in
     # its own direction) sees each face with denominator >= 0.34, so no face
     # is skipped and the float winding number is noise far below the 1e-6
