@@ -90,9 +90,9 @@ struct Step3D {
   const float* tksg;
   const int* faces;         // [nf*3]
   const int* vf;            // [nv*MAXDEG] incident (face*3 + corner), ascending, -1 pad
-  const int* adj;           // [nv*MAXDEG] neighbour vertices, ascending, -1 pad
+  const int* adj;           // [nv*MAXDEG] neighbour vertices (ring order if closed), -1 pad
   int nv, nf, ncells;
-  int closed;  // every edge of the mesh lies on exactly two faces
+  int closed;  // every edge on two faces, every vertex's faces one oriented cycle
   // the wave form's workgroup-shared topology, byte for byte its LDS image
   // (packed faces, 16-bit incidence and neighbour rows), in 16-byte words
   const uint4* wtopo;

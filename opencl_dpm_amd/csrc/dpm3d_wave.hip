@@ -188,8 +188,19 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         const V3<R> P0 = lds_v(sx, sy, sz, i0), P1 = lds_v(sx, sy, sz, i1), P2 = lds_v(sx, sy, sz, i2);
         const V3<R> l0v = vsub(P1, P0), l2v = vsub(P0, P2);
         if (sub_on) {
-          const V3<R> un = vnormalize(vcross(l0v, vsub(P2, P0)));
-          dn = un.z < R(-0.1f);
+          // un.z < -0.1 for un = n / |n|: decided from n.z and |n|^2 where
+          // they are clear of the threshold by 1e-5 (relative), else by the
+          // normalised vector itself
+          const V3<R> n = vcross(l0v, vsub(P2, P0));
+          const R q = n.z * n.z, t = (R(0.1f) * R(0.1f)) * vdot(n, n);
+          if (n.z >= R(0) || q < t * R(0.99999f)) {
+            dn = false;
+          } else if (q > t * R(1.00001f)) {
+            dn = true;
+          } else {
+            const V3<R> un = vnormalize(n);
+            dn = un.z < R(-0.1f);
+          }
           if (dn && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
         }
         const V3<R> l1v = vsub(P2, P1);
@@ -208,42 +219,55 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
     degenerate = wmin_all((double)e2min) < 4e-24;
     wave_lds_sync();
   }
+  const bool bend_on = (mask & M_BENDING) && Kbi != R(0);
+  const bool ring_pass = shape_on && !degenerate && a.closed;  // wave-uniform
   for (int v = lane; v < nv; v += DPM_WAVE) {
     const V3<R> P = lds_v(sx, sy, sz, v);
     V3<R> F = zero;
-    if (shape_on && !degenerate && a.closed) {
-      // volume and substrate per incident face, edge springs per ring edge:
-      // corner c of a face takes sc (e_c - e_(c-1)) = sc (s(a - p) + s(b - p))
-      // with s(d) = d (|d| / l0 - 1) / |d| odd and a, b its other corners
-      // (Cell3D_Kernel.cl:140-176), and every ring edge lies on two of the
-      // vertex's faces of a closed mesh.
-      int ndown = 0;
-      const uint4 vrow = rows[LW.vf / 16 + v];
+    if (ring_pass) {
+      // One walk around the vertex's ring u_0 .. u_(k-1), listed so that
+      // face q is (v, u_q, u_(q+1)) in its own corner order (closed mesh,
+      // every vertex's faces one cycle).  With d_q = u_q - p:
+      // volume (Cell3D_Kernel.cl:95-111): corner v of face q takes
+      //   kvol (u_q - C) x (u_(q+1) - C); summed over the closed ring this is
+      //   kvol sum_q d_q x d_(q+1) (the C terms cancel), taken in that form;
+      // edge springs (Cell3D_Kernel.cl:140-176): corner c of a face takes
+      //   sc (e_c - e_(c-1)) = sc (s(a - p) + s(b - p)) with
+      //   s(d) = d (|d| / l0 - 1) / |d| and a, b its other corners, and every
+      //   ring edge lies on two of the vertex's faces: 2 sc sum_q s(d_q);
+      // bending's first sum, sum_q d_q, kept in nx for its second pass.
+      const uint4 arow = rows[LW.adj / 16 + v];
+      const V3<R> d0 = vsub(lds_v(sx, sy, sz, row16(arow, 0)), P);
+      V3<R> dq = d0, Fv = zero, Fs = zero, sb = zero;
       for (int q = 0; q < DPM3D_MAXDEG; q++) {
-        const int e = row16(vrow, q);  // face*3 + corner, ascending
-        if (e == 0xffff) break;
-        const int f = e / 3, c = e - 3 * f;
-        if (vol_on) {  // Cell3D_Kernel.cl:95-111
-          int i0, i1, i2;
-          wface(sf, f, i0, i1, i2);
-          const int ia = c == 0 ? i1 : c == 1 ? i2 : i0, ib = c == 0 ? i2 : c == 1 ? i0 : i1;
-          F = vadd(F, vscale(vcross(vsub(lds_v(sx, sy, sz, ia), Ci), vsub(lds_v(sx, sy, sz, ib), Ci)), kvol));
+        if (surf_on) {
+          const R len = sqrt(vdot(dq, dq));
+          Fs = vadd(Fs, vscale(dq, (len * inv_l0 - R(1)) / len));
         }
-        if (sub_on) ndown += (down[f >> 5] >> (f & 31)) & 1u;
+        sb = vadd(sb, dq);
+        const int un = q + 1 < DPM3D_MAXDEG ? row16(arow, q + 1) : 0xffff;
+        const V3<R> dn = un == 0xffff ? d0 : vsub(lds_v(sx, sy, sz, un), P);
+        if (vol_on) Fv = vadd(Fv, vcross(dq, dn));
+        if (un == 0xffff) break;
+        dq = dn;
       }
-      if (surf_on) {
-        V3<R> Fs = zero;
-        const uint4 arow = rows[LW.adj / 16 + v];
+      F = vadd(vscale(Fv, kvol), vscale(Fs, R(2) * sc));
+      if (bend_on) {
+        nx[v] = sb.x;
+        ny[v] = sb.y;
+        nz[v] = sb.z;
+      }
+      if (sub_on) {  // Cell3D_Kernel.cl:206-246, per facing-down incident face
+        int ndown = 0;
+        const uint4 vrow = rows[LW.vf / 16 + v];
         for (int q = 0; q < DPM3D_MAXDEG; q++) {
-          const int u = row16(arow, q);
-          if (u == 0xffff) break;
-          const V3<R> d = vsub(lds_v(sx, sy, sz, u), P);
-          const R len = sqrt(vdot(d, d));
-          Fs = vadd(Fs, vscale(d, (len * inv_l0 - R(1)) / len));
+          const int e = row16(vrow, q);  // face*3 + corner
+          if (e == 0xffff) break;
+          const int f = e / 3;
+          ndown += (down[f >> 5] >> (f & 31)) & 1u;
         }
-        F = vadd(F, vscale(Fs, R(2) * sc));
+        if (ndown) F = vadd(F, vscale(substrate_corner(P, Ci, Ksi, ksg, L, l0i), (R)ndown));
       }
-      if (ndown) F = vadd(F, vscale(substrate_corner(P, Ci, Ksi, ksg, L, l0i), (R)ndown));
     } else if (shape_on) {
       const uint4 vrow = rows[LW.vf / 16 + v];
       for (int q = 0; q < DPM3D_MAXDEG; q++) {
@@ -283,8 +307,8 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   }
 
   // --------------------------------------------------------------- bending
-  if ((mask & M_BENDING) && Kbi != R(0)) {  // wave-uniform; scratch in nx
-    for (int v = lane; v < nv; v += DPM_WAVE) {
+  if (bend_on) {  // wave-uniform; scratch in nx
+    for (int v = lane; v < nv && !ring_pass; v += DPM_WAVE) {
       const V3<R> P = lds_v(sx, sy, sz, v);
       V3<R> s = zero;
       const uint4 arow = rows[LW.adj / 16 + v];
@@ -634,11 +658,12 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
     py = wsum_all(py);
     pz = wsum_all(pz);
     const V3<R> C = {(R)(px / nv), (R)(py / nv), (R)(pz / nv)};
-    R r = R(0);
+    R r2 = R(0);  // max |d|^2, then one square root (monotonic, correctly rounded)
     for (int v = lane; v < nv; v += DPM_WAVE) {
       const V3<R> d = vsub(lds_v(sx, sy, sz, v), C);
-      r = fmax(r, sqrt(vdot(d, d)));
+      r2 = fmax(r2, vdot(d, d));
     }
+    const R r = sqrt(r2);
     double vol = 0.0;
     R e2 = R(0);
     for (int f = lane; f < nf; f += DPM_WAVE) {

@@ -322,6 +322,42 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
     for (size_t k = 0; k < inc[v].size(); k++) vf[(size_t)v * DPM3D_MAXDEG + k] = inc[v][k];
     for (size_t k = 0; k < nb[v].size(); k++) adj[(size_t)v * DPM3D_MAXDEG + k] = nb[v][k];
   }
+  // Neighbour rows list each vertex's ring u_0 .. u_(k-1) so that face q is
+  // (v, u_q, u_(q+1)) in its corner order (the wave form walks it once per
+  // vertex; every form sums bending in this order).  That needs every
+  // vertex's faces to form one cycle; otherwise the mesh takes the
+  // per-corner path (closed = 0) and the rows stay ascending.
+  std::vector<int> ring = adj;
+  for (int v = 0; v < nv && closed; v++) {
+    const size_t k = inc[v].size();
+    if (k < 3 || nb[v].size() != k) {
+      closed = false;
+      break;
+    }
+    std::vector<std::pair<int, int>> ab;  // (next corner, the one after) per face
+    for (int e : inc[v]) {
+      const int f = e / 3, c = e % 3;
+      ab.push_back({d->faces[3 * f + (c + 1) % 3], d->faces[3 * f + (c + 2) % 3]});
+    }
+    int u = ab[0].first;
+    std::vector<int> r;
+    for (size_t q = 0; q < k && closed; q++) {
+      r.push_back(u);
+      int b = -1, hits = 0;
+      for (const auto& x : ab)
+        if (x.first == u) {
+          b = x.second;
+          hits++;
+        }
+      closed = hits == 1;
+      u = b;
+    }
+    std::vector<int> rs_sorted = r;
+    std::sort(rs_sorted.begin(), rs_sorted.end());
+    closed = closed && u == ab[0].first && rs_sorted == nb[v];
+    for (size_t q = 0; q < k && closed; q++) ring[(size_t)v * DPM3D_MAXDEG + q] = r[q];
+  }
+  if (closed) adj = ring;
 
   dpm_tissue* h = new dpm_tissue();
   h->dim = 3;

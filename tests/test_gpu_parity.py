@@ -285,6 +285,26 @@ def test_3d_open_mesh_takes_the_per_corner_path():
     np.testing.assert_array_equal(F[0], F[1])
 
 
+def test_3d_unoriented_mesh_takes_the_per_corner_path():
+    # every edge still on two faces, but one face listed in the opposite
+    # corner order: no vertex ring of that face's corners is one oriented
+    # cycle, so the wave form takes the per-corner path (the workgroup
+    # form's arithmetic, bit for bit)
+    m = W.monolayer3d(2, 2.4, seed=9)
+    faces = np.array(m.mesh.faces)
+    faces[5] = faces[5][::-1]
+    faces = np.ascontiguousarray(faces)
+    shape = capi.DPM3D_VOLUME | capi.DPM3D_SURFACE | capi.DPM3D_SUBSTRATE | capi.DPM3D_BENDING
+    F = []
+    for kind in (1, 2):
+        t = capi.Tissues3D(faces, [0, m.C], m.Kv, m.Ka, m.Ks, m.v0, m.a0, m.L, m.Kre, Kat=m.Kat, Kb=m.Kb,
+                           mask=shape, nv=m.mesh.nv)
+        t.set_step_kernel(kind)
+        t.set_positions(m.X)
+        F.append(t.probe(contact=False)["forces"])
+    np.testing.assert_array_equal(F[0], F[1])
+
+
 def test_3d_autotune_restores_state_and_keeps_a_configuration():
     ms = [W.monolayer3d(6, 1.9, seed=30 + k) for k in range(3)]
     cb = np.concatenate([[0], np.cumsum([m.C for m in ms])])

@@ -324,6 +324,49 @@ def test_surface_force_is_gradient_of_edge_springs():
     np.testing.assert_allclose(F, -fd_grad(E, X), rtol=1e-6, atol=1e-9)
 
 
+def vertex_rings(faces, nv):
+    # u_0 .. u_(k-1) per vertex with face q = (v, u_q, u_(q+1)) in its corner order
+    nxt = [dict() for _ in range(nv)]
+    for f in faces:
+        for c in range(3):
+            nxt[f[c]][f[(c + 1) % 3]] = f[(c + 2) % 3]
+    rings = []
+    for v in range(nv):
+        u0 = next(iter(nxt[v]))
+        r, u = [u0], nxt[v][u0]
+        while u != u0:
+            r.append(u)
+            u = nxt[v][u]
+        assert len(r) == len(nxt[v])
+        rings.append(r)
+    return rings
+
+
+def test_ring_walk_gives_the_volume_and_edge_spring_forces():
+    # The wave-per-cell step walks each vertex's ring once: volume
+    # kvol sum_q d_q x d_(q+1) (d_q = u_q - p; the centre-relative corner
+    # terms sum to this on a closed ring) and edge springs 2 sc sum_q s(d_q).
+    mesh, X = ico(2, 1.2, (0.4, -0.3, 1.3), seed=11, noise=0.03)
+    Kv, v0, Ka, a0 = 1.3, 3.0, 0.7, 0.05
+    rings = vertex_rings(mesh.faces, mesh.nv)
+    strain = volume_np(X, mesh.faces) / v0 - 1.0
+    kvol = -Kv * strain / 6.0
+    l0 = math.sqrt(4 * a0) / math.sqrt(3)
+    sc = Ka * math.sqrt(a0) / l0 * float(np.float32(0.3))  # the kernels' 0.3f
+    Fv, Fs = np.zeros_like(X), np.zeros_like(X)
+    for v, r in enumerate(rings):
+        d = X[r] - X[v]
+        Fv[v] = kvol * np.cross(d, np.roll(d, -1, axis=0)).sum(0)
+        n = np.linalg.norm(d, axis=1, keepdims=True)
+        Fs[v] = 2 * sc * (d * (n / l0 - 1.0) / n).sum(0)
+    np.testing.assert_allclose(Fv, F3(X, mesh, 1, Kv=Kv, v0=v0)["forces"], rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(Fs, F3(X, mesh, 2, Ka=Ka, a0=a0)["forces"], rtol=1e-10, atol=1e-13)
+    # the opposite ring direction flips the volume term: orientation matters
+    Fr = np.array([kvol * np.cross(X[r[::-1]] - X[v], np.roll(X[r[::-1]] - X[v], -1, axis=0)).sum(0)
+                   for v, r in enumerate(rings)])
+    assert np.abs(Fr + Fv).max() < 1e-12 and np.abs(Fv).max() > 1e-3
+
+
 def test_bending_is_gradient_and_invariant():
     mesh, X = ico(1, 1.1, (0.3, 0.2, 1.5), seed=4, noise=0.05)
     Kb = 0.3
