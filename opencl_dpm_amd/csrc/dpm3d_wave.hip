@@ -80,6 +80,33 @@ __device__ __forceinline__ double wmax_all(double v) {
   return v;
 }
 
+// One cell's positions (nv <= kWaveMaxNv) into a wave's LDS arrays, every
+// load issued before the first store (one memory round trip, not one per
+// 64 vertices).
+template <typename R>
+__device__ __forceinline__ void wave_stage_cell(R* dx, R* dy, R* dz, const R* x, const R* y, const R* z,
+                                                size_t base, int nv, int lane) {
+  R bx[kWaveMaxNv / DPM_WAVE], by[kWaveMaxNv / DPM_WAVE], bz[kWaveMaxNv / DPM_WAVE];
+#pragma unroll
+  for (int k = 0; k < kWaveMaxNv / DPM_WAVE; k++) {
+    const int v = lane + k * DPM_WAVE;
+    if (v < nv) {
+      bx[k] = x[base + v];
+      by[k] = y[base + v];
+      bz[k] = z[base + v];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kWaveMaxNv / DPM_WAVE; k++) {
+    const int v = lane + k * DPM_WAVE;
+    if (v < nv) {
+      dx[v] = bx[k];
+      dy[v] = by[k];
+      dz[v] = bz[k];
+    }
+  }
+}
+
 // winding_wave over per-wave arrays and packed faces (same arithmetic).
 template <typename R>
 __device__ __forceinline__ R winding_w(const R* nx, const R* ny, const R* nz, const unsigned* sf, Q4<R>* ut,
@@ -155,10 +182,8 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   const size_t base = (size_t)ci * nv;
   const V3<R> zero = {R(0), R(0), R(0)};
 
+  wave_stage_cell(sx, sy, sz, a.xin, a.yin, a.zin, base, nv, lane);
   for (int v = lane; v < nv; v += DPM_WAVE) {
-    sx[v] = a.xin[base + v];
-    sy[v] = a.yin[base + v];
-    sz[v] = a.zin[base + v];
     if (a.w_out) a.w_out[base + v] = R(0);
     if (a.rec_cell)
       for (int q = 0; q < DPM3D_MAXPART; q++) a.rec_cell[(base + v) * DPM3D_MAXPART + q] = -1;
@@ -458,11 +483,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       for (int qc = 0; qc < ncl; qc++) {
         const int cj2 = cl[qc];
         const CellStat<R> stj = a.st_in[cj2];
-        for (int v = lane; v < nv; v += DPM_WAVE) {
-          nx[v] = a.xin[(size_t)cj2 * nv + v];
-          ny[v] = a.yin[(size_t)cj2 * nv + v];
-          nz[v] = a.zin[(size_t)cj2 * nv + v];
-        }
+        wave_stage_cell(nx, ny, nz, a.xin, a.yin, a.zin, (size_t)cj2 * nv, nv, lane);
         V3<R> sh = zero;
         if (pbc)
           sh = v3(image_shift(Ci.x - stj.cx, L, invL), image_shift(Ci.y - stj.cy, L, invL),
@@ -570,14 +591,20 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
                 // delta = pj - p and p - pj, minimum image per component
                 // (Cell3D_Kernel.cl:341-346); decisions in the reference's
                 // arithmetic, accepted forces in double on the same image
+                // Without an image, p - pj is -(pj - p) exactly and the two
+                // squared distances are the same number.
                 V3<R> dl = vsub(pj, Pv);
-                V3<R> dr = vsub(Pv, pj);
                 V3<R> nl = zero, nr = zero;
+                R d2r;
                 if (pbc && !(fabs(dl.x) < half_l && fabs(dl.y) < half_l && fabs(dl.z) < half_l)) {
+                  V3<R> dr = vsub(Pv, pj);
                   nl = v3(round(dl.x / L), round(dl.y / L), round(dl.z / L));
                   nr = v3(round(dr.x / L), round(dr.y / L), round(dr.z / L));
                   dl = vsub(dl, v3(L * nl.x, L * nl.y, L * nl.z));
                   dr = vsub(dr, v3(L * nr.x, L * nr.y, L * nr.z));
+                  d2r = vdot(dr, dr);
+                } else {
+                  d2r = vdot(dl, dl);
                 }
                 const R d2 = vdot(dl, dl);
                 if (d2 < lim_own) {
@@ -587,7 +614,6 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
                     cnt_own++;
                   }
                 }
-                const R d2r = vdot(dr, dr);
                 if (d2r < lim_rev) {
                   const R dsr = sqrt(d2r);
                   if (dsr < L0j * R(2) && dsr > R(1e-12f)) Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j));
