@@ -23,7 +23,7 @@ constexpr int kWaveCand = 128;   // candidate cells per batch
 // cell-list gather, which is finished before the first candidate.
 struct Lds3DWave {
   size_t vf, adj, img, fend, slice, total;
-  size_t sx, nx, fx, jl, il, down, cl, ut, gl, gath;
+  size_t sx, nx, fx, il, down, cl, ut, gl, gath;
 };
 __host__ __device__ inline Lds3DWave ldsw_layout(int nv, int nf, int cpw, int rsize) {
   Lds3DWave L;
@@ -37,9 +37,8 @@ __host__ __device__ inline Lds3DWave ldsw_layout(int nv, int nf, int cpw, int rs
   L.img = al(L.adj + 2u * n * DPM3D_MAXDEG);
   size_t s = 0;
   L.sx = s;   s = al(s + 3u * rs * n);  // own positions (x | y | z)
-  L.nx = s;   s = al(s + 3u * rs * n);  // candidate positions; bending scratch
+  L.nx = s;   s = al(s + 3u * rs * n);  // candidate positions (those in reach compacted in place); bending scratch
   L.fx = s;   s = al(s + 3u * rs * n);  // force accumulators
-  L.jl = s;   s += 2u * n;              // candidate vertices in reach
   L.il = s;   s = al(s + 2u * n);       // own vertices in reach
   L.down = s; s = al(s + 4u * (size_t)((nf + 31) / 32));  // facing-down faces
   L.cl = s;   s = al(s + 4u * (size_t)kWaveCand);  // candidate cells of a batch
@@ -160,7 +159,6 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   R* fx = reinterpret_cast<R*>(slb + LW.fx);
   R* fy = fx + nv;
   R* fz = fy + nv;
-  unsigned short* jl = reinterpret_cast<unsigned short*>(slb + LW.jl);
   unsigned short* il = reinterpret_cast<unsigned short*>(slb + LW.il);
   Q4<R>* ut = reinterpret_cast<Q4<R>*>(slb + LW.ut);
   int* gl = reinterpret_cast<int*>(slb + LW.gl);
@@ -554,18 +552,28 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
           // within reach of cell j (|q - Ci| < 2 l0 + Ri, and reversed)
           const R rk = (Ri + R(2) * l0m) * R(1.0001f) + R(1e-6f);
           const R r2 = (stj.r + R(2) * l0m) * R(1.0001f) + R(1e-6f);
+          // Cell j's vertices in reach are compacted in place to the front of
+          // nx/ny/nz (ascending; every slot written lies below the indices
+          // still to be read, and a chunk's loads precede its stores).
           int nnear = 0, nmine = 0;
           for (int k0v = 0; k0v < nv; k0v += DPM_WAVE) {
             const int v = k0v + lane;
             bool near = false, mine = false;
+            V3<R> q = zero;
             if (v < nv) {
-              const V3<R> ds = {nx[v] + sh.x - Ci.x, ny[v] + sh.y - Ci.y, nz[v] + sh.z - Ci.z};
+              q = lds_v(nx, ny, nz, v);
+              const V3<R> ds = {q.x + sh.x - Ci.x, q.y + sh.y - Ci.y, q.z + sh.z - Ci.z};
               near = all_images || vdot(ds, ds) <= rk * rk;
               const V3<R> dp = {sx[v] - stj.cx - sh.x, sy[v] - stj.cy - sh.y, sz[v] - stj.cz - sh.z};
               mine = all_images || vdot(dp, dp) <= r2 * r2;
             }
             const unsigned long long mn = __ballot(near), mm = __ballot(mine);
-            if (near) jl[nnear + __popcll(mn & below)] = (unsigned short)v;
+            if (near) {
+              const int jn = nnear + __popcll(mn & below);
+              nx[jn] = q.x;
+              ny[jn] = q.y;
+              nz[jn] = q.z;
+            }
             const int slot = nmine + __popcll(mm & below);
             if (mine) il[slot] = (unsigned short)v;
             nnear += __popcll(mn);
@@ -578,6 +586,11 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
             const int S = nmine >= DPM_WAVE ? 1 : max(1, min(nnear, DPM_WAVE / nmine));
             const int per = (nnear + S - 1) / S;
             const R half_l = R(0.49f) * L;
+            // every |p - pj| component stays below 0.49 L, and no pair needs an
+            // image, when the unshifted centres and radii keep it below 0.48 L
+            const bool no_image = !pbc || (fabs(Ci.x - stj.cx) + (Ri + stj.r) < R(0.48f) * L &&
+                                           fabs(Ci.y - stj.cy) + (Ri + stj.r) < R(0.48f) * L &&
+                                           fabs(Ci.z - stj.cz) + (Ri + stj.r) < R(0.48f) * L);
             const R lim_own = (L0i * R(2)) * (L0i * R(2)) * R(1.00001f);
             const R lim_rev = (L0j * R(2)) * (L0j * R(2)) * R(1.00001f);
             for (int tw = lane; tw < S * nmine; tw += DPM_WAVE) {
@@ -587,7 +600,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
               int cnt_own = 0;
               const int m1 = min(nnear, (sl + 1) * per);
               for (int m = sl * per; m < m1; m++) {
-                const V3<R> pj = lds_v(nx, ny, nz, (int)jl[m]);
+                const V3<R> pj = lds_v(nx, ny, nz, m);
                 // delta = pj - p and p - pj, minimum image per component
                 // (Cell3D_Kernel.cl:341-346); decisions in the reference's
                 // arithmetic, accepted forces in double on the same image
@@ -596,7 +609,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
                 V3<R> dl = vsub(pj, Pv);
                 V3<R> nl = zero, nr = zero;
                 R d2r;
-                if (pbc && !(fabs(dl.x) < half_l && fabs(dl.y) < half_l && fabs(dl.z) < half_l)) {
+                if (!no_image && !(fabs(dl.x) < half_l && fabs(dl.y) < half_l && fabs(dl.z) < half_l)) {
                   V3<R> dr = vsub(Pv, pj);
                   nl = v3(round(dl.x / L), round(dl.y / L), round(dl.z / L));
                   nr = v3(round(dr.x / L), round(dr.y / L), round(dr.z / L));
