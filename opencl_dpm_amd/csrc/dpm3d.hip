@@ -200,6 +200,18 @@ __device__ __forceinline__ bool outside_by_margin(const R* xs, const R* ys, cons
   return dn > R(0) && dn * dn - h >= ((m + R(1e-4f) * (dn + Rj)) + R(1e-6f) * M) * dn;
 }
 
+// Facing-down decision of a face with normal n = (P1 - P0) x (P2 - P0):
+// normalize(n).z < -0.1 (Cell3D_Kernel.cl:206-214), decided from n_z and |n|^2
+// where n_z^2 and 0.01 |n|^2 are clear of each other by 1e-5 (relative), else
+// by the normalised vector itself (the reference's operation order).
+template <typename R>
+__device__ __forceinline__ bool facing_down(V3<R> n) {
+  const R q = n.z * n.z, t = (R(0.1f) * R(0.1f)) * vdot(n, n);
+  if (n.z >= R(0) || q < t * R(0.99999f)) return false;
+  if (q > t * R(1.00001f)) return true;
+  return vnormalize(n).z < R(-0.1f);
+}
+
 // LDS layout (byte offsets), shared by host (sizing) and device (carving).
 // The per-(face, corner) shape forces are dead once every vertex gathered
 // them, so they share their bytes with the tables of the later passes
@@ -593,6 +605,8 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
 
   const V3<R> P = own ? lds_v(l.sx, l.sy, l.sz, tid) : zero;
   V3<R> F = zero;
+  bool ring_done = false;  // shape forces by the ring walk (bending's first sum in ring_sb)
+  V3<R> ring_sb = zero;
 
   // ---------------------------------------------------------- shape forces
   // Per-face pass: each face's volume, edge-spring and substrate terms for
@@ -608,7 +622,73 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
     const bool sub_on = (mask & M_SUBSTRATE) && !(Ksi < R(1e-12f));
     const R sc = Kai * sqrt(a0i) / l0i * R(0.3f);
     const R inv_l0 = R(1) / l0i;
-    for (int f = tid; f < nf; f += NT) {
+    if (sizeof(R) == 4 && a.closed) {  // block-uniform
+      // Closed mesh, neighbour rows in ring order (face q is (v, u_q, u_(q+1))):
+      // one walk around each vertex's ring, as the wave-per-cell step takes
+      // it — volume kvol sum_q d_q x d_(q+1), edge springs 2 sc sum_q s(d_q),
+      // bending's first sum, substrate per facing-down incident face.  A
+      // ring edge shorter than 2e-12 (the reference skips such faces'
+      // springs) sends the cell back to the face pass below.  Float only:
+      // in double the walk's registers push the rest of the step into
+      // spills (measured 5 % slower on 4096 x 642 cells), and the face pass
+      // stays.
+      unsigned* dbits = reinterpret_cast<unsigned*>(l.fcx);  // dead until the candidates
+      if (sub_on) {
+        for (int f0 = wave * DPM_WAVE; f0 < nf; f0 += NT) {  // 64 consecutive faces per wave
+          const int f = f0 + lane;
+          bool dn = false;
+          if (f < nf) {
+            int i0, i1, i2;
+            face_of(l, f, i0, i1, i2);
+            const V3<R> P0 = lds_v(l.sx, l.sy, l.sz, i0);
+            dn = facing_down(vcross(vsub(lds_v(l.sx, l.sy, l.sz, i1), P0), vsub(lds_v(l.sx, l.sy, l.sz, i2), P0)));
+            if (dn && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
+          }
+          const unsigned long long m = __ballot(dn);
+          if (lane == 0) {
+            dbits[f0 / 32] = (unsigned)m;
+            if (f0 / 32 + 1 < (nf + 31) / 32) dbits[f0 / 32 + 1] = (unsigned)(m >> 32);
+          }
+        }
+        __syncthreads();
+      }
+      bool bad = false;
+      V3<R> Fr = zero;
+      if (own) {
+        const V3<R> d0 = vsub(lds_v(l.sx, l.sy, l.sz, adjr[0]), P);
+        V3<R> dq = d0, Fv = zero, Fs = zero, sb = zero;
+#pragma unroll
+        for (int q = 0; q < DPM3D_MAXDEG; q++) {
+          if (surf_on) {
+            const R len = sqrt(vdot(dq, dq));
+            bad = bad || len < R(2e-12f);
+            Fs = vadd(Fs, vscale(dq, (len * inv_l0 - R(1)) / len));
+          }
+          sb = vadd(sb, dq);
+          const int un = q + 1 < DPM3D_MAXDEG ? adjr[q + 1] : -1;
+          const V3<R> dn = un < 0 ? d0 : vsub(lds_v(l.sx, l.sy, l.sz, un), P);
+          if (vol_on) Fv = vadd(Fv, vcross(dq, dn));
+          if (un < 0) break;
+          dq = dn;
+        }
+        Fr = vadd(vscale(Fv, kvol), vscale(Fs, R(2) * sc));
+        ring_sb = sb;
+        if (sub_on) {
+          int ndown = 0;
+#pragma unroll
+          for (int k = 0; k < DPM3D_MAXDEG; k++) {
+            const int e = vfr[k];  // face*3 + corner
+            if (e < 0) break;
+            const int f = e / 3;
+            ndown += (dbits[f >> 5] >> (f & 31)) & 1u;
+          }
+          if (ndown) Fr = vadd(Fr, vscale(substrate_corner(P, Ci, Ksi, ksg, L, l0i), (R)ndown));
+        }
+      }
+      ring_done = !__syncthreads_or(bad);  // also: every facing-down bit read
+      if (ring_done) F = Fr;
+    }
+    for (int f = tid; f < nf && !ring_done; f += NT) {
       int i0, i1, i2;
       face_of(l, f, i0, i1, i2);
       const V3<R> P0 = lds_v(l.sx, l.sy, l.sz, i0);
@@ -652,27 +732,33 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
       l.fcy[3 * f + 2] = c2.y;
       l.fcz[3 * f + 2] = c2.z;
     }
-    __syncthreads();
-    if (own) {
+    if (!ring_done) {  // block-uniform
+      __syncthreads();
+      if (own) {
 #pragma unroll
-      for (int k = 0; k < DPM3D_MAXDEG; k++) {
-        const int e = vfr[k];  // face*3 + corner
-        if (e < 0) break;
-        F = vadd(F, V3<R>{l.fcx[e], l.fcy[e], l.fcz[e]});
+        for (int k = 0; k < DPM3D_MAXDEG; k++) {
+          const int e = vfr[k];  // face*3 + corner
+          if (e < 0) break;
+          F = vadd(F, V3<R>{l.fcx[e], l.fcy[e], l.fcz[e]});
+        }
       }
+      __syncthreads();  // the shape-force bytes are reused below
     }
-    __syncthreads();  // the shape-force bytes are reused below
   }
 
   // --------------------------------------------------------------- bending
   if ((mask & M_BENDING) && Kbi != R(0)) {  // block-uniform
     if (own) {
       V3<R> s = zero;
+      if (ring_done) {
+        s = ring_sb;
+      } else {
 #pragma unroll
-      for (int k = 0; k < DPM3D_MAXDEG; k++) {
-        const int u = adjr[k];
-        if (u < 0) break;
-        s = vadd(s, vsub(lds_v(l.sx, l.sy, l.sz, u), P));
+        for (int k = 0; k < DPM3D_MAXDEG; k++) {
+          const int u = adjr[k];
+          if (u < 0) break;
+          s = vadd(s, vsub(lds_v(l.sx, l.sy, l.sz, u), P));
+        }
       }
       l.sb[3 * tid] = s.x;
       l.sb[3 * tid + 1] = s.y;

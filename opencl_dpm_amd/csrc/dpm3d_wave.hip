@@ -211,19 +211,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         const V3<R> P0 = lds_v(sx, sy, sz, i0), P1 = lds_v(sx, sy, sz, i1), P2 = lds_v(sx, sy, sz, i2);
         const V3<R> l0v = vsub(P1, P0), l2v = vsub(P0, P2);
         if (sub_on) {
-          // un.z < -0.1 for un = n / |n|: decided from n.z and |n|^2 where
-          // they are clear of the threshold by 1e-5 (relative), else by the
-          // normalised vector itself
-          const V3<R> n = vcross(l0v, vsub(P2, P0));
-          const R q = n.z * n.z, t = (R(0.1f) * R(0.1f)) * vdot(n, n);
-          if (n.z >= R(0) || q < t * R(0.99999f)) {
-            dn = false;
-          } else if (q > t * R(1.00001f)) {
-            dn = true;
-          } else {
-            const V3<R> un = vnormalize(n);
-            dn = un.z < R(-0.1f);
-          }
+          dn = facing_down(vcross(l0v, vsub(P2, P0)));
           if (dn && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
         }
         const V3<R> l1v = vsub(P2, P1);
