@@ -74,6 +74,11 @@ __device__ __forceinline__ double wmin_all(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, DPM_WAVE));
   return v;
 }
+template <typename R>
+__device__ __forceinline__ R wmax_r(R v) {  // in the value's own precision
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, DPM_WAVE));
+  return v;
+}
 __device__ __forceinline__ double wmax_all(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, DPM_WAVE));
   return v;
@@ -475,10 +480,150 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
           sh = v3(image_shift(Ci.x - stj.cx, L, invL), image_shift(Ci.y - stj.cy, L, invL),
                   image_shift(Ci.z - stj.cz, L, invL));
         wave_lds_sync();
-        if (do_contact) {
-          const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
-          const R mj = contact_margin(stj.r, stj.em);
-          const R rj = (stj.r + mj) * R(1.0001f) + R(1e-6f);
+        const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
+        const R mj = contact_margin(stj.r, stj.em);
+        const R rj = (stj.r + mj) * R(1.0001f) + R(1e-6f);
+        const R L0j = fmax(stj.l0, R(1e-12f));
+        const R l0m = fmax(L0i, L0j);
+        const bool all_images = pbc && R(2) * (Ri + stj.r + R(2) * l0m) >= R(0.999f) * L;
+        // every |p - pj| component stays below 0.49 L, and no pair needs an
+        // image, when the unshifted centres and radii keep it below 0.48 L
+        // (then the shift is 0 too)
+        const bool no_image = !pbc || (fabs(Ci.x - stj.cx) + (Ri + stj.r) < R(0.48f) * L &&
+                                       fabs(Ci.y - stj.cy) + (Ri + stj.r) < R(0.48f) * L &&
+                                       fabs(Ci.z - stj.cz) + (Ri + stj.r) < R(0.48f) * L);
+        // vertices of cell j within reach of this cell, and of this cell
+        // within reach of cell j (|q - Ci| < 2 l0 + Ri, and reversed)
+        const R rk = (Ri + R(2) * l0m) * R(1.0001f) + R(1e-6f);
+        const R r2 = (stj.r + R(2) * l0m) * R(1.0001f) + R(1e-6f);
+        // Separation along the centre line n = (C_j - C_i) / D: every vertex p
+        // of this cell has (p - C_i).n <= hi and every point x of cell j's
+        // (shifted) hull has (C_j - x).n <= hj, so |x - p| >= D - hi - hj.
+        const V3<R> dC = vsub(Cj, Ci);
+        const R D = sqrt(vdot(dC, dC));
+        const V3<R> nC = D > R(0) ? vscale(dC, R(1) / D) : zero;
+        R hi = R(-3.0e38f), hj = R(-3.0e38f);
+        // One pass: the separation, whether an own vertex is within contact
+        // reach, and the adhesion lists.  Cell j's vertices in reach are
+        // compacted in place to the front of nx/ny/nz (ascending; every slot
+        // written lies below the indices still to be read, and a chunk's loads
+        // precede its stores).
+        bool any_need = false;
+        int nnear = 0, nmine = 0;
+        for (int k0v = 0; k0v < nv; k0v += DPM_WAVE) {
+          const int v = k0v + lane;
+          bool need = false, near = false, mine = false;
+          V3<R> q = zero;
+          if (v < nv) {
+            const V3<R> P = lds_v(sx, sy, sz, v);
+            q = lds_v(nx, ny, nz, v);
+            hi = fmax(hi, vdot(vsub(P, Ci), nC));
+            hj = fmax(hj, vdot(v3(stj.cx - q.x, stj.cy - q.y, stj.cz - q.z), nC));
+            const V3<R> dp = vsub(P, Cj);
+            const R d2 = vdot(dp, dp);
+            need = do_contact && d2 <= rj * rj;
+            if (do_adh) {
+              mine = all_images || d2 <= r2 * r2;
+              const V3<R> ds = {q.x + sh.x - Ci.x, q.y + sh.y - Ci.y, q.z + sh.z - Ci.z};
+              near = all_images || vdot(ds, ds) <= rk * rk;
+            }
+          }
+          any_need = any_need || __ballot(need) != 0ull;
+          const unsigned long long mn = __ballot(near), mm = __ballot(mine);
+          if (near) {
+            const int jn = nnear + __popcll(mn & below);
+            nx[jn] = q.x;
+            ny[jn] = q.y;
+            nz[jn] = q.z;
+          }
+          const int slot = nmine + __popcll(mm & below);
+          if (mine) il[slot] = (unsigned short)v;
+          nnear += __popcll(mn);
+          nmine += __popcll(mm);
+        }
+        hi = wmax_r(hi);
+        hj = wmax_r(hj);
+        // the separation less slack for the float sums (relative 1e-4 of the
+        // cells' extent, 1e-6 of the coordinates); -1 when it is not defined
+        const R Mc = ((fabs(Ci.x) + fabs(Ci.y)) + fabs(Ci.z)) + ((fabs(Cj.x) + fabs(Cj.y)) + fabs(Cj.z));
+        const R gap = D > R(0) ? ((D - hi) - hj) - (R(1e-4f) * ((Ri + stj.r) + D) + R(1e-6f) * Mc) : R(-1);
+        // Adhesion: no pair of vertices closer than 2 l0 when gap >= 2 l0 (on
+        // pairs that take no image; the others are always evaluated).
+        if (do_adh && nnear > 0 && nmine > 0 && (all_images || !no_image || !(gap >= R(2) * l0m))) {  // wave-uniform
+          wave_lds_sync();
+          // (own vertex, slice of near vertices) per lane; the slices are
+          // summed in order afterwards
+          const int S = nmine >= DPM_WAVE ? 1 : max(1, min(nnear, DPM_WAVE / nmine));
+          const int per = (nnear + S - 1) / S;
+          const R half_l = R(0.49f) * L;
+          const R lim_own = (L0i * R(2)) * (L0i * R(2)) * R(1.00001f);
+          const R lim_rev = (L0j * R(2)) * (L0j * R(2)) * R(1.00001f);
+          for (int tw = lane; tw < S * nmine; tw += DPM_WAVE) {
+            const int iv = tw % nmine, sl = tw / nmine;
+            const V3<R> Pv = lds_v(sx, sy, sz, (int)il[iv]);
+            V3<R> Fa = zero;
+            int cnt_own = 0;
+            const int m1 = min(nnear, (sl + 1) * per);
+            for (int m = sl * per; m < m1; m++) {
+              const V3<R> pj = lds_v(nx, ny, nz, m);
+              // delta = pj - p and p - pj, minimum image per component
+              // (Cell3D_Kernel.cl:341-346); decisions in the reference's
+              // arithmetic, accepted forces in double on the same image
+              // Without an image, p - pj is -(pj - p) exactly and the two
+              // squared distances are the same number.
+              V3<R> dl = vsub(pj, Pv);
+              V3<R> nl = zero, nr = zero;
+              R d2r;
+              if (!no_image && !(fabs(dl.x) < half_l && fabs(dl.y) < half_l && fabs(dl.z) < half_l)) {
+                V3<R> dr = vsub(Pv, pj);
+                nl = v3(round(dl.x / L), round(dl.y / L), round(dl.z / L));
+                nr = v3(round(dr.x / L), round(dr.y / L), round(dr.z / L));
+                dl = vsub(dl, v3(L * nl.x, L * nl.y, L * nl.z));
+                dr = vsub(dr, v3(L * nr.x, L * nr.y, L * nr.z));
+                d2r = vdot(dr, dr);
+              } else {
+                d2r = vdot(dl, dl);
+              }
+              const R d2 = vdot(dl, dl);
+              if (d2 < lim_own) {
+                const R dsq = sqrt(d2);
+                if (dsq < L0i * R(2) && dsq > R(1e-12f)) {
+                  Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i));
+                  cnt_own++;
+                }
+              }
+              if (d2r < lim_rev) {
+                const R dsr = sqrt(d2r);
+                if (dsr < L0j * R(2) && dsr > R(1e-12f)) Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j));
+              }
+            }
+            ut[sl * nmine + iv] = Q4<R>{Fa.x, Fa.y, Fa.z, (R)cnt_own};
+          }
+          wave_lds_sync();
+          for (int slot = lane; slot < nmine; slot += DPM_WAVE) {
+            const int v = il[slot];
+            V3<R> F = lds_v(fx, fy, fz, v);
+            int cnt_own = 0;
+            for (int s2 = 0; s2 < S; s2++) {
+              const Q4<R> q4 = ut[s2 * nmine + slot];
+              F = vadd(F, V3<R>{q4.x, q4.y, q4.z});
+              cnt_own += (int)q4.w;
+            }
+            fx[v] = F.x;
+            fy[v] = F.y;
+            fz[v] = F.z;
+            if (a.p_adh) a.p_adh[base + v] += cnt_own;
+          }
+        }
+        // Contact: a point at least m_j from every face of cell j has winding
+        // number 0 (contact_margin), so the pass is needed only below that
+        // separation; it restages cell j (the adhesion lists rewrote nx).
+        if (do_contact && any_need && !(gap >= mj)) {  // wave-uniform
+          if (do_adh) {
+            wave_lds_sync();
+            wave_stage_cell(nx, ny, nz, a.xin, a.yin, a.zin, (size_t)cj2 * nv, nv, lane);
+          }
+          wave_lds_sync();
           for (int k0v = 0; k0v < nv; k0v += DPM_WAVE) {
             const int v = k0v + lane;
             bool need = false;
@@ -529,113 +674,6 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
                 }
                 if (a.p_contact) a.p_contact[pidx] = 1;
               }
-            }
-          }
-        }
-        if (do_adh) {
-          const R L0j = fmax(stj.l0, R(1e-12f));
-          const R l0m = fmax(L0i, L0j);
-          const bool all_images = pbc && R(2) * (Ri + stj.r + R(2) * l0m) >= R(0.999f) * L;
-          // vertices of cell j within reach of this cell, and of this cell
-          // within reach of cell j (|q - Ci| < 2 l0 + Ri, and reversed)
-          const R rk = (Ri + R(2) * l0m) * R(1.0001f) + R(1e-6f);
-          const R r2 = (stj.r + R(2) * l0m) * R(1.0001f) + R(1e-6f);
-          // Cell j's vertices in reach are compacted in place to the front of
-          // nx/ny/nz (ascending; every slot written lies below the indices
-          // still to be read, and a chunk's loads precede its stores).
-          int nnear = 0, nmine = 0;
-          for (int k0v = 0; k0v < nv; k0v += DPM_WAVE) {
-            const int v = k0v + lane;
-            bool near = false, mine = false;
-            V3<R> q = zero;
-            if (v < nv) {
-              q = lds_v(nx, ny, nz, v);
-              const V3<R> ds = {q.x + sh.x - Ci.x, q.y + sh.y - Ci.y, q.z + sh.z - Ci.z};
-              near = all_images || vdot(ds, ds) <= rk * rk;
-              const V3<R> dp = {sx[v] - stj.cx - sh.x, sy[v] - stj.cy - sh.y, sz[v] - stj.cz - sh.z};
-              mine = all_images || vdot(dp, dp) <= r2 * r2;
-            }
-            const unsigned long long mn = __ballot(near), mm = __ballot(mine);
-            if (near) {
-              const int jn = nnear + __popcll(mn & below);
-              nx[jn] = q.x;
-              ny[jn] = q.y;
-              nz[jn] = q.z;
-            }
-            const int slot = nmine + __popcll(mm & below);
-            if (mine) il[slot] = (unsigned short)v;
-            nnear += __popcll(mn);
-            nmine += __popcll(mm);
-          }
-          if (nnear > 0 && nmine > 0) {  // wave-uniform
-            wave_lds_sync();
-            // (own vertex, slice of near vertices) per lane; the slices are
-            // summed in order afterwards
-            const int S = nmine >= DPM_WAVE ? 1 : max(1, min(nnear, DPM_WAVE / nmine));
-            const int per = (nnear + S - 1) / S;
-            const R half_l = R(0.49f) * L;
-            // every |p - pj| component stays below 0.49 L, and no pair needs an
-            // image, when the unshifted centres and radii keep it below 0.48 L
-            const bool no_image = !pbc || (fabs(Ci.x - stj.cx) + (Ri + stj.r) < R(0.48f) * L &&
-                                           fabs(Ci.y - stj.cy) + (Ri + stj.r) < R(0.48f) * L &&
-                                           fabs(Ci.z - stj.cz) + (Ri + stj.r) < R(0.48f) * L);
-            const R lim_own = (L0i * R(2)) * (L0i * R(2)) * R(1.00001f);
-            const R lim_rev = (L0j * R(2)) * (L0j * R(2)) * R(1.00001f);
-            for (int tw = lane; tw < S * nmine; tw += DPM_WAVE) {
-              const int iv = tw % nmine, sl = tw / nmine;
-              const V3<R> Pv = lds_v(sx, sy, sz, (int)il[iv]);
-              V3<R> Fa = zero;
-              int cnt_own = 0;
-              const int m1 = min(nnear, (sl + 1) * per);
-              for (int m = sl * per; m < m1; m++) {
-                const V3<R> pj = lds_v(nx, ny, nz, m);
-                // delta = pj - p and p - pj, minimum image per component
-                // (Cell3D_Kernel.cl:341-346); decisions in the reference's
-                // arithmetic, accepted forces in double on the same image
-                // Without an image, p - pj is -(pj - p) exactly and the two
-                // squared distances are the same number.
-                V3<R> dl = vsub(pj, Pv);
-                V3<R> nl = zero, nr = zero;
-                R d2r;
-                if (!no_image && !(fabs(dl.x) < half_l && fabs(dl.y) < half_l && fabs(dl.z) < half_l)) {
-                  V3<R> dr = vsub(Pv, pj);
-                  nl = v3(round(dl.x / L), round(dl.y / L), round(dl.z / L));
-                  nr = v3(round(dr.x / L), round(dr.y / L), round(dr.z / L));
-                  dl = vsub(dl, v3(L * nl.x, L * nl.y, L * nl.z));
-                  dr = vsub(dr, v3(L * nr.x, L * nr.y, L * nr.z));
-                  d2r = vdot(dr, dr);
-                } else {
-                  d2r = vdot(dl, dl);
-                }
-                const R d2 = vdot(dl, dl);
-                if (d2 < lim_own) {
-                  const R dsq = sqrt(d2);
-                  if (dsq < L0i * R(2) && dsq > R(1e-12f)) {
-                    Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i));
-                    cnt_own++;
-                  }
-                }
-                if (d2r < lim_rev) {
-                  const R dsr = sqrt(d2r);
-                  if (dsr < L0j * R(2) && dsr > R(1e-12f)) Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j));
-                }
-              }
-              ut[sl * nmine + iv] = Q4<R>{Fa.x, Fa.y, Fa.z, (R)cnt_own};
-            }
-            wave_lds_sync();
-            for (int slot = lane; slot < nmine; slot += DPM_WAVE) {
-              const int v = il[slot];
-              V3<R> F = lds_v(fx, fy, fz, v);
-              int cnt_own = 0;
-              for (int s2 = 0; s2 < S; s2++) {
-                const Q4<R> q4 = ut[s2 * nmine + slot];
-                F = vadd(F, V3<R>{q4.x, q4.y, q4.z});
-                cnt_own += (int)q4.w;
-              }
-              fx[v] = F.x;
-              fy[v] = F.y;
-              fz[v] = F.z;
-              if (a.p_adh) a.p_adh[base + v] += cnt_own;
             }
           }
         }
