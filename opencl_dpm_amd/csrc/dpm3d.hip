@@ -903,7 +903,42 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
           const size_t b1 = (size_t)l.clist[q + 1] * nv + tid;
           pre = v3(a.xin[b1], a.yin[b1], a.zin[b1]);
         }
-        if (do_contact) {
+        // Separation along the centre line n = (C_j - C_i) / D (float
+        // steps): every vertex p of this cell has (p - C_i).n <= hi and every
+        // point x of cell j's (shifted) hull has (C_j - x).n <= hj, so
+        // |x - p| >= D - hi - hj, less slack for the float sums (1e-4 of the
+        // cells' extent, 1e-6 of the coordinates).  Contact needs a point
+        // closer than the culling margin m_j to a face of j (contact_margin),
+        // adhesion a vertex pair closer than 2 l0 — on pairs that take no
+        // periodic image (the unshifted centres and radii keep every
+        // component below 0.48 L; the shift is 0 then).  In double the extra
+        // registers cost more than the skipped work (spills, measured on
+        // 4096 x 642 cells), and every candidate is evaluated.
+        bool skip_contact = false, skip_adh = false;
+        if constexpr (sizeof(R) == 4) {
+          const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
+          const R l0m = fmax(L0i, fmax(stj.l0, R(1e-12f)));
+          const bool all_images = pbc && R(2) * (Ri + stj.r + R(2) * l0m) >= R(0.999f) * L;
+          const bool no_image = !pbc || (fabs(Ci.x - stj.cx) + (Ri + stj.r) < R(0.48f) * L &&
+                                         fabs(Ci.y - stj.cy) + (Ri + stj.r) < R(0.48f) * L &&
+                                         fabs(Ci.z - stj.cz) + (Ri + stj.r) < R(0.48f) * L);
+          const V3<R> dC = vsub(Cj, Ci);
+          const R D = sqrt(vdot(dC, dC));
+          const V3<R> nC = D > R(0) ? vscale(dC, R(1) / D) : zero;
+          R hi = R(-3.0e38f), hj = R(-3.0e38f);
+          if (own) {
+            hi = vdot(vsub(P, Ci), nC);
+            hj = vdot(v3(stj.cx - l.nx[tid], stj.cy - l.ny[tid], stj.cz - l.nz[tid]), nC);
+          }
+          block_max2(hi, hj, reinterpret_cast<R*>(l.red));
+          const R Mc = ((fabs(Ci.x) + fabs(Ci.y)) + fabs(Ci.z)) + ((fabs(Cj.x) + fabs(Cj.y)) + fabs(Cj.z));
+          if (D > R(0)) {
+            const R gap = ((D - hi) - hj) - (R(1e-4f) * ((Ri + stj.r) + D) + R(1e-6f) * Mc);
+            skip_contact = gap >= contact_margin(stj.r, stj.em);
+            skip_adh = !all_images && no_image && gap >= R(2) * l0m;
+          }
+        }
+        if (do_contact && !skip_contact) {  // block-uniform
           const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
           const R mj = contact_margin(stj.r, stj.em);
           const R rj = (stj.r + mj) * R(1.0001f) + R(1e-6f);
@@ -958,7 +993,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
         const R L0j = fmax(stj.l0, R(1e-12f));
         const R l0m = fmax(L0i, L0j);
         const bool all_images = pbc && R(2) * (Ri + stj.r + R(2) * l0m) >= R(0.999f) * L;
-        if (do_adh) {  // block-uniform
+        if (do_adh && !skip_adh) {  // block-uniform
           // Vertices of cell j within reach of this cell (a pair closer than
           // 2 l0 has |q - Ci| <= |q - p| + |p - Ci| < 2 l0 + Ri) and vertices
           // of this cell within reach of cell j, the same bound reversed.

@@ -176,7 +176,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   const R L = (R)a.tL[t];
   const int pbc = a.tpbc[t];
   const R Kre = (R)a.tKre[t], Kat = (R)a.tKat[t], ksg = (R)a.tksg[t];
-  const CellStat<R> sti = a.st_in[ci];
+  const CellStat<R> sti = a.st_in[__builtin_amdgcn_readfirstlane(ci)];  // scalar loads (ci is wave-uniform)
   const V3<R> Ci = {sti.cx, sti.cy, sti.cz};
   const R Ri = sti.r;
   const R Kvi = (R)a.Kv[ci], Kai = (R)a.Ka[ci], Ksi = (R)a.Ks[ci], Kbi = (R)a.Kb[ci];

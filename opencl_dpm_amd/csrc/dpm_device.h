@@ -168,6 +168,36 @@ __device__ __forceinline__ void block_max_sum_max(double& m1, double& s, double&
   __syncthreads();
 }
 
+// Two block-wide maxima in one pass (every thread gets them); scratch: >= 34
+// values of LDS.  Contains __syncthreads: call from all threads.
+template <typename R>
+__device__ __forceinline__ void block_max2(R& a, R& b, R* scratch) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
+  const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
+  for (int o = 32; o > 0; o >>= 1) {
+    a = fmax(a, __shfl_down(a, o, DPM_WAVE));
+    b = fmax(b, __shfl_down(b, o, DPM_WAVE));
+  }
+  if (lane == 0) {
+    scratch[wave] = a;
+    scratch[16 + wave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    R x = scratch[0], y = scratch[16];
+    for (int w = 1; w < nw; w++) {
+      x = fmax(x, scratch[w]);
+      y = fmax(y, scratch[16 + w]);
+    }
+    scratch[32] = x;
+    scratch[33] = y;
+  }
+  __syncthreads();
+  a = scratch[32];
+  b = scratch[33];
+  __syncthreads();
+}
+
 template <typename R>
 __device__ __forceinline__ R block_max(R v, R* scratch) {
   const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
