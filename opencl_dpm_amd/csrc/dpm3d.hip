@@ -378,15 +378,30 @@ __device__ __forceinline__ void load_own(const Step3D<R>& a, const Lds3D<R>& l, 
 }
 
 // Kat/2 (d/L0 - 1) delta/d with delta = (b - a) - L n, the adhesion term of
-// Cell3D_Kernel.cl:354-357, evaluated in double.
+// Cell3D_Kernel.cl:354-357, evaluated in double.  Float steps take it as
+// Kat/2 (1/L0 - 1/d) delta with 1/d from the hardware reciprocal square root
+// refined by two Newton steps (each squares the relative error) and kh =
+// Kat/2, inv_l0 = 1/L0 taken once per candidate pair — the same double value
+// to rounding, a third of the instructions.  Double steps keep the square
+// root and divisions (their registers are the tighter constraint there).
 template <typename R>
-__device__ __forceinline__ V3<R> adhesion_term(V3<R> b, V3<R> a, V3<R> n, R L, R Kat, R L0) {
+__device__ __forceinline__ V3<R> adhesion_term(V3<R> b, V3<R> a, V3<R> n, R L, R Kat, R L0, double kh,
+                                               double inv_l0) {
   const double Ld = (double)L;
   const double dx = ((double)b.x - (double)a.x) - Ld * (double)n.x;
   const double dy = ((double)b.y - (double)a.y) - Ld * (double)n.y;
   const double dz = ((double)b.z - (double)a.z) - Ld * (double)n.z;
-  const double d = sqrt(dx * dx + dy * dy + dz * dz);
-  const double k = (double)Kat * 0.5 * (d / (double)L0 - 1.0) / d;
+  double k;
+  if constexpr (sizeof(R) == 4) {
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    double y = __builtin_amdgcn_rsq(d2);
+    y = y * (1.5 - (0.5 * d2) * (y * y));
+    y = y * (1.5 - (0.5 * d2) * (y * y));
+    k = kh * (inv_l0 - y);
+  } else {
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    k = (double)Kat * 0.5 * (d / (double)L0 - 1.0) / d;
+  }
   return {(R)(k * dx), (R)(k * dy), (R)(k * dz)};
 }
 
@@ -1025,6 +1040,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
               const R half_l = R(0.49f) * L;
               const R lim_own = (L0i * R(2)) * (L0i * R(2)) * R(1.00001f);
               const R lim_rev = (L0j * R(2)) * (L0j * R(2)) * R(1.00001f);
+              const double kh = 0.5 * (double)Kat, inv_l0i = 1.0 / (double)L0i, inv_l0j = 1.0 / (double)L0j;
               const int m1 = min(nnear, (sl + 1) * per);
               for (int m = sl * per; m < m1; m++) {
                 const int vj = jl[m];
@@ -1051,7 +1067,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
                 if (d2 < lim_own) {
                   const R ds = sqrt(d2);
                   if (ds < L0i * R(2) && ds > R(1e-12f)) {
-                    Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i));
+                    Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i, kh, inv_l0i));
                     cnt_own++;
                   }
                 }
@@ -1059,7 +1075,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
                 if (d2r < lim_rev) {
                   const R dsr = sqrt(d2r);
                   if (dsr < L0j * R(2) && dsr > R(1e-12f))
-                    Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j));
+                    Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j, kh, inv_l0j));
                 }
               }
               acc[sl * nmine + iv] = Q4<R>{Fa.x, Fa.y, Fa.z, (R)cnt_own};

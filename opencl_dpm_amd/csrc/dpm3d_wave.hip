@@ -558,6 +558,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
           const R half_l = R(0.49f) * L;
           const R lim_own = (L0i * R(2)) * (L0i * R(2)) * R(1.00001f);
           const R lim_rev = (L0j * R(2)) * (L0j * R(2)) * R(1.00001f);
+          const double kh = 0.5 * (double)Kat, inv_l0i = 1.0 / (double)L0i, inv_l0j = 1.0 / (double)L0j;
           for (int tw = lane; tw < S * nmine; tw += DPM_WAVE) {
             const int iv = tw % nmine, sl = tw / nmine;
             const V3<R> Pv = lds_v(sx, sy, sz, (int)il[iv]);
@@ -588,13 +589,13 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
               if (d2 < lim_own) {
                 const R dsq = sqrt(d2);
                 if (dsq < L0i * R(2) && dsq > R(1e-12f)) {
-                  Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i));
+                  Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i, kh, inv_l0i));
                   cnt_own++;
                 }
               }
               if (d2r < lim_rev) {
                 const R dsr = sqrt(d2r);
-                if (dsr < L0j * R(2) && dsr > R(1e-12f)) Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j));
+                if (dsr < L0j * R(2) && dsr > R(1e-12f)) Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j, kh, inv_l0j));
               }
             }
             ut[sl * nmine + iv] = Q4<R>{Fa.x, Fa.y, Fa.z, (R)cnt_own};
