@@ -14,9 +14,10 @@ Workload (one "step" = one overdamped Euler step of every vertex):
 `value` is the batched whole-job rate in vertex-steps/s with all inputs
 resident in HBM; the plain rate is reported beside it.
 
-roofline: the only kernel of a batched step is k3d_wstep_f32 (one wave per
-cell; the plain 256-cell tissue runs k3d_step_f32, one workgroup per cell,
-which fills the device better at that size).  Algorithmic
+roofline: the only kernel of a batched step is k3d_wstep_f32_v162 (one wave
+per cell; the wave-form step compiled for 162-vertex, 320-face cells; the
+plain 256-cell tissue runs k3d_step_f32, one workgroup per cell, which fills
+the device better at that size).  Algorithmic
 bytes per vertex-step = 24 (position read + write, float xyz) + 24/162
 (per-cell state amortised) = 24.15 B; `achieved` = that x vertices per
 launch / the average launch duration measured here with HIP events
@@ -221,7 +222,7 @@ def main():
         "winding_evals_per_step": windings,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "k3d_wstep_f32" if b_kind == 2 else "k3d_step_f32", "launch_ms": b_launch,
+                     "kernel": "k3d_wstep_f32_v162" if b_kind == 2 else "k3d_step_f32", "launch_ms": b_launch,
                      "bytes_per_vertex_step": B_ALG},
     }
     if not args.no_cpu:

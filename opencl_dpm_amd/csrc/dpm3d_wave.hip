@@ -140,10 +140,13 @@ __device__ __forceinline__ R winding_w(const R* nx, const R* ny, const R* nz, co
   return (R)(__shfl(tot, 0, DPM_WAVE) / (double)(R(4) * R(3.14159265358979323846)));
 }
 
-template <typename R>
+// NVC, NFC: vertex and face counts fixed at compile time (0: taken from the
+// arguments).  Fixed sizes turn every per-wave LDS array offset into an
+// immediate and bound the per-vertex loops (the 162-vertex icosphere).
+template <typename R, int NVC, int NFC>
 __device__ void k3d_wstep_body(const Step3D<R>& a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int nv = a.nv, nf = a.nf;
+  const int nv = NVC > 0 ? NVC : a.nv, nf = NFC > 0 ? NFC : a.nf;
   const int lane = threadIdx.x & (DPM_WAVE - 1), wv = threadIdx.x / DPM_WAVE;
   const int cpw = blockDim.x / DPM_WAVE;
   const Lds3DWave LW = ldsw_layout(nv, nf, cpw, (int)sizeof(R));
@@ -764,5 +767,11 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   }
 }
 
-extern "C" __global__ void __launch_bounds__(256, 4) k3d_wstep_f32(const Step3D<float> a) { k3d_wstep_body(a); }
-extern "C" __global__ void __launch_bounds__(256) k3d_wstep_f64(const Step3D<double> a) { k3d_wstep_body(a); }
+extern "C" __global__ void __launch_bounds__(256, 4) k3d_wstep_f32(const Step3D<float> a) {
+  k3d_wstep_body<float, 0, 0>(a);
+}
+// the icosphere of subdivision 2 (162 vertices, 320 faces)
+extern "C" __global__ void __launch_bounds__(256, 4) k3d_wstep_f32_v162(const Step3D<float> a) {
+  k3d_wstep_body<float, 162, 320>(a);
+}
+extern "C" __global__ void __launch_bounds__(256) k3d_wstep_f64(const Step3D<double> a) { k3d_wstep_body<double, 0, 0>(a); }

@@ -257,7 +257,10 @@ static bool all_finite(const R* p, size_t n) {
 }
 
 // The wave-per-cell step kernel of a precision.
-static const void* wstep_kernel(int rs) {
+// The wave form's kernel for a handle: the compile-time-sized instance for
+// float 162-vertex, 320-face meshes, else the general one.
+static const void* wstep_kernel(int rs, int nv, int nf) {
+  if (rs == 4 && nv == 162 && nf == 320) return (const void*)k3d_wstep_f32_v162;
   return rs == 8 ? (const void*)k3d_wstep_f64 : (const void*)k3d_wstep_f32;
 }
 
@@ -506,7 +509,7 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
         return fail(DPM_EDEVICE, "topology upload failed");
       }
       h->wtopo = q;
-      (void)hipFuncSetAttribute(wstep_kernel(rs), hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_w);
+      (void)hipFuncSetAttribute(wstep_kernel(rs, nv, nf), hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_w);
     }
   }
   h->lds_vjp = ldsvjp_layout(nv, nf, rs).total;
@@ -809,8 +812,7 @@ static int launch3d(dpm_tissue* h, Kind3D kind, int in, float dt, int flags, Edi
     }
     if (kind == K_STEP && h->wave_cpw &&
         (h->step_kernel == 2 || (h->step_kernel == 0 && h->C >= h->wave_min_cells))) {
-      void (*kw)(Step3D<R>) = nullptr;
-      if constexpr (sizeof(R) == 8) kw = k3d_wstep_f64; else kw = k3d_wstep_f32;
+      auto kw = reinterpret_cast<void (*)(Step3D<R>)>(const_cast<void*>(wstep_kernel(h->rs, h->nv, h->nf)));
       hipLaunchKernelGGL(kw, dim3((h->C + h->wave_cpw - 1) / h->wave_cpw), dim3(64 * h->wave_cpw), h->lds_w,
                          h->stream, a);
     } else {
@@ -905,7 +907,7 @@ static int set_wave_cpw(dpm_tissue* h, int cpw) {
   if (lds > 65536) return fail(DPM_ERANGE, "cells per workgroup exceed the LDS of a workgroup");
   h->wave_cpw = cpw;
   h->lds_w = lds;
-  HIPCHECK(hipFuncSetAttribute(wstep_kernel(h->rs), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  HIPCHECK(hipFuncSetAttribute(wstep_kernel(h->rs, h->nv, h->nf), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (h->graph) {
     (void)hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
