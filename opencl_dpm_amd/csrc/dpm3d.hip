@@ -584,13 +584,16 @@ __device__ __forceinline__ R winding_wave(const Lds3D<R>& l, Q4<R>* ut, int nv, 
 
 // One overdamped Euler step of every cell (or, without F_MOVE, one force
 // evaluation for diagnostics).  blockDim.x >= nv, a multiple of 64.
-template <typename R>
+// NVC, NFC, NTC, WWC: vertex and face counts, threads per workgroup and
+// winding waves fixed at compile time (0: taken from the arguments).
+template <typename R, int NVC, int NFC, int NTC, int WWC>
 __device__ void k3d_step_body(const Step3D<R>& a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int nv = a.nv, nf = a.nf, NT = blockDim.x, tid = threadIdx.x;
+  const int nv = NVC > 0 ? NVC : a.nv, nf = NFC > 0 ? NFC : a.nf, NT = NTC > 0 ? NTC : (int)blockDim.x, tid = threadIdx.x;
+  const int ww = WWC > 0 ? WWC : a.ww;
   const int lane = tid & (DPM_WAVE - 1), wave = tid / DPM_WAVE;
   const int ci = xcd_remap(blockIdx.x, gridDim.x);
-  Lds3D<R> l = carve3d<R>(smem, nv, nf, NT, a.ww);
+  Lds3D<R> l = carve3d<R>(smem, nv, nf, NT, ww);
 
   const int t = a.cell_tissue[ci];
   const int cb = a.tissue_begin[t], ce = a.tissue_begin[t + 1];
@@ -964,8 +967,8 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
             need = d2 <= rj * rj;
           }
           const int np = block_compact(need, tid, l.plist, l.cnt);
-          if (wave < a.ww) {
-            for (int pi = wave; pi < np; pi += a.ww) {  // wave-uniform
+          if (wave < ww) {
+            for (int pi = wave; pi < np; pi += ww) {  // wave-uniform
               const int v = l.plist[pi];
               const V3<R> Pv = lds_v(l.sx, l.sy, l.sz, v);
               // a point beyond the margin from j's hull has winding number 0
@@ -1608,8 +1611,16 @@ __device__ void k3d_vjp_body(const Vjp3D<R>& a) {
 
 extern "C" __global__ void __launch_bounds__(1024) k3d_stats_f32(const Step3D<float> a) { k3d_stats_body(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_stats_f64(const Step3D<double> a) { k3d_stats_body(a); }
-extern "C" __global__ void __launch_bounds__(1024) k3d_step_f32(const Step3D<float> a) { k3d_step_body(a); }
-extern "C" __global__ void __launch_bounds__(1024) k3d_step_f64(const Step3D<double> a) { k3d_step_body(a); }
+extern "C" __global__ void __launch_bounds__(1024) k3d_step_f32(const Step3D<float> a) { k3d_step_body<float, 0, 0, 0, 0>(a); }
+extern "C" __global__ void __launch_bounds__(1024) k3d_step_f64(const Step3D<double> a) { k3d_step_body<double, 0, 0, 0, 0>(a); }
+// Instances for the icospheres of subdivision 2 and 3 (162 / 642 vertices):
+// sizes, workgroup size and winding waves fixed at compile time, so LDS
+// offsets are immediates and the register budget is that of 256 or 704
+// threads instead of 1024 (the general kernel spills at 128 registers).
+// The host takes them where the handle's nv, nf, NT and ww match.
+extern "C" __global__ void __launch_bounds__(256) k3d_step_f32_v162(const Step3D<float> a) { k3d_step_body<float, 162, 320, 256, 4>(a); }
+extern "C" __global__ void __launch_bounds__(704) k3d_step_f32_v642(const Step3D<float> a) { k3d_step_body<float, 642, 1280, 704, 4>(a); }
+extern "C" __global__ void __launch_bounds__(704) k3d_step_f64_v642(const Step3D<double> a) { k3d_step_body<double, 642, 1280, 704, 2>(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_energy_f32(const Step3D<float> a) { k3d_energy_body(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_energy_f64(const Step3D<double> a) { k3d_energy_body(a); }
 // lambda += dt * (contributions from other cells), contributions cleared.

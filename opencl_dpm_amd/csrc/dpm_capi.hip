@@ -474,6 +474,9 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
                             {(const void*)k3d_step_f64, (const void*)k3d_stats_f64, (const void*)k3d_energy_f64}};
   for (const void* k : kern[rs == 8 ? 1 : 0])
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
+  (void)hipFuncSetAttribute((const void*)k3d_step_f32_v162, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
+  (void)hipFuncSetAttribute((const void*)k3d_step_f32_v642, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
+  (void)hipFuncSetAttribute((const void*)k3d_step_f64_v642, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
   (void)hipFuncSetAttribute(rs == 8 ? (const void*)k3d_bin_f64 : (const void*)k3d_bin_f32,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->bin_lds);
   if (nv <= kWaveMaxNv) {  // cells per workgroup: as many as fit 64 KiB of LDS, at most 4
@@ -789,6 +792,12 @@ static int launch3d(dpm_tissue* h, Kind3D kind, int in, float dt, int flags, Edi
       k = kind == K_STEP ? k3d_step_f64 : kind == K_STATS ? k3d_stats_f64 : k3d_energy_f64;
     else
       k = kind == K_STEP ? k3d_step_f32 : kind == K_STATS ? k3d_stats_f32 : k3d_energy_f32;
+    if constexpr (sizeof(R) == 4)
+      if (kind == K_STEP && h->ww == 4 && h->nv == 162 && h->nf == 320 && h->NT == 256) k = k3d_step_f32_v162;
+    if constexpr (sizeof(R) == 4)
+      if (kind == K_STEP && h->ww == 4 && h->nv == 642 && h->nf == 1280 && h->NT == 704) k = k3d_step_f32_v642;
+    if constexpr (sizeof(R) == 8)
+      if (kind == K_STEP && h->ww == 2 && h->nv == 642 && h->nf == 1280 && h->NT == 704) k = k3d_step_f64_v642;
     if (kind == K_STEP && a.bin_list) {  // this step's cell list, from its input statistics
       Bin3D<R> b{};
       b.tissue_begin = h->tissue_begin;
