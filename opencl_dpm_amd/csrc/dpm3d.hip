@@ -640,16 +640,17 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
     const bool sub_on = (mask & M_SUBSTRATE) && !(Ksi < R(1e-12f));
     const R sc = Kai * sqrt(a0i) / l0i * R(0.3f);
     const R inv_l0 = R(1) / l0i;
-    if (sizeof(R) == 4 && a.closed) {  // block-uniform
+    if ((sizeof(R) == 4 || NTC > 0) && a.closed) {  // block-uniform
       // Closed mesh, neighbour rows in ring order (face q is (v, u_q, u_(q+1))):
       // one walk around each vertex's ring, as the wave-per-cell step takes
       // it — volume kvol sum_q d_q x d_(q+1), edge springs 2 sc sum_q s(d_q),
       // bending's first sum, substrate per facing-down incident face.  A
       // ring edge shorter than 2e-12 (the reference skips such faces'
-      // springs) sends the cell back to the face pass below.  Float only:
-      // in double the walk's registers push the rest of the step into
-      // spills (measured 5 % slower on 4096 x 642 cells), and the face pass
-      // stays.
+      // springs) sends the cell back to the face pass below.  Float steps
+      // and the compile-time-sized instances only: in the general double
+      // kernel (128 registers at 1024 threads) the walk's registers push
+      // the rest of the step into spills (measured 5 % slower on 4096 x 642
+      // cells), and the face pass stays.
       unsigned* dbits = reinterpret_cast<unsigned*>(l.fcx);  // dead until the candidates
       if (sub_on) {
         for (int f0 = wave * DPM_WAVE; f0 < nf; f0 += NT) {  // 64 consecutive faces per wave
@@ -929,11 +930,12 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
         // closer than the culling margin m_j to a face of j (contact_margin),
         // adhesion a vertex pair closer than 2 l0 — on pairs that take no
         // periodic image (the unshifted centres and radii keep every
-        // component below 0.48 L; the shift is 0 then).  In double the extra
-        // registers cost more than the skipped work (spills, measured on
-        // 4096 x 642 cells), and every candidate is evaluated.
+        // component below 0.48 L; the shift is 0 then).  In the general
+        // double kernel the extra registers cost more than the skipped work
+        // (spills, measured on 4096 x 642 cells), and every candidate is
+        // evaluated.
         bool skip_contact = false, skip_adh = false;
-        if constexpr (sizeof(R) == 4) {
+        if constexpr (sizeof(R) == 4 || NTC > 0) {
           const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
           const R l0m = fmax(L0i, fmax(stj.l0, R(1e-12f)));
           const bool all_images = pbc && R(2) * (Ri + stj.r + R(2) * l0m) >= R(0.999f) * L;
