@@ -116,6 +116,7 @@ template <typename R>
 __device__ __forceinline__ R winding_w(const R* nx, const R* ny, const R* nz, const unsigned* sf, Q4<R>* ut,
                                        int nv, int nf, V3<R> sh, V3<R> p) {
   const int lane = threadIdx.x & (DPM_WAVE - 1);
+  #pragma unroll 1
   for (int v = lane; v < nv; v += DPM_WAVE) {
     const V3<R> A = {(nx[v] + sh.x) - p.x, (ny[v] + sh.y) - p.y, (nz[v] + sh.z) - p.z};
     const V3<R> u = vnormalize(A);
@@ -123,6 +124,7 @@ __device__ __forceinline__ R winding_w(const R* nx, const R* ny, const R* nz, co
   }
   wave_lds_sync();
   R part = R(0);
+  #pragma unroll 1
   for (int f = lane; f < nf; f += DPM_WAVE) {
     int ia, ib, ic;
     wface(sf, f, ia, ib, ic);
@@ -142,12 +144,18 @@ __device__ __forceinline__ R winding_w(const R* nx, const R* ny, const R* nz, co
 
 // NVC, NFC: vertex and face counts fixed at compile time (0: taken from the
 // arguments).  Fixed sizes turn every per-wave LDS array offset into an
-// immediate and bound the per-vertex loops (the 162-vertex icosphere).
+// immediate (the 162-vertex icosphere).  The vertex and face loops stay
+// rolled: unrolled, their iterations' registers spill to scratch memory
+// (208 B per lane, 6x the step's HBM traffic, at the same speed).
 template <typename R, int NVC, int NFC>
 __device__ void k3d_wstep_body(const Step3D<R>& a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nv = NVC > 0 ? NVC : a.nv, nf = NFC > 0 ? NFC : a.nf;
-  const int lane = threadIdx.x & (DPM_WAVE - 1), wv = threadIdx.x / DPM_WAVE;
+  // wv, and every per-cell value derived from it, is wave-uniform: through
+  // readfirstlane it lives in scalar registers (the cell's parameters and
+  // statistics come by scalar loads), which keeps the step within 128 vector
+  // registers without spilling to scratch memory
+  const int lane = threadIdx.x & (DPM_WAVE - 1), wv = __builtin_amdgcn_readfirstlane(threadIdx.x / DPM_WAVE);
   const int cpw = blockDim.x / DPM_WAVE;
   const Lds3DWave LW = ldsw_layout(nv, nf, cpw, (int)sizeof(R));
   unsigned* sf = reinterpret_cast<unsigned*>(smem);
@@ -179,7 +187,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   const R L = (R)a.tL[t];
   const int pbc = a.tpbc[t];
   const R Kre = (R)a.tKre[t], Kat = (R)a.tKat[t], ksg = (R)a.tksg[t];
-  const CellStat<R> sti = a.st_in[__builtin_amdgcn_readfirstlane(ci)];  // scalar loads (ci is wave-uniform)
+  const CellStat<R> sti = a.st_in[ci];
   const V3<R> Ci = {sti.cx, sti.cy, sti.cz};
   const R Ri = sti.r;
   const R Kvi = (R)a.Kv[ci], Kai = (R)a.Ka[ci], Ksi = (R)a.Ks[ci], Kbi = (R)a.Kb[ci];
@@ -189,6 +197,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   const V3<R> zero = {R(0), R(0), R(0)};
 
   wave_stage_cell(sx, sy, sz, a.xin, a.yin, a.zin, base, nv, lane);
+  #pragma unroll 1
   for (int v = lane; v < nv; v += DPM_WAVE) {
     if (a.w_out) a.w_out[base + v] = R(0);
     if (a.rec_cell)
@@ -210,6 +219,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   bool degenerate = false;
   if (shape_on && (sub_on || surf_on)) {  // wave-uniform
     R e2min = R(3.0e38f);
+    #pragma unroll 1
     for (int f0 = 0; f0 < nf; f0 += DPM_WAVE) {
       const int f = f0 + lane;
       bool dn = false;
@@ -240,6 +250,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   }
   const bool bend_on = (mask & M_BENDING) && Kbi != R(0);
   const bool ring_pass = shape_on && !degenerate && a.closed;  // wave-uniform
+  #pragma unroll 1
   for (int v = lane; v < nv; v += DPM_WAVE) {
     const V3<R> P = lds_v(sx, sy, sz, v);
     V3<R> F = zero;
@@ -341,6 +352,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       nz[v] = s.z;
     }
     wave_lds_sync();
+    #pragma unroll 1
     for (int v = lane; v < nv; v += DPM_WAVE) {
       V3<R> acc = zero;
       int deg = 0;
@@ -513,6 +525,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         // precede its stores).
         bool any_need = false;
         int nnear = 0, nmine = 0;
+        #pragma unroll 1
         for (int k0v = 0; k0v < nv; k0v += DPM_WAVE) {
           const int v = k0v + lane;
           bool need = false, near = false, mine = false;
@@ -628,6 +641,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
             wave_stage_cell(nx, ny, nz, a.xin, a.yin, a.zin, (size_t)cj2 * nv, nv, lane);
           }
           wave_lds_sync();
+          #pragma unroll 1
           for (int k0v = 0; k0v < nv; k0v += DPM_WAVE) {
             const int v = k0v + lane;
             bool need = false;
@@ -688,6 +702,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
 
   // ------------------------------------------------------- Euler and stats
   double px = 0.0, py = 0.0, pz = 0.0;
+  #pragma unroll 1
   for (int v = lane; v < nv; v += DPM_WAVE) {
     const V3<R> F = lds_v(fx, fy, fz, v);
     if (a.flags & F_WRITE_FORCES) {
@@ -728,6 +743,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
     pz = wsum_all(pz);
     const V3<R> C = {(R)(px / nv), (R)(py / nv), (R)(pz / nv)};
     R r2 = R(0);  // max |d|^2, then one square root (monotonic, correctly rounded)
+    #pragma unroll 1
     for (int v = lane; v < nv; v += DPM_WAVE) {
       const V3<R> d = vsub(lds_v(sx, sy, sz, v), C);
       r2 = fmax(r2, vdot(d, d));
@@ -735,6 +751,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
     const R r = sqrt(r2);
     double vol = 0.0;
     R e2 = R(0);
+    #pragma unroll 1
     for (int f = lane; f < nf; f += DPM_WAVE) {
       int i0, i1, i2;
       wface(sf, f, i0, i1, i2);
