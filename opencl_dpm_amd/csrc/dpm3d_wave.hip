@@ -84,6 +84,11 @@ __device__ __forceinline__ double wmax_all(double v) {
   return v;
 }
 
+// Set bits of a ballot in the lanes below this one (v_mbcnt).
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
 // One cell's positions (nv <= kWaveMaxNv) into a wave's LDS arrays, every
 // load issued before the first store (one memory round trip, not one per
 // 64 vertices).
@@ -453,7 +458,6 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         nsrc = n;
       }
     }
-    const unsigned long long below = (1ull << lane) - 1ull;
     int c0 = 0;
     while (c0 < nsrc) {  // wave-uniform: candidates in batches of at most kWaveCand
       int ncl = 0;
@@ -480,7 +484,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
           cand = cand || vdot(d, d) <= rr * rr;
         }
         const unsigned long long cmask = __ballot(cand);
-        if (cand) cl[ncl + __popcll(cmask & below)] = cj;  // ascending
+        if (cand) cl[ncl + lanes_below(cmask)] = cj;  // ascending
         ncl += __popcll(cmask);
         c0 += DPM_WAVE;
       }
@@ -547,12 +551,12 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
           any_need = any_need || __ballot(need) != 0ull;
           const unsigned long long mn = __ballot(near), mm = __ballot(mine);
           if (near) {
-            const int jn = nnear + __popcll(mn & below);
+            const int jn = nnear + lanes_below(mn);
             nx[jn] = q.x;
             ny[jn] = q.y;
             nz[jn] = q.z;
           }
-          const int slot = nmine + __popcll(mm & below);
+          const int slot = nmine + lanes_below(mm);
           if (mine) il[slot] = (unsigned short)v;
           nnear += __popcll(mn);
           nmine += __popcll(mm);
