@@ -17,6 +17,7 @@
 constexpr int kWaveGlist = 128;  // gathered cell-list candidates per wave
 constexpr int kWaveMaxNv = 256;  // meshes the wave-per-cell step takes
 constexpr int kWaveCand = 128;   // candidate cells per batch
+static_assert(kWaveGlist <= kWaveCand, "the gathered list is sorted into the candidate array");
 
 // LDS: packed faces shared by the workgroup, then one slice per wave.  The
 // winding directions (or adhesion slices) share their bytes with the
@@ -87,6 +88,18 @@ __device__ __forceinline__ double wmax_all(double v) {
 // Set bits of a ballot in the lanes below this one (v_mbcnt).
 __device__ __forceinline__ int lanes_below(unsigned long long m) {
   return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// A wave-uniform value computed by vector instructions, moved to a scalar
+// register (every lane holds the same bits).
+__device__ __forceinline__ float wave_uniform(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double wave_uniform(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // One cell's positions (nv <= kWaveMaxNv) into a wave's LDS arrays, every
@@ -380,14 +393,13 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   // ------------------------------------------- contact and vertex adhesion
   const bool do_contact = (mask & M_CONTACT) && Kre != R(0);
   const bool do_adh = (mask & M_ADHESION) && Kat != R(0);
-  unsigned long long n_wind = 0, n_contact = 0, n_cand = 0;
+  unsigned n_wind = 0, n_contact = 0, n_cand = 0;  // per step and cell: 32 bits suffice
   if (do_contact || do_adh) {  // wave-uniform
-    const R L0i = fmax(l0i, R(1e-12f));
-    const R invL = R(1) / L;
+    const R L0i = wave_uniform(fmax(l0i, R(1e-12f)));
+    const R invL = wave_uniform(R(1) / L);
     // candidate source: every cell of the tissue, or the cells of the 27
     // buckets around this cell sorted by index (held two per lane)
     bool gathered = false;
-    int g0 = 0, g1 = 0;
     int nsrc = ce - cb;
     if (a.bin_list && a.tbin_nb[t] > 0) {
       const int nb = a.tbin_nb[t], nbuck = a.tbin_nbuck[t];
@@ -447,13 +459,11 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
           rk0 += g < val0;
           rk1 += g < val1;
         }
+        // the sorted list goes to the candidate array, which the scan below
+        // filters in place (a candidate's slot never lies above its source)
+        if (lane < n) cl[rk0] = val0;
+        if (lane + DPM_WAVE < n) cl[rk1] = val1;
         wave_lds_sync();
-        if (lane < n) gl[rk0] = val0;
-        if (lane + DPM_WAVE < n) gl[rk1] = val1;
-        wave_lds_sync();
-        if (lane < n) g0 = gl[lane];
-        if (lane + DPM_WAVE < n) g1 = gl[lane + DPM_WAVE];
-        wave_lds_sync();  // the gather's bytes are the winding table's
         gathered = true;
         nsrc = n;
       }
@@ -463,7 +473,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       int ncl = 0;
       while (c0 < nsrc && ncl + DPM_WAVE <= kWaveCand) {
         const int k0 = c0 + lane;
-        const int cj = k0 < nsrc ? (gathered ? (c0 == 0 ? g0 : g1) : cb + k0) : ce;
+        const int cj = k0 < nsrc ? (gathered ? cl[k0] : cb + k0) : ce;
         bool cand = false;
         if (cj < ce && cj != ci) {
           const CellStat<R> stj = a.st_in[cj];
@@ -488,7 +498,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         ncl += __popcll(cmask);
         c0 += DPM_WAVE;
       }
-      n_cand += (unsigned long long)ncl;
+      n_cand += (unsigned)ncl;
       wave_lds_sync();
       for (int qc = 0; qc < ncl; qc++) {
         const int cj2 = cl[qc];
@@ -781,9 +791,9 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
     if (lane == 0) {
       unsigned long long* slot = a.counters + 4 * ((blockIdx.x * cpw + wv) % DPM_COUNTER_SLOTS);
       if (ci == 0) atomicAdd(&slot[0], 1ull);
-      if (n_wind) atomicAdd(&slot[1], n_wind);
+      if (n_wind) atomicAdd(&slot[1], (unsigned long long)n_wind);
       if (c) atomicAdd(&slot[2], c);
-      if (n_cand) atomicAdd(&slot[3], n_cand);
+      if (n_cand) atomicAdd(&slot[3], (unsigned long long)n_cand);
     }
   }
 }
