@@ -200,6 +200,19 @@ __device__ __forceinline__ bool outside_by_margin(const R* xs, const R* ys, cons
   return dn > R(0) && dn * dn - h >= ((m + R(1e-4f) * (dn + Rj)) + R(1e-6f) * M) * dn;
 }
 
+// Edge-spring factor (|d|/l0 - 1)/|d| of an edge vector with |d|^2 = d2
+// (Cell3D_Kernel.cl:140-176): in float 1/l0 - rsqrt(d2) by the hardware
+// reciprocal square root (about 1 ulp), in double as written.
+template <typename R>
+__device__ __forceinline__ R spring_factor(R d2, R inv_l0) {
+  if constexpr (sizeof(R) == 4) {
+    return inv_l0 - __builtin_amdgcn_rsqf(d2);
+  } else {
+    const R len = sqrt(d2);
+    return (len * inv_l0 - R(1)) / len;
+  }
+}
+
 // Facing-down decision of a face with normal n = (P1 - P0) x (P2 - P0):
 // normalize(n).z < -0.1 (Cell3D_Kernel.cl:206-214), decided from n_z and |n|^2
 // where n_z^2 and 0.01 |n|^2 are clear of each other by 1e-5 (relative), else
@@ -679,9 +692,9 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
 #pragma unroll
         for (int q = 0; q < DPM3D_MAXDEG; q++) {
           if (surf_on) {
-            const R len = sqrt(vdot(dq, dq));
-            bad = bad || len < R(2e-12f);
-            Fs = vadd(Fs, vscale(dq, (len * inv_l0 - R(1)) / len));
+            const R d2 = vdot(dq, dq);
+            bad = bad || d2 < R(4e-24f);
+            Fs = vadd(Fs, vscale(dq, spring_factor(d2, inv_l0)));
           }
           sb = vadd(sb, dq);
           const int un = q + 1 < DPM3D_MAXDEG ? adjr[q + 1] : -1;

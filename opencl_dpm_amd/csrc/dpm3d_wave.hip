@@ -288,10 +288,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       const V3<R> d0 = vsub(lds_v(sx, sy, sz, row16(arow, 0)), P);
       V3<R> dq = d0, Fv = zero, Fs = zero, sb = zero;
       for (int q = 0; q < DPM3D_MAXDEG; q++) {
-        if (surf_on) {
-          const R len = sqrt(vdot(dq, dq));
-          Fs = vadd(Fs, vscale(dq, (len * inv_l0 - R(1)) / len));
-        }
+        if (surf_on) Fs = vadd(Fs, vscale(dq, spring_factor(vdot(dq, dq), inv_l0)));
         sb = vadd(sb, dq);
         const int un = q + 1 < DPM3D_MAXDEG ? row16(arow, q + 1) : 0xffff;
         const V3<R> dn = un == 0xffff ? d0 : vsub(lds_v(sx, sy, sz, un), P);
