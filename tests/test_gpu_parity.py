@@ -84,6 +84,25 @@ def test_3d_forces_and_pairs(case, mask, kernel):
         assert g["contact"].sum() > 0
 
 
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_3d_pair_sets_across_the_separation_cutoffs(kernel):
+    # neighbours from overlapping to well apart (periodic 3 x 3 monolayer,
+    # spacing 1.9 ... 2.7): the centre-line separation crosses the contact
+    # culling margin (~0.4 e_max) and the adhesion reach 2 l0 on the way; the
+    # pairs it lets the step skip must be exactly the ones ref32 rejects
+    for k, sp in enumerate(np.linspace(1.9, 2.7, 17)):
+        m = W.monolayer3d(3, float(sp), seed=40 + k)
+        t = tissue3d(m)
+        t.set_step_kernel(kernel)
+        t.set_positions(m.X)
+        g = t.probe()
+        r32 = ref3d(m, m.X, capi.DPM3D_ALL, 32, pairs=True)
+        assert_forces(g["forces"], ref3d(m, m.X, capi.DPM3D_ALL, 64)["forces"], f"3d spacing={sp:.2f} kernel={kernel}")
+        np.testing.assert_array_equal(g["contact"], r32["contact"])
+        np.testing.assert_array_equal(g["adhesion"], r32["adhesion"])
+        print(f"spacing {sp:.2f}: contact={int(g['contact'].sum())} adhesion={int(g['adhesion'].sum())}", flush=True)
+
+
 @pytest.mark.parametrize("spacing,seed", [(1.7, 21), (1.6, 22)])
 def test_3d_pairs_after_deformation(spacing, seed):
     # cells squeezed against each other for 300 steps: deformed meshes with
