@@ -80,10 +80,6 @@ __device__ __forceinline__ R wmax_r(R v) {  // in the value's own precision
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, DPM_WAVE));
   return v;
 }
-__device__ __forceinline__ double wmax_all(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, DPM_WAVE));
-  return v;
-}
 
 // Set bits of a ballot in the lanes below this one (v_mbcnt).
 __device__ __forceinline__ int lanes_below(unsigned long long m) {
@@ -773,7 +769,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       const V3<R> e0 = vsub(P1, P0), e1 = vsub(P2, P1), ee = vsub(P0, P2);
       e2 = fmax(e2, fmax(vdot(e0, e0), fmax(vdot(e1, e1), vdot(ee, ee))));
     }
-    const double rd = wmax_all((double)r), e2d = wmax_all((double)e2);
+    const R rd = wmax_r(r), e2d = wmax_r(e2);  // maxima are exact in either precision
     vol = wsum_all(vol);
     if (lane == 0) {
       const R em = sqrt((R)e2d) * R(1.000001f);
