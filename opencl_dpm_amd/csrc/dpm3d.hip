@@ -1636,6 +1636,7 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step_f64(const Step3D<dou
 extern "C" __global__ void __launch_bounds__(256) k3d_step_f32_v162(const Step3D<float> a) { k3d_step_body<float, 162, 320, 256, 4>(a); }
 extern "C" __global__ void __launch_bounds__(704) k3d_step_f32_v642(const Step3D<float> a) { k3d_step_body<float, 642, 1280, 704, 4>(a); }
 extern "C" __global__ void __launch_bounds__(704) k3d_step_f64_v642(const Step3D<double> a) { k3d_step_body<double, 642, 1280, 704, 2>(a); }
+extern "C" __global__ void __launch_bounds__(256) k3d_step_f64_v162(const Step3D<double> a) { k3d_step_body<double, 162, 320, 256, 4>(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_energy_f32(const Step3D<float> a) { k3d_energy_body(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_energy_f64(const Step3D<double> a) { k3d_energy_body(a); }
 // lambda += dt * (contributions from other cells), contributions cleared.
@@ -1660,5 +1661,9 @@ extern "C" __global__ void k3d_vjp_cross_f64(double* lx, double* ly, double* lz,
 }
 extern "C" __global__ void __launch_bounds__(1024) k3d_vjp_f32(const Vjp3D<float> a) { k3d_vjp_body(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_vjp_f64(const Vjp3D<double> a) { k3d_vjp_body(a); }
+// the double adjoint for workgroups of 256 threads (meshes of up to 256
+// vertices): the register budget of 256 threads, where the general kernel
+// spills at 128 registers
+extern "C" __global__ void __launch_bounds__(256) k3d_vjp_f64_t256(const Vjp3D<double> a) { k3d_vjp_body(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_bin_f32(const Bin3D<float> a) { k3d_bin_body(a); }
 extern "C" __global__ void __launch_bounds__(1024) k3d_bin_f64(const Bin3D<double> a) { k3d_bin_body(a); }

@@ -477,6 +477,7 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
   (void)hipFuncSetAttribute((const void*)k3d_step_f32_v162, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
   (void)hipFuncSetAttribute((const void*)k3d_step_f32_v642, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
   (void)hipFuncSetAttribute((const void*)k3d_step_f64_v642, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
+  (void)hipFuncSetAttribute((const void*)k3d_step_f64_v162, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
   (void)hipFuncSetAttribute(rs == 8 ? (const void*)k3d_bin_f64 : (const void*)k3d_bin_f32,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->bin_lds);
   if (nv <= kWaveMaxNv) {  // cells per workgroup: as many as fit 64 KiB of LDS, at most 4
@@ -518,6 +519,9 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
   h->lds_vjp = ldsvjp_layout(nv, nf, rs).total;
   (void)hipFuncSetAttribute(rs == 8 ? (const void*)k3d_vjp_f64 : (const void*)k3d_vjp_f32,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_vjp);
+  if (rs == 8)
+    (void)hipFuncSetAttribute((const void*)k3d_vjp_f64_t256, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)h->lds_vjp);
   *out = h;
   return DPM_OK;
 }
@@ -798,6 +802,8 @@ static int launch3d(dpm_tissue* h, Kind3D kind, int in, float dt, int flags, Edi
       if (kind == K_STEP && h->ww == 4 && h->nv == 642 && h->nf == 1280 && h->NT == 704) k = k3d_step_f32_v642;
     if constexpr (sizeof(R) == 8)
       if (kind == K_STEP && h->ww == 2 && h->nv == 642 && h->nf == 1280 && h->NT == 704) k = k3d_step_f64_v642;
+    if constexpr (sizeof(R) == 8)
+      if (kind == K_STEP && h->ww == 4 && h->nv == 162 && h->nf == 320 && h->NT == 256) k = k3d_step_f64_v162;
     if (kind == K_STEP && a.bin_list) {  // this step's cell list, from its input statistics
       Bin3D<R> b{};
       b.tissue_begin = h->tissue_begin;
@@ -1274,7 +1280,10 @@ static int adjoint3d(dpm_tissue* h, const In* lambda_final, double* dtheta, Out*
       const int cb = (int)std::min<size_t>((NVT + 255) / 256, 1024);
       if (h->rs == 8) {
         Vjp3D<double> a = vjp_args<double>(h, s);
-        hipLaunchKernelGGL(k3d_vjp_f64, dim3(h->C), dim3(h->NT), h->lds_vjp, h->stream, a);
+        if (h->NT == 256)
+          hipLaunchKernelGGL(k3d_vjp_f64_t256, dim3(h->C), dim3(h->NT), h->lds_vjp, h->stream, a);
+        else
+          hipLaunchKernelGGL(k3d_vjp_f64, dim3(h->C), dim3(h->NT), h->lds_vjp, h->stream, a);
         hipLaunchKernelGGL(k3d_vjp_cross_f64, dim3(cb), dim3(256), 0, h->stream, a.lx, a.ly, a.lz, a.cx, a.cy,
                            a.cz, (double)a.dt, NVT);
       } else {
