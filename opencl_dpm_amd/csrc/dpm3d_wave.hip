@@ -470,13 +470,20 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         bool cand = false;
         if (cj < ce && cj != ci) {
           const CellStat<R> stj = a.st_in[cj];
+          // a superset test: the nearest image by the rounded product with
+          // 1/L (where that is ambiguous, near a half box length, both images
+          // lie beyond any reach below 0.4995 L; longer reaches take every
+          // cell); the pair's exact shift is taken again for a candidate
           V3<R> sh = zero;
           if (pbc)
-            sh = v3(image_shift(Ci.x - stj.cx, L, invL), image_shift(Ci.y - stj.cy, L, invL),
-                    image_shift(Ci.z - stj.cz, L, invL));
+            sh = v3(L * round((Ci.x - stj.cx) * invL), L * round((Ci.y - stj.cy) * invL),
+                    L * round((Ci.z - stj.cz) * invL));
           const V3<R> d = {stj.cx + sh.x - Ci.x, stj.cy + sh.y - Ci.y, stj.cz + sh.z - Ci.z};
           R reach = R(0);
-          if (do_contact) reach = Ri + stj.r + contact_margin(stj.r, stj.em);
+          if (do_contact) {
+            reach = Ri + stj.r + contact_margin(stj.r, stj.em);
+            if (pbc && R(2) * reach >= R(0.999f) * L) cand = true;
+          }
           if (do_adh) {
             const R l0m = fmax(L0i, fmax(stj.l0, R(1e-12f)));
             const R r2 = Ri + stj.r + R(2) * l0m;
@@ -520,11 +527,13 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         const R r2 = (stj.r + R(2) * l0m) * R(1.0001f) + R(1e-6f);
         // Separation along the centre line n = (C_j - C_i) / D: every vertex p
         // of this cell has (p - C_i).n <= hi and every point x of cell j's
-        // (shifted) hull has (C_j - x).n <= hj, so |x - p| >= D - hi - hj.
+        // (shifted) hull has (C_j - x).n <= hj, so |x - p| >= D - hi - hj
+        // = min (q + sh - C_i).n - max (p - C_j).n - D over the vertices.
         const V3<R> dC = vsub(Cj, Ci);
         const R D = sqrt(vdot(dC, dC));
         const V3<R> nC = D > R(0) ? vscale(dC, R(1) / D) : zero;
-        R hi = R(-3.0e38f), hj = R(-3.0e38f);
+        R hp = R(-3.0e38f), hq = R(3.0e38f);
+        const V3<R> Cis = vsub(Ci, sh);
         // One pass: the separation, whether an own vertex is within contact
         // reach, and the adhesion lists.  Cell j's vertices in reach are
         // compacted in place to the front of nx/ny/nz (ascending; every slot
@@ -540,14 +549,14 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
           if (v < nv) {
             const V3<R> P = lds_v(sx, sy, sz, v);
             q = lds_v(nx, ny, nz, v);
-            hi = fmax(hi, vdot(vsub(P, Ci), nC));
-            hj = fmax(hj, vdot(v3(stj.cx - q.x, stj.cy - q.y, stj.cz - q.z), nC));
             const V3<R> dp = vsub(P, Cj);
+            const V3<R> ds = vsub(q, Cis);
+            hp = fmax(hp, vdot(dp, nC));
+            hq = fmin(hq, vdot(ds, nC));
             const R d2 = vdot(dp, dp);
             need = do_contact && d2 <= rj * rj;
             if (do_adh) {
               mine = all_images || d2 <= r2 * r2;
-              const V3<R> ds = {q.x + sh.x - Ci.x, q.y + sh.y - Ci.y, q.z + sh.z - Ci.z};
               near = all_images || vdot(ds, ds) <= rk * rk;
             }
           }
@@ -564,12 +573,12 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
           nnear += __popcll(mn);
           nmine += __popcll(mm);
         }
-        hi = wmax_r(hi);
-        hj = wmax_r(hj);
+        hp = wmax_r(hp);
+        for (int o = 32; o > 0; o >>= 1) hq = fmin(hq, __shfl_xor(hq, o, DPM_WAVE));
         // the separation less slack for the float sums (relative 1e-4 of the
         // cells' extent, 1e-6 of the coordinates); -1 when it is not defined
         const R Mc = ((fabs(Ci.x) + fabs(Ci.y)) + fabs(Ci.z)) + ((fabs(Cj.x) + fabs(Cj.y)) + fabs(Cj.z));
-        const R gap = D > R(0) ? ((D - hi) - hj) - (R(1e-4f) * ((Ri + stj.r) + D) + R(1e-6f) * Mc) : R(-1);
+        const R gap = D > R(0) ? ((hq - hp) - D) - (R(1e-4f) * ((Ri + stj.r) + D) + R(1e-6f) * Mc) : R(-1);
         // Adhesion: no pair of vertices closer than 2 l0 when gap >= 2 l0 (on
         // pairs that take no image; the others are always evaluated).
         if (do_adh && nnear > 0 && nmine > 0 && (all_images || !no_image || !(gap >= R(2) * l0m))) {  // wave-uniform
