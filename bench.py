@@ -27,6 +27,10 @@ cpu_baseline: the reference algorithm restated (oracle ref32: all-pairs,
 serial float, the four terms the reference launches), OpenMP over cells on
 the host cores, one step of the same 256-cell tissue.
 
+reference_2d: the reference's CPU-runnable 2D case (16 polygons x 24
+vertices, 1000 Euler steps) through the 2D step, with the oracle's serial
+float run of the same case on one host core beside it.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
 from __future__ import annotations
@@ -102,6 +106,39 @@ def cpu_baseline(m, steps):
             "sample": f"{steps} step(s) of the 256-cell x 162-vertex monolayer, the reference's four "
                       f"terms (volume, surface, substrate, all-pairs winding contact), float, "
                       f"OpenMP over cells: {sec:.2f} s"}
+
+
+def plain2d_gpu(capi, hiptime, W, steps=1000, dt=0.005):
+    """The reference's CPU-runnable 2D case (16 bidisperse 24-vertex polygons at packing
+    fraction 0.6, Ka = Kl = 1, Kb = 0, vertex repulsion, dt = 0.005, 1000 Euler steps),
+    through the 2D step (one launch for all steps), timed with HIP events."""
+    p = W.tissue2d(16, 24, 1.4, 0.6, seed=0)
+    t = capi.Tissues2D([0, p.C], p.nv, p.Ka, p.Kl, p.Kb, p.a0, p.l0, p.r0, p.L, Kre=p.Kre, Kat=p.Kat,
+                       mask=capi.DPM2D_ALL)
+    t.set_positions(p.X)
+    t.euler(steps, dt)  # warm-up
+    t.set_positions(p.X)
+    e0, e1 = hiptime.Event(), hiptime.Event()
+    s = t.stream()
+    e0.record(s)
+    t.euler_async(steps, dt)
+    e1.record(s)
+    ms = e0.elapsed_ms(e1)
+    nvt = int(sum(p.nv))
+    return p, {"workload": "2D, 16 polygons x 24 vertices, packing fraction 0.6, 1000 Euler steps, dt=0.005",
+               "steps": steps, "ms": ms, "value": nvt * steps / (ms / 1e3), "unit": "vertex-steps/s"}
+
+
+def plain2d_cpu(p, steps=1000, dt=0.005):
+    """The same 2D run by the oracle's float restatement of the reference kernels,
+    serial on one host core."""
+    import oracle
+    P, stride = p.padded()
+    t0 = time.perf_counter()
+    oracle.run2d(P, stride, p.nv, p.Ka, p.Kl, p.Kb, p.a0, p.l0, p.r0, p.L, p.Kre, p.Kat, steps, dt, prec=32)
+    sec = time.perf_counter() - t0
+    nvt = int(sum(p.nv))
+    return {"value": nvt * steps / sec, "unit": "vertex-steps/s", "cores": 1, "kind": "port", "seconds": sec}
 
 
 def measured_traffic_per_vertex_step():
@@ -225,10 +262,15 @@ def main():
                      "kernel": "k3d_wstep_f32_v162" if b_kind == 2 else "k3d_step_f32", "launch_ms": b_launch,
                      "bytes_per_vertex_step": B_ALG},
     }
+    p2, out["reference_2d"] = plain2d_gpu(capi, hiptime, W)
+    log(f"[bench] 2D reference case: {out['reference_2d']['ms']:.2f} ms for 1000 steps "
+        f"({out['reference_2d']['value']:.4g} vertex-steps/s)")
     if not args.no_cpu:
         log(f"[bench] cpu baseline: {args.cpu_steps} step(s)")
         out["cpu_baseline"] = cpu_baseline(m, args.cpu_steps)
         log(f"[bench] cpu {out['cpu_baseline']['value']:.4g} vertex-steps/s")
+        out["reference_2d"]["cpu"] = plain2d_cpu(p2)
+        log(f"[bench] 2D reference case on the cpu: {out['reference_2d']['cpu']['seconds']:.2f} s")
     print(json.dumps(out), flush=True)
 
 
