@@ -382,7 +382,9 @@ __device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DC
   const bool scan_all = nc > DPM2D_MAXCAND;
   const int ncl = scan_all ? T.nc : nc;
   const R half_l = R(0.49f) * L;
-  if (mask & M2_ATTRACTION) {
+  // With Kat = 0 the attraction adds exact zeros: it is evaluated only where
+  // its partner count is asked for (n_attr).
+  if ((mask & M2_ATTRACTION) && (T.Kat != R(0) || n_attr)) {
     const R l0 = (R)g.l0[gc];
     const R kat = T.Kat / (R)n;
     const R lim = l0 * l0 * R(1.00001f);
@@ -417,7 +419,7 @@ __device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DC
         }
       }
     }
-    *n_attr = cnt;
+    if (n_attr) *n_attr = cnt;
   }
   if (mask & M2_VREPULSION) {
     // U = Kre/2 (1 - d/sigma)^2 for d < sigma = (l0_i + l0_j)/2
@@ -523,7 +525,7 @@ extern "C" __global__ void __launch_bounds__(1024) k2d_run(const Run2D a) {
       rfy[r] = 0.0f;
       if (k < T.nvt) {
         int no = 0, na = 0;
-        force_vertex(g, T, l, k, rfx[r], rfy[r], &no, &na, (double*)nullptr, (double*)nullptr);
+        force_vertex(g, T, l, k, rfx[r], rfy[r], &no, a.p_attr ? &na : nullptr, (double*)nullptr, (double*)nullptr);
         n_over_total += no;
         if (a.p_overlap) a.p_overlap[T.v0 + k] = no;
         if (a.p_attr) a.p_attr[T.v0 + k] = na;
@@ -706,8 +708,8 @@ __device__ int fire_min(const Relax2D<R>& a, const Tissue2DCtx<R>& T, const Lds2
     double P = 0.0, F2 = 0.0, v2 = 0.0, ui = 0.0, fm = 0.0;
     for (int k = tid; k < T.nvt; k += NT) {
       R fx, fy;
-      int no = 0, na = 0;
-      force_vertex(a.g, T, l, k, fx, fy, &no, &na, (double*)nullptr, u_clear >= 0.0 ? &ui : (double*)nullptr);
+      int no = 0;
+      force_vertex(a.g, T, l, k, fx, fy, &no, (int*)nullptr, (double*)nullptr, u_clear >= 0.0 ? &ui : (double*)nullptr);
       a.fx[T.v0 + k] = fx;
       a.fy[T.v0 + k] = fy;
       const double vx = (double)a.vx[T.v0 + k], vy = (double)a.vy[T.v0 + k];
@@ -768,8 +770,8 @@ __device__ void energy_pass(const Relax2D<R>& a, const Tissue2DCtx<R>& T, const 
   double u = 0.0, ui = 0.0, unused = 0.0, fm = 0.0;
   for (int k = threadIdx.x; k < T.nvt; k += blockDim.x) {
     R fx, fy;
-    int no = 0, na = 0;
-    force_vertex(a.g, T, l, k, fx, fy, &no, &na, &u, &ui);
+    int no = 0;
+    force_vertex(a.g, T, l, k, fx, fy, &no, (int*)nullptr, &u, &ui);
     a.fx[T.v0 + k] = fx;
     a.fy[T.v0 + k] = fy;
     fm = fmax(fm, (double)fx * fx + (double)fy * fy);
@@ -824,8 +826,8 @@ __device__ void k2d_relax_body(const Relax2D<R>& a) {
       prepare_forces(g, T, l);
       for (int k = tid; k < T.nvt; k += NT) {
         R fx, fy;
-        int no = 0, na = 0;
-        force_vertex(g, T, l, k, fx, fy, &no, &na, (double*)nullptr, (double*)nullptr);
+        int no = 0;
+        force_vertex(g, T, l, k, fx, fy, &no, (int*)nullptr, (double*)nullptr, (double*)nullptr);
         a.fx[T.v0 + k] = fx;
         a.fy[T.v0 + k] = fy;
       }
