@@ -310,6 +310,21 @@ __device__ __forceinline__ bool crossing_odd(const Lds2D<R>& l, int cj, R px, R 
   return odd;
 }
 
+// A point outside cell cj's raw bounding box by more than 1e-5 (1 + L),
+// whose raw differences to every vertex of cj stay within L (so crossing_odd
+// applies no image), has even crossing parity: above or below the box no
+// edge straddles its row; left of the box every crossing lies to its right
+// and none counts; right of the box every one counts, and a closed ring has
+// an even number of edges straddling any row.
+template <typename R>
+__device__ __forceinline__ bool outside_box_even(const Lds2D<R>& l, int cj, R px, R py, int pbc, R L) {
+  const R x0 = l.ax0[cj], x1 = l.ax1[cj], y0 = l.ay0[cj], y1 = l.ay1[cj];
+  if (pbc && !(fabs(px - x0) <= L && fabs(px - x1) <= L && fabs(py - y0) <= L && fabs(py - y1) <= L))
+    return false;
+  const R m = R(1e-5f) * (R(1) + L);
+  return px < x0 - m || px > x1 + m || py < y0 - m || py > y1 + m;
+}
+
 // All force terms on vertex k (tissue-local index).  With `energy`, also
 // accumulates this vertex's share of the conservative energy (perimeter edge
 // k -> k+1, bending at k, the cell's area energy at its first vertex) and
@@ -450,7 +465,7 @@ __device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DC
     bool over = false;
     for (int q = 0; q < ncl && !over; q++) {
       const int cj = scan_all ? q : (int)l.cand[ci * DPM2D_MAXCAND + q];
-      if (cj == ci) continue;
+      if (cj == ci || outside_box_even(l, cj, px, py, T.pbc, L)) continue;
       over = crossing_odd(l, cj, px, py, T.pbc, L);
     }
     if (over) {
