@@ -287,6 +287,35 @@ def test_3d_batch_equals_individual(kernel):
         off += n
 
 
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_3d_single_cell_tissue(kernel):
+    # one cell in its box: no candidate cells, so no contact or adhesion pair,
+    # only shape, substrate and bending forces; alone and batched beside a
+    # 3 x 3 tissue (the batched step gives it the same bits as its own run)
+    m = W.monolayer3d(1, 2.4, seed=7)
+    t = tissue3d(m)
+    t.set_step_kernel(kernel)
+    t.set_positions(m.X)
+    g = t.probe()
+    r32 = ref3d(m, m.X, capi.DPM3D_ALL, 32, pairs=True)
+    assert_forces(g["forces"], ref3d(m, m.X, capi.DPM3D_ALL, 64)["forces"], f"3d single cell kernel={kernel}")
+    np.testing.assert_array_equal(g["down"], r32["down"])
+    assert int(g["contact"].sum()) == 0 and int(g["adhesion"].sum()) == 0
+    t.euler(30, 0.002)
+    X1 = t.positions()
+    assert np.isfinite(X1).all()
+    ms = [m, W.monolayer3d(3, 1.8, seed=8)]
+    cb = np.concatenate([[0], np.cumsum([q.C for q in ms])])
+    cat = lambda f: np.concatenate([np.broadcast_to(np.asarray(f(q), np.float32), (q.C,)) for q in ms])
+    b = capi.Tissues3D(m.mesh.faces, cb, cat(lambda q: q.Kv), cat(lambda q: q.Ka), cat(lambda q: q.Ks),
+                       cat(lambda q: q.v0), cat(lambda q: q.a0), [q.L for q in ms], [q.Kre for q in ms],
+                       Kat=[q.Kat for q in ms], Kb=cat(lambda q: q.Kb), mask=capi.DPM3D_ALL, nv=m.mesh.nv)
+    b.set_step_kernel(kernel)
+    b.set_positions(np.concatenate([q.X for q in ms]))
+    b.euler(30, 0.002)
+    np.testing.assert_array_equal(b.positions()[:m.mesh.nv], X1)
+
+
 def test_3d_open_mesh_takes_the_per_corner_path():
     # a mesh with a face removed is not closed: the wave form's ring-edge
     # sum does not apply and it evaluates every (face, corner) term, with
@@ -467,6 +496,22 @@ def test_2d_batch_equals_individual():
         t.euler(200, 0.005)
         np.testing.assert_array_equal(Xb[off:off + len(p.X)], t.positions())
         off += len(p.X)
+
+
+def test_2d_single_cell_tissue():
+    # one cell (its vertices jittered by 5 % of l0, away from the rest
+    # shape): no overlap or attraction partner, shape forces only
+    p = W.tissue2d(1, 24, 1.4, 0.6, Kb=0.01, Kat=0.5, seed=3)
+    rng = np.random.default_rng(5)
+    X = (p.X + rng.normal(0, 0.05 * float(np.mean(p.l0)), p.X.shape)).astype(np.float32)
+    t = tissue2d_handle(p)
+    t.set_positions(X)
+    g = t.probe()
+    F64, _, _ = ref2d(p, X, 64)
+    assert_forces(g["forces"], F64, "2d single cell")
+    assert int(g["overlap"].sum()) == 0 and int(g["attraction"].sum()) == 0
+    t.euler(100, 0.005)
+    assert np.isfinite(t.positions()).all()
 
 
 def test_2d_dropin_class_matches_c_abi():
