@@ -325,14 +325,38 @@ __device__ __forceinline__ bool outside_box_even(const Lds2D<R>& l, int cj, R px
   return px < x0 - m || px > x1 + m || py < y0 - m || py > y1 + m;
 }
 
+// Crossing parity of every vertex against every candidate cell, one
+// (vertex, candidate) pair per thread and iteration, so that a few vertices
+// with many wrapped candidates spread over the whole workgroup: bit k of
+// `bits` (cleared beforehand) is set when vertex k lies inside some
+// candidate by the odd-crossing rule (Cell2D_kernel.cl:165-202).  That is
+// the OR force_vertex forms over the same candidates, so the two agree bit
+// for bit.  Pairs q = w / nvt beyond a cell's candidate count do nothing.
+template <typename R>
+__device__ void over_pass(const Tissue2DCtx<R>& T, const Lds2D<R>& l, unsigned* bits) {
+  const int nvt = T.nvt, nc = T.nc;
+  for (int w = threadIdx.x; w < nvt * nc; w += blockDim.x) {
+    const int k = w % nvt, q = w / nvt;
+    const int ci = cell_of(l.vb, nc, k);
+    const int ncd = l.ncand[ci];
+    const bool scan_all = ncd > DPM2D_MAXCAND;
+    if (q >= (scan_all ? nc : ncd)) continue;
+    const int cj = scan_all ? q : (int)l.cand[ci * DPM2D_MAXCAND + q];
+    const R px = l.x[k], py = l.y[k];
+    if (cj == ci || outside_box_even(l, cj, px, py, T.pbc, T.L)) continue;
+    if (crossing_odd(l, cj, px, py, T.pbc, T.L)) atomicOr(&bits[k >> 5], 1u << (k & 31));
+  }
+}
+
 // All force terms on vertex k (tissue-local index).  With `energy`, also
 // accumulates this vertex's share of the conservative energy (perimeter edge
 // k -> k+1, bending at k, the cell's area energy at its first vertex) and
-// with `e_int` half of each vertex-repulsion pair energy.
+// with `e_int` half of each vertex-repulsion pair energy.  With `over_bits`
+// the crossing parity comes from over_pass instead of the loop here.
 template <typename R>
 __device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Lds2D<R>& l,
                                           int k, R& Fx, R& Fy, int* n_over, int* n_attr, double* energy,
-                                          double* e_int) {
+                                          double* e_int, const unsigned* over_bits = nullptr) {
   const int ci = cell_of(l.vb, T.nc, k);
   const int gc = T.c0 + ci;
   const int b = l.vb[ci], e = l.vb[ci + 1], n = e - b, vi = k - b;
@@ -462,8 +486,8 @@ __device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DC
     }
   }
   if (mask & M2_REPULSION) {
-    bool over = false;
-    for (int q = 0; q < ncl && !over; q++) {
+    bool over = over_bits && ((over_bits[k >> 5] >> (k & 31)) & 1u);
+    for (int q = 0; q < ncl && !over && !over_bits; q++) {
       const int cj = scan_all ? q : (int)l.cand[ci * DPM2D_MAXCAND + q];
       if (cj == ci || outside_box_even(l, cj, px, py, T.pbc, L)) continue;
       over = crossing_odd(l, cj, px, py, T.pbc, L);
@@ -527,12 +551,25 @@ extern "C" __global__ void __launch_bounds__(1024) k2d_run(const Run2D a) {
   const int nsteps = a.nsteps;
   const int nevals = nsteps > 0 ? nsteps : 1;
   unsigned long long n_over_total = 0;
+  // Crossing parity in its own pass when the (vertex, candidate) pairs are at
+  // most 16 per thread and one bit per vertex fits the candidate-bit words
+  // (nc * DPM2D_CSPLIT of them, unused between the candidate and the next
+  // step's cell pass).
+  const bool split_over = (g.mask & M2_REPULSION) && T.nvt <= 32 * DPM2D_CSPLIT * T.nc &&
+                          T.nvt * T.nc <= 16 * NT;
+  unsigned* obits = l.cbits;
   float rfx[DPM2D_VPT], rfy[DPM2D_VPT];
   for (int s = 0; s < nevals; s++) {
     cell_pass(g, T, l);
     __syncthreads();
     cand_pass(g, T, l);
     __syncthreads();
+    if (split_over) {  // block-uniform; the candidate bits are dead here
+      for (int i = tid; i < (T.nvt + 31) / 32; i += NT) obits[i] = 0u;
+      __syncthreads();
+      over_pass(T, l, obits);
+      __syncthreads();
+    }
 #pragma unroll
     for (int r = 0; r < DPM2D_VPT; r++) {
       const int k = tid + r * NT;
@@ -540,7 +577,8 @@ extern "C" __global__ void __launch_bounds__(1024) k2d_run(const Run2D a) {
       rfy[r] = 0.0f;
       if (k < T.nvt) {
         int no = 0, na = 0;
-        force_vertex(g, T, l, k, rfx[r], rfy[r], &no, a.p_attr ? &na : nullptr, (double*)nullptr, (double*)nullptr);
+        force_vertex(g, T, l, k, rfx[r], rfy[r], &no, a.p_attr ? &na : nullptr, (double*)nullptr, (double*)nullptr,
+                     split_over ? obits : nullptr);
         n_over_total += no;
         if (a.p_overlap) a.p_overlap[T.v0 + k] = no;
         if (a.p_attr) a.p_attr[T.v0 + k] = na;
