@@ -1354,9 +1354,9 @@ static int create2d(const dpm2d_desc* d, dpm_tissue** out, int rs) {
   h->T = T;
   h->C = C;
   h->V = cvb[C];
-  // two threads per vertex (up to 1024): the crossing-parity pass of k2d_run
+  // four threads per vertex (up to 1024): the crossing-parity pass of k2d_run
   // spreads (vertex, candidate) pairs over all of them
-  h->NT2 = std::min(1024, std::max(64, (2 * maxv + 63) / 64 * 64));
+  h->NT2 = std::min(1024, std::max(64, (4 * maxv + 63) / 64 * 64));
   h->NTr = std::min(1024, std::max(64, (maxv / 4 + 63) / 64 * 64));
   if (rs == 4 && maxv > DPM2D_VPT * h->NT2) {
     delete h;
