@@ -138,6 +138,17 @@ int dpm3d_cell_stats(dpm_tissue* h, double* volume, float* com);
 int dpm3d_probe(dpm_tissue* h, float* forces, unsigned char* contact,
                 float* winding, unsigned char* down, int* adhesion);
 
+/* Diagnostic: the adhesion pair sets of one force evaluation at the current
+ * positions (as dpm3d_probe, which gives their sizes).  A partner vertex is
+ * identified by the bit patterns of its float coordinates,
+ * key = f(x ^ f(y ^ f(z))) with f the 32-bit finaliser of MurmurHash3;
+ * per vertex gv, own_keys[gv] is the wrapping 32-bit sum of the keys of the
+ * partners its own loop accepts (Cell3D_Kernel.cl:341-353, |d| < 2 l0 of
+ * gv's cell) and rev_keys[gv] that of the vertices whose own loop accepts gv
+ * (the pairs whose reaction lands on gv, Cell3D_Kernel.cl:359).  Either may
+ * be NULL. */
+int dpm3d_probe_adhesion(dpm_tissue* h, unsigned* own_keys, unsigned* rev_keys);
+
 /* Full double precision.  dpm3d_create_f64 makes a handle whose positions,
  * forces and per-cell statistics are double on the device and whose step
  * evaluates every formula in double (the float thresholds of the reference
@@ -256,6 +267,12 @@ int dpm2d_set_tissue_params(dpm_tissue* h, const float* L, const int* pbc,
  * = number of attraction partners (Cell2D_kernel.cl:261).  On a double
  * tissue only the forces are available (overlap, attraction must be NULL). */
 int dpm2d_probe(dpm_tissue* h, float* forces, int* overlap, int* attraction);
+
+/* Diagnostic: per vertex, the wrapping 32-bit sum of the keys f(x ^ f(y))
+ * of its attraction partners (Cell2D_kernel.cl:250-266, d < l0 of its cell),
+ * the partner identified by the bit patterns of its coordinates (f as in
+ * dpm3d_probe_adhesion).  Float tissues. */
+int dpm2d_probe_attraction(dpm_tissue* h, unsigned* keys);
 
 /* Double-precision 2D tissue (positions, forces, velocities, box in double;
  * the same formulas), as dpm3d_create_f64.  Its Euler steps, FIRE and

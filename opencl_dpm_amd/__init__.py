@@ -9,6 +9,10 @@ this directory on sys.path (``import clDPM``), or use
 `opencl_dpm_amd.capi` binds the C ABI (include/dpm.h) directly for batched
 tissues and timing.  Both load libdpm_hip.so, the HIP implementation; there
 is no CPU fallback.
+
+The module is loaded on first access of ``opencl_dpm_amd.clDPM`` so that the
+build recipe (``opencl_dpm_amd.build``) can be imported before anything is
+built.
 """
 from __future__ import annotations
 
@@ -29,6 +33,12 @@ def _load_module():
             "clDPM extension is not built in-tree; run `python -m opencl_dpm_amd.build`") from e
 
 
-clDPM = _load_module()
+def __getattr__(name):
+    if name == "clDPM":
+        mod = _load_module()
+        globals()["clDPM"] = mod
+        return mod
+    raise AttributeError(f"module 'opencl_dpm_amd' has no attribute {name!r}")
+
 
 __all__ = ["clDPM"]

@@ -33,6 +33,7 @@ EXPORTS = [
     "dpm_last_error", "dpm_version", "dpm_device_count",
     "dpm3d_create", "dpm3d_set_positions", "dpm3d_get_positions", "dpm3d_get_forces",
     "dpm3d_set_cell_params", "dpm3d_set_tissue_params", "dpm3d_cell_stats", "dpm3d_probe",
+    "dpm3d_probe_adhesion", "dpm2d_probe_attraction",
     "dpm3d_create_f64", "dpm3d_set_positions_f64", "dpm3d_get_positions_f64",
     "dpm3d_get_forces_f64", "dpm3d_energy", "dpm3d_trajectory", "dpm3d_adjoint", "dpm3d_adjoint_f64",
     "dpm3d_set_compensated", "dpm3d_set_cell_list", "dpm3d_set_step_kernel", "dpm3d_autotune",
@@ -107,6 +108,8 @@ def load() -> ctypes.CDLL:
     lib.dpm3d_cell_stats.argtypes = [vp, vp, vp]
     lib.dpm3d_probe.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.dpm2d_probe.argtypes = [vp, vp, vp, vp]
+    lib.dpm3d_probe_adhesion.argtypes = [vp, vp, vp]
+    lib.dpm2d_probe_attraction.argtypes = [vp, vp]
     _lib = lib
     return lib
 
@@ -375,7 +378,12 @@ class Tissues3D(_Handle):
         down = np.zeros((self.C, self.nf), np.uint8)
         adh = np.zeros(self.nverts, np.int32)
         _check(load().dpm3d_probe(self._h, None, _p(ct), _p(w), _p(down), _p(adh)))
-        return {"forces": self.forces(), "contact": ct, "winding": w, "down": down, "adhesion": adh}
+        out = {"forces": self.forces(), "contact": ct, "winding": w, "down": down, "adhesion": adh}
+        own = np.zeros(self.nverts, np.uint32)
+        rev = np.zeros(self.nverts, np.uint32)
+        _check(load().dpm3d_probe_adhesion(self._h, _p(own), _p(rev)))
+        out["adhesion_hash"] = np.stack([own, rev], 1)
+        return out
 
 
 class Tissues2D(_Handle):
@@ -489,4 +497,6 @@ class Tissues2D(_Handle):
         ov = np.zeros(self.V, np.int32)
         at = np.zeros(self.V, np.int32)
         _check(load().dpm2d_probe(self._h, _p(f), _p(ov), _p(at)))
-        return {"forces": f, "overlap": ov, "attraction": at}
+        keys = np.zeros(self.V, np.uint32)
+        _check(load().dpm2d_probe_attraction(self._h, _p(keys)))
+        return {"forces": f, "overlap": ov, "attraction": at, "attraction_hash": keys}

@@ -356,7 +356,8 @@ __device__ void over_pass(const Tissue2DCtx<R>& T, const Lds2D<R>& l, unsigned* 
 template <typename R>
 __device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Lds2D<R>& l,
                                           int k, R& Fx, R& Fy, int* n_over, int* n_attr, double* energy,
-                                          double* e_int, const unsigned* over_bits = nullptr) {
+                                          double* e_int, const unsigned* over_bits = nullptr,
+                                          unsigned* attr_key = nullptr) {
   const int ci = cell_of(l.vb, T.nc, k);
   const int gc = T.c0 + ci;
   const int b = l.vb[ci], e = l.vb[ci + 1], n = e - b, vi = k - b;
@@ -455,6 +456,7 @@ __device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DC
           fx += (R)(kk * ddx);
           fy += (R)(kk * ddy);
           cnt++;
+          if (attr_key) atomicAdd(attr_key, vertex_key(l.x[j], l.y[j]));
         }
       }
     }
@@ -535,6 +537,7 @@ struct Run2D {
   unsigned long long* counters;  // [4 * DPM_COUNTER_SLOTS]
   int* p_overlap;                // diagnostics, per vertex (nullptr normally)
   int* p_attr;
+  unsigned* p_attr_key;  // attraction partner-set key sums (vertex_key)
 };
 
 // Forces stay in registers: up to DPM2D_VPT vertices per thread.
@@ -578,7 +581,7 @@ extern "C" __global__ void __launch_bounds__(1024) k2d_run(const Run2D a) {
       if (k < T.nvt) {
         int no = 0, na = 0;
         force_vertex(g, T, l, k, rfx[r], rfy[r], &no, a.p_attr ? &na : nullptr, (double*)nullptr, (double*)nullptr,
-                     split_over ? obits : nullptr);
+                     split_over ? obits : nullptr, a.p_attr_key ? a.p_attr_key + T.v0 + k : nullptr);
         n_over_total += no;
         if (a.p_overlap) a.p_overlap[T.v0 + k] = no;
         if (a.p_attr) a.p_attr[T.v0 + k] = na;

@@ -107,6 +107,9 @@ struct Step3D {
   float* p_winding;
   unsigned char* p_down;
   int* p_adh;
+  // per vertex: key sums (vertex_key) of the adhesion partners its own loop
+  // accepts, then of the partners whose loop accepts it
+  unsigned* p_adh_hash;
   int ncmax;
   double* energy;  // k3d_energy output, per cell
   R* w_out;        // per vertex: sum of |w| over its contact partners (nullptr: not recorded)
@@ -599,7 +602,7 @@ __device__ __forceinline__ R winding_wave(const Lds3D<R>& l, Q4<R>* ut, int nv, 
 // evaluation for diagnostics).  blockDim.x >= nv, a multiple of 64.
 // NVC, NFC, NTC, WWC: vertex and face counts, threads per workgroup and
 // winding waves fixed at compile time (0: taken from the arguments).
-template <typename R, int NVC, int NFC, int NTC, int WWC>
+template <typename R, int NVC, int NFC, int NTC, int WWC, bool PROBE = true>
 __device__ void k3d_step_body(const Step3D<R>& a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nv = NVC > 0 ? NVC : a.nv, nf = NFC > 0 ? NFC : a.nf, NT = NTC > 0 ? NTC : (int)blockDim.x, tid = threadIdx.x;
@@ -674,7 +677,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
             face_of(l, f, i0, i1, i2);
             const V3<R> P0 = lds_v(l.sx, l.sy, l.sz, i0);
             dn = facing_down(vcross(vsub(lds_v(l.sx, l.sy, l.sz, i1), P0), vsub(lds_v(l.sx, l.sy, l.sz, i2), P0)));
-            if (dn && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
+            if (PROBE && dn && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
           }
           const unsigned long long m = __ballot(dn);
           if (lane == 0) {
@@ -748,7 +751,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
       if (sub_on) {  // Cell3D_Kernel.cl:206-246
         const V3<R> un = vnormalize(vcross(vsub(P1, P0), vsub(P2, P0)));
         if (un.z < R(-0.1f)) {
-          if (a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
+          if (PROBE && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
           c0 = vadd(c0, substrate_corner(P0, Ci, Ksi, ksg, L, l0i));
           c1 = vadd(c1, substrate_corner(P1, Ci, Ksi, ksg, L, l0i));
           c2 = vadd(c2, substrate_corner(P2, Ci, Ksi, ksg, L, l0i));
@@ -999,7 +1002,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
           if (need) {
             const R wv = l.wb[tid];
             const size_t pidx = (base + tid) * (size_t)a.ncmax + (size_t)(cj2 - cb);
-            if (a.p_winding) a.p_winding[pidx] = (float)wv;
+            if (PROBE && a.p_winding) a.p_winding[pidx] = (float)wv;
             if (!(fabs(wv) < R(1e-6f))) {
               const V3<R> dir = vnormalize(vsub(Ci, P));
               const R mag = fabs(wv) * R(0.5f) * Kre;
@@ -1019,7 +1022,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
                 }
                 npart++;
               }
-              if (a.p_contact) a.p_contact[pidx] = 1;
+              if (PROBE && a.p_contact) a.p_contact[pidx] = 1;
             }
           }
         }
@@ -1087,13 +1090,17 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
                   if (ds < L0i * R(2) && ds > R(1e-12f)) {
                     Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i, kh, inv_l0i));
                     cnt_own++;
+                    if (PROBE && a.p_adh_hash) atomicAdd(&a.p_adh_hash[2 * (base + il[iv])], vertex_key(pj.x, pj.y, pj.z));
                   }
                 }
                 const R d2r = vdot(dr, dr);
                 if (d2r < lim_rev) {
                   const R dsr = sqrt(d2r);
-                  if (dsr < L0j * R(2) && dsr > R(1e-12f))
+                  if (dsr < L0j * R(2) && dsr > R(1e-12f)) {
                     Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j, kh, inv_l0j));
+                    if (PROBE && a.p_adh_hash)
+                      atomicAdd(&a.p_adh_hash[2 * (base + il[iv]) + 1], vertex_key(pj.x, pj.y, pj.z));
+                  }
                 }
               }
               acc[sl * nmine + iv] = Q4<R>{Fa.x, Fa.y, Fa.z, (R)cnt_own};
@@ -1106,7 +1113,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
                 F = vadd(F, V3<R>{q4.x, q4.y, q4.z});
                 cnt_own += (int)q4.w;
               }
-              if (a.p_adh) a.p_adh[base + tid] += cnt_own;
+              if (PROBE && a.p_adh) a.p_adh[base + tid] += cnt_own;
             }
           }
         }
@@ -1633,7 +1640,13 @@ extern "C" __global__ void __launch_bounds__(1024) k3d_step_f64(const Step3D<dou
 // offsets are immediates and the register budget is that of 256 or 704
 // threads instead of 1024 (the general kernel spills at 128 registers).
 // The host takes them where the handle's nv, nf, NT and ww match.
-extern "C" __global__ void __launch_bounds__(256) k3d_step_f32_v162(const Step3D<float> a) { k3d_step_body<float, 162, 320, 256, 4>(a); }
+extern "C" __global__ void __launch_bounds__(256) k3d_step_f32_v162(const Step3D<float> a) {
+  k3d_step_body<float, 162, 320, 256, 4, false>(a);
+}
+// the same with the diagnostic outputs (dpm3d_probe*) compiled in
+extern "C" __global__ void __launch_bounds__(256) k3d_step_f32_v162_probe(const Step3D<float> a) {
+  k3d_step_body<float, 162, 320, 256, 4, true>(a);
+}
 extern "C" __global__ void __launch_bounds__(704) k3d_step_f32_v642(const Step3D<float> a) { k3d_step_body<float, 642, 1280, 704, 4>(a); }
 extern "C" __global__ void __launch_bounds__(704) k3d_step_f64_v642(const Step3D<double> a) { k3d_step_body<double, 642, 1280, 704, 2>(a); }
 extern "C" __global__ void __launch_bounds__(256) k3d_step_f64_v162(const Step3D<double> a) { k3d_step_body<double, 162, 320, 256, 4>(a); }

@@ -161,7 +161,7 @@ __device__ __forceinline__ R winding_w(const R* nx, const R* ny, const R* nz, co
 // immediate (the 162-vertex icosphere).  The vertex and face loops stay
 // rolled: unrolled, their iterations' registers spill to scratch memory
 // (208 B per lane, 6x the step's HBM traffic, at the same speed).
-template <typename R, int NVC, int NFC>
+template <typename R, int NVC, int NFC, bool PROBE = true>
 __device__ void k3d_wstep_body(const Step3D<R>& a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nv = NVC > 0 ? NVC : a.nv, nf = NFC > 0 ? NFC : a.nf;
@@ -244,7 +244,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         const V3<R> l0v = vsub(P1, P0), l2v = vsub(P0, P2);
         if (sub_on) {
           dn = facing_down(vcross(l0v, vsub(P2, P0)));
-          if (dn && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
+          if (PROBE && dn && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
         }
         const V3<R> l1v = vsub(P2, P1);
         e2min = fmin(e2min, fmin(vdot(l0v, l0v), fmin(vdot(l1v, l1v), vdot(l2v, l2v))));
@@ -623,11 +623,16 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
                 if (dsq < L0i * R(2) && dsq > R(1e-12f)) {
                   Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i, kh, inv_l0i));
                   cnt_own++;
+                  if (PROBE && a.p_adh_hash) atomicAdd(&a.p_adh_hash[2 * (base + il[iv])], vertex_key(pj.x, pj.y, pj.z));
                 }
               }
               if (d2r < lim_rev) {
                 const R dsr = sqrt(d2r);
-                if (dsr < L0j * R(2) && dsr > R(1e-12f)) Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j, kh, inv_l0j));
+                if (dsr < L0j * R(2) && dsr > R(1e-12f)) {
+                  Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j, kh, inv_l0j));
+                  if (PROBE && a.p_adh_hash)
+                    atomicAdd(&a.p_adh_hash[2 * (base + il[iv]) + 1], vertex_key(pj.x, pj.y, pj.z));
+                }
               }
             }
             ut[sl * nmine + iv] = Q4<R>{Fa.x, Fa.y, Fa.z, (R)cnt_own};
@@ -645,7 +650,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
             fx[v] = F.x;
             fy[v] = F.y;
             fz[v] = F.z;
-            if (a.p_adh) a.p_adh[base + v] += cnt_own;
+            if (PROBE && a.p_adh) a.p_adh[base + v] += cnt_own;
           }
         }
         // Contact: a point at least m_j from every face of cell j has winding
@@ -681,7 +686,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
             }
             if (need) {
               const size_t pidx = (base + v) * (size_t)a.ncmax + (size_t)(cj2 - cb);
-              if (a.p_winding) a.p_winding[pidx] = (float)wk;
+              if (PROBE && a.p_winding) a.p_winding[pidx] = (float)wk;
               if (!(fabs(wk) < R(1e-6f))) {
                 const V3<R> dir = vnormalize(vsub(Ci, P));
                 const R mag = fabs(wk) * R(0.5f) * Kre;
@@ -706,7 +711,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
                     atomicOr(a.rec_overflow, 1);
                   }
                 }
-                if (a.p_contact) a.p_contact[pidx] = 1;
+                if (PROBE && a.p_contact) a.p_contact[pidx] = 1;
               }
             }
           }
@@ -803,8 +808,13 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
 extern "C" __global__ void __launch_bounds__(256, 4) k3d_wstep_f32(const Step3D<float> a) {
   k3d_wstep_body<float, 0, 0>(a);
 }
-// the icosphere of subdivision 2 (162 vertices, 320 faces)
+// the icosphere of subdivision 2 (162 vertices, 320 faces); the production
+// instance has the diagnostic outputs compiled out (their registers would
+// spill), the _probe instance of the same source records them
 extern "C" __global__ void __launch_bounds__(256, 4) k3d_wstep_f32_v162(const Step3D<float> a) {
-  k3d_wstep_body<float, 162, 320>(a);
+  k3d_wstep_body<float, 162, 320, false>(a);
+}
+extern "C" __global__ void __launch_bounds__(256, 4) k3d_wstep_f32_v162_probe(const Step3D<float> a) {
+  k3d_wstep_body<float, 162, 320, true>(a);
 }
 extern "C" __global__ void __launch_bounds__(256) k3d_wstep_f64(const Step3D<double> a) { k3d_wstep_body<double, 0, 0>(a); }

@@ -259,8 +259,9 @@ static bool all_finite(const R* p, size_t n) {
 // The wave-per-cell step kernel of a precision.
 // The wave form's kernel for a handle: the compile-time-sized instance for
 // float 162-vertex, 320-face meshes, else the general one.
-static const void* wstep_kernel(int rs, int nv, int nf) {
-  if (rs == 4 && nv == 162 && nf == 320) return (const void*)k3d_wstep_f32_v162;
+static const void* wstep_kernel(int rs, int nv, int nf, bool probe = false) {
+  if (rs == 4 && nv == 162 && nf == 320) return probe ? (const void*)k3d_wstep_f32_v162_probe
+                                                      : (const void*)k3d_wstep_f32_v162;
   return rs == 8 ? (const void*)k3d_wstep_f64 : (const void*)k3d_wstep_f32;
 }
 
@@ -475,6 +476,8 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
   for (const void* k : kern[rs == 8 ? 1 : 0])
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
   (void)hipFuncSetAttribute((const void*)k3d_step_f32_v162, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
+  (void)hipFuncSetAttribute((const void*)k3d_step_f32_v162_probe, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)h->lds);
   (void)hipFuncSetAttribute((const void*)k3d_step_f32_v642, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
   (void)hipFuncSetAttribute((const void*)k3d_step_f64_v642, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
   (void)hipFuncSetAttribute((const void*)k3d_step_f64_v162, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds);
@@ -513,7 +516,9 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
         return fail(DPM_EDEVICE, "topology upload failed");
       }
       h->wtopo = q;
-      (void)hipFuncSetAttribute(wstep_kernel(rs, nv, nf), hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_w);
+      for (bool probe : {false, true})
+        (void)hipFuncSetAttribute(wstep_kernel(rs, nv, nf, probe), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)h->lds_w);
     }
   }
   h->lds_vjp = ldsvjp_layout(nv, nf, rs).total;
@@ -796,8 +801,11 @@ static int launch3d(dpm_tissue* h, Kind3D kind, int in, float dt, int flags, Edi
       k = kind == K_STEP ? k3d_step_f64 : kind == K_STATS ? k3d_stats_f64 : k3d_energy_f64;
     else
       k = kind == K_STEP ? k3d_step_f32 : kind == K_STATS ? k3d_stats_f32 : k3d_energy_f32;
+    // diagnostic outputs requested: the instances that record them
+    const bool probe = a.p_contact || a.p_winding || a.p_down || a.p_adh || a.p_adh_hash;
     if constexpr (sizeof(R) == 4)
-      if (kind == K_STEP && h->ww == 4 && h->nv == 162 && h->nf == 320 && h->NT == 256) k = k3d_step_f32_v162;
+      if (kind == K_STEP && h->ww == 4 && h->nv == 162 && h->nf == 320 && h->NT == 256)
+        k = probe ? k3d_step_f32_v162_probe : k3d_step_f32_v162;
     if constexpr (sizeof(R) == 4)
       if (kind == K_STEP && h->ww == 4 && h->nv == 642 && h->nf == 1280 && h->NT == 704) k = k3d_step_f32_v642;
     if constexpr (sizeof(R) == 8)
@@ -827,7 +835,7 @@ static int launch3d(dpm_tissue* h, Kind3D kind, int in, float dt, int flags, Edi
     }
     if (kind == K_STEP && h->wave_cpw &&
         (h->step_kernel == 2 || (h->step_kernel == 0 && h->C >= h->wave_min_cells))) {
-      auto kw = reinterpret_cast<void (*)(Step3D<R>)>(const_cast<void*>(wstep_kernel(h->rs, h->nv, h->nf)));
+      auto kw = reinterpret_cast<void (*)(Step3D<R>)>(const_cast<void*>(wstep_kernel(h->rs, h->nv, h->nf, probe)));
       hipLaunchKernelGGL(kw, dim3((h->C + h->wave_cpw - 1) / h->wave_cpw), dim3(64 * h->wave_cpw), h->lds_w,
                          h->stream, a);
     } else {
@@ -922,7 +930,9 @@ static int set_wave_cpw(dpm_tissue* h, int cpw) {
   if (lds > 65536) return fail(DPM_ERANGE, "cells per workgroup exceed the LDS of a workgroup");
   h->wave_cpw = cpw;
   h->lds_w = lds;
-  HIPCHECK(hipFuncSetAttribute(wstep_kernel(h->rs, h->nv, h->nf), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  for (bool probe : {false, true})
+    HIPCHECK(hipFuncSetAttribute(wstep_kernel(h->rs, h->nv, h->nf, probe), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds));
   if (h->graph) {
     (void)hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
@@ -1095,6 +1105,35 @@ extern "C" int dpm3d_probe(dpm_tissue* h, float* forces, unsigned char* contact,
   if (rc) return rc;
   h->forces_valid = true;
   if (forces) TRY(dpm3d_get_forces(h, forces, NVT));
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_probe_adhesion(dpm_tissue* h, unsigned* own_keys, unsigned* rev_keys) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  if (!own_keys && !rev_keys) return DPM_OK;
+  TRY(ensure_stats(h));
+  const size_t NVT = (size_t)h->C * h->nv;
+  unsigned* dk = nullptr;
+  hipStream_t s = h->stream;
+  int rc = dev_alloc(&dk, 2 * NVT);
+  if (!rc) {
+    (void)hipMemsetAsync(dk, 0, 2 * NVT * sizeof(unsigned), s);
+    rc = launch3d(h, K_STEP, h->cur, 0.0f, F_WRITE_FORCES, [&](auto& a) {
+      a.counters = nullptr;
+      a.p_adh_hash = dk;
+    });
+  }
+  std::vector<unsigned> k(rc ? 0 : 2 * NVT);
+  if (!rc)
+    rc = hipMemcpyAsync(k.data(), dk, 2 * NVT * sizeof(unsigned), hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(DPM_EDEVICE, "stream sync failed");
+  if (dk) (void)hipFree(dk);
+  if (rc) return rc;
+  h->forces_valid = true;
+  for (size_t i = 0; i < NVT; i++) {
+    if (own_keys) own_keys[i] = k[2 * i];
+    if (rev_keys) rev_keys[i] = k[2 * i + 1];
+  }
   return DPM_OK;
 }
 
@@ -1851,6 +1890,29 @@ extern "C" int dpm2d_probe(dpm_tissue* h, float* forces, int* overlap, int* attr
   if (rc) return rc;
   if (forces) TRY(dpm2d_get_forces(h, forces, h->V));
   return DPM_OK;
+}
+
+extern "C" int dpm2d_probe_attraction(dpm_tissue* h, unsigned* keys) {
+  if (!h || h->dim != 2) return fail(DPM_EINVAL, "not a 2D tissue");
+  if (h->rs == 8) return fail(DPM_EINVAL, "pair diagnostics are recorded by float tissues only");
+  if (!keys) return DPM_OK;
+  hipStream_t s = h->stream;
+  unsigned* dk = nullptr;
+  int rc = dev_alloc(&dk, h->V);
+  if (!rc) {
+    (void)hipMemsetAsync(dk, 0, h->V * sizeof(unsigned), s);
+    Run2D a = run2d_args(h, 0, 0.0f);
+    a.counters = nullptr;
+    a.p_attr_key = dk;
+    hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds2, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail(DPM_EDEVICE, std::string("k2d_run: ") + hipGetErrorString(e));
+  }
+  if (!rc)
+    rc = hipMemcpyAsync(keys, dk, h->V * sizeof(unsigned), hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(DPM_EDEVICE, "stream sync failed");
+  if (dk) (void)hipFree(dk);
+  return rc;
 }
 
 // ================================================================ common ==

@@ -74,6 +74,34 @@ __device__ __forceinline__ V3<R> vnormalize(V3<R> a) {
   return {a.x * inv, a.y * inv, a.z * inv};
 }
 
+// ------------------------------------------------------ pair-set fingerprints
+// Diagnostics of the probe entry points: a partner vertex is identified by
+// the bit patterns of its coordinates, key = f(x ^ f(y ^ f(z))) with f the
+// 32-bit finaliser of MurmurHash3 (a bijection of 32-bit words), and a
+// vertex's partner set by the wrapping sum of its partners' keys, which does
+// not depend on the order in which pairs are found (atomics may add them).
+__device__ __forceinline__ unsigned fmix32(unsigned h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+__device__ __forceinline__ unsigned rbits(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ unsigned rbits(double v) {  // both halves of a double
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return fmix32((unsigned)b) ^ (unsigned)(b >> 32);
+}
+template <typename R>
+__device__ __forceinline__ unsigned vertex_key(R x, R y) {
+  return fmix32(rbits(x) ^ fmix32(rbits(y)));
+}
+template <typename R>
+__device__ __forceinline__ unsigned vertex_key(R x, R y, R z) {
+  return fmix32(rbits(x) ^ fmix32(rbits(y) ^ fmix32(rbits(z))));
+}
+
 // ---------------------------------------------------------------- reductions
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, DPM_WAVE);

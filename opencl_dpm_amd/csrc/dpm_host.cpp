@@ -27,6 +27,14 @@ void check(int status) {
   throw std::runtime_error(msg);
 }
 
+// Python may assign Verts and Forces lists of any length; the getters read
+// NV entries (the reference indexes past the end of a shorter list).
+static void require_entries(size_t have, unsigned int nv, const char* what) {
+  if (have < nv)
+    throw std::invalid_argument(std::string(what) + " has " + std::to_string(have) +
+                                " entries, fewer than NV = " + std::to_string(nv));
+}
+
 // ------------------------------------------------------------------ Cell2D
 
 Cell2D::Cell2D(float x0, float y0, float CalA, unsigned int numVerts, float _r0) {
@@ -47,6 +55,7 @@ Cell2D::Cell2D(float x0, float y0, float CalA, unsigned int numVerts, float _r0)
 }
 
 float Cell2D::GetArea() const {
+  require_entries(Verticies.size(), NV, "Verts");
   float area = 0.0f;
   unsigned int j = NV - 1;
   for (unsigned int i = 0; i < NV; i++) {
@@ -61,6 +70,7 @@ float Cell2D::GetArea() const {
 // reads one vertex past the end for its closing edge; this closes the
 // polygon on vertex NV-1 -> 0.)
 float Cell2D::GetPerim() const {
+  require_entries(Verticies.size(), NV, "Verts");
   float dist = 0.0f;
   for (unsigned int i = 0; i < NV; i++) {
     const unsigned int j = (i + 1) % NV;
@@ -156,6 +166,7 @@ unsigned int Cell3D::AddMiddlePoint(unsigned int p1, unsigned int p2) {
 }
 
 float Cell3D::GetVolume() const {
+  require_entries(Verts.size(), NV, "Verts");
   float volume = 0.0f;
   for (const auto& tri : Faces) {
     const auto& p0 = Verts[tri[0]];
@@ -173,6 +184,7 @@ float Cell3D::GetVolume() const {
 // Sum of |cross| over faces: twice the triangulated area, as the reference
 // returns it (cell.cpp:213-230).
 float Cell3D::GetSurfaceArea() const {
+  require_entries(Verts.size(), NV, "Verts");
   float sa = 0.0f;
   for (const auto& tri : Faces) {
     const auto& p0 = Verts[tri[0]];
@@ -189,6 +201,7 @@ float Cell3D::GetSurfaceArea() const {
 }
 
 std::vector<std::vector<float>> Cell3D::GetPositions() const {
+  require_entries(Verts.size(), NV, "Verts");
   std::vector<std::vector<float>> p(3, std::vector<float>(NV));
   for (unsigned int i = 0; i < NV; i++)
     for (int k = 0; k < 3; k++) p[k][i] = Verts[i][k];
@@ -197,6 +210,7 @@ std::vector<std::vector<float>> Cell3D::GetPositions() const {
 
 // Positions wrapped on a cylinder of circumference L (cell.cpp:242-253).
 std::vector<std::vector<float>> Cell3D::GetVesselPositions(float L) const {
+  require_entries(Verts.size(), NV, "Verts");
   std::vector<std::vector<float>> p(3, std::vector<float>(NV));
   const float scale = (2.0 * M_PI) / L;
   const float radius = L / (2 * M_PI);
@@ -210,6 +224,7 @@ std::vector<std::vector<float>> Cell3D::GetVesselPositions(float L) const {
 }
 
 std::vector<std::vector<float>> Cell3D::GetForces() const {
+  require_entries(Forces.size(), NV, "Forces");
   std::vector<std::vector<float>> p(3, std::vector<float>(NV));
   for (unsigned int i = 0; i < NV; i++)
     for (int k = 0; k < 3; k++) p[k][i] = Forces[i][k];
@@ -224,6 +239,7 @@ std::vector<std::array<int, 3>> Cell3D::GetFaces() const {
 }
 
 std::array<float, 3> Cell3D::GetCOM() const {
+  require_entries(Verts.size(), NV, "Verts");
   std::array<float, 3> com = {0.0f, 0.0f, 0.0f};
   for (unsigned int vi = 0; vi < NV; vi++)
     for (int k = 0; k < 3; k++) com[k] += Verts[vi][k];
@@ -312,6 +328,8 @@ void relax_centres(const std::vector<float>& radius, float scale, float L, std::
 }  // namespace
 
 void Tissue2D::Disperse() {
+  NCELLS = (int)cells.size();  // Cells may have been reassigned from Python
+  for (const auto& c : cells) require_entries(c.Verticies.size(), c.NV, "Verts");
   std::vector<float> r(NCELLS), X, Y;
   for (int i = 0; i < NCELLS; i++) r[i] = cells[i].r0;
   relax_centres(r, 1.0f, L, X, Y, false);
@@ -335,6 +353,14 @@ void Tissue2D::CLEulerUpdate(int nsteps, float dt) {
   std::vector<float> Ka(C), Kl(C), Kb(C), a0(C), l0(C), r0(C);
   size_t V = 0;
   for (int c = 0; c < C; c++) {
+    // The reference packs the first NV entries of Verticies and writes back
+    // NV entries of Verticies and Forces (Tissue2D.cpp:137-139, 235-238);
+    // a Verts list assigned shorter than NV from Python is an error here
+    // (the reference reads past its end), a longer one keeps its tail.
+    if (cells[c].Verticies.size() < cells[c].NV)
+      throw std::invalid_argument("cell " + std::to_string(c) + ": Verts has " +
+                                  std::to_string(cells[c].Verticies.size()) +
+                                  " entries, fewer than NV = " + std::to_string(cells[c].NV));
     nv[c] = (int)cells[c].NV;
     Ka[c] = cells[c].Ka;
     Kl[c] = cells[c].Kl;
@@ -375,9 +401,9 @@ void Tissue2D::CLEulerUpdate(int nsteps, float dt) {
   std::vector<float> xy(2 * V);
   size_t k = 0;
   for (const auto& c : cells)
-    for (const auto& v : c.Verticies) {
-      xy[k++] = v[0];
-      xy[k++] = v[1];
+    for (unsigned int vi = 0; vi < c.NV; vi++) {
+      xy[k++] = c.Verticies[vi][0];
+      xy[k++] = c.Verticies[vi][1];
     }
   check(dpm2d_set_positions(h_.get(), xy.data(), V));
   check(dpm_euler_async(h_.get(), nsteps, dt));
@@ -386,11 +412,13 @@ void Tissue2D::CLEulerUpdate(int nsteps, float dt) {
   check(dpm2d_get_positions(h_.get(), xy.data(), V));
   check(dpm2d_get_forces(h_.get(), f.data(), V));
   k = 0;
-  for (auto& c : cells)
+  for (auto& c : cells) {
+    if (c.Forces.size() < c.NV) c.Forces.resize(c.NV);
     for (unsigned int vi = 0; vi < c.NV; vi++, k++) {
       c.Verticies[vi] = {xy[2 * k], xy[2 * k + 1]};
       c.Forces[vi] = {f[2 * k], f[2 * k + 1]};
     }
+  }
 }
 
 // ---------------------------------------------------------------- Tissue3D
@@ -405,6 +433,8 @@ Tissue3D::Tissue3D(std::vector<Cell3D> cells, float phi0) {
 }
 
 void Tissue3D::Disperse2D() {
+  NCELLS = (int)Cells.size();  // Cells may have been reassigned from Python
+  for (const auto& c : Cells) require_entries(c.Verts.size(), c.NV, "Verts");
   std::vector<float> r(NCELLS), X, Y;
   for (int i = 0; i < NCELLS; i++) r[i] = Cells[i].r0;
   relax_centres(r, 2.0f, L, X, Y, true);
@@ -537,6 +567,7 @@ void Tissue3D::CLEulerUpdate(int nsteps, float dt) {
   // Result checks and host reductions as Tissue3D.cpp:477-521.
   for (int c = 0; c < C; c++) {
     Cell3D& cell = Cells[c];
+    if ((int)cell.Forces.size() < nv) cell.Forces.resize(nv);
     for (int v = 0; v < nv; v++) {
       for (int k = 0; k < 3; k++) {
         const size_t i = ((size_t)c * nv + v) * 3 + k;

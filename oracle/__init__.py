@@ -37,20 +37,25 @@ def _t(prec):
 
 
 def forces2d(verts, stride, nv, Ka, Kl, Kb, a0, l0, r0, L, Kre, Kat, pbc=1, mask=31, prec=32,
-             lib="liboracle.so"):
-    """One 2D force evaluation.  verts: padded [ncells*stride, 2]."""
+             lib="liboracle.so", attr_hash=False):
+    """One 2D force evaluation.  verts: padded [ncells*stride, 2].  Returns forces,
+    overlap flags, attraction partner counts and, with attr_hash, each vertex's
+    attraction partner-set key sum (vertex_keys)."""
     dt_, ct, sfx = _t(prec)
     nc = len(nv)
     V = np.ascontiguousarray(np.asarray(verts, dt_).reshape(nc * stride, 2))
     F = np.zeros_like(V)
     ov = np.zeros(nc * stride, np.int32)
     at = np.zeros(nc * stride, np.int32)
+    ah = np.zeros(nc * stride, np.uint32) if attr_hash else None
     arr = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dt_), (nc,)))
     ps = [arr(a) for a in (Ka, Kl, Kb, a0, l0, r0)]
     nvv = np.ascontiguousarray(np.asarray(nv, np.int32))
     f = getattr(_lib(lib), "ora2d_forces" + sfx)
     f(ctypes.c_int(nc), ctypes.c_int(stride), _p(nvv), _p(V), *[_p(a) for a in ps], ct(L),
-      ctypes.c_int(pbc), ct(Kre), ct(Kat), ctypes.c_uint(mask), _p(F), _p(ov), _p(at))
+      ctypes.c_int(pbc), ct(Kre), ct(Kat), ctypes.c_uint(mask), _p(F), _p(ov), _p(at), _p(ah))
+    if attr_hash:
+        return F, ov, at, ah
     return F, ov, at
 
 
@@ -170,12 +175,14 @@ def forces3d(X, faces, nv, Kv, Ka, Ks, v0, a0, L, Kre, Kat=0.0, Kb=0.0, pbc=1, m
     wn = np.zeros((nc * nv, nc), dt_) if winding else None
     down = np.zeros((nc, nf), np.uint8)
     adh = np.zeros(nc * nv, np.int32)
+    ah = np.zeros((nc * nv, 2), np.uint32) if pairs else None
     f = getattr(_lib(lib), "ora3d_forces" + sfx)
     rc = f(ctypes.c_int(nc), ctypes.c_int(nv), ctypes.c_int(nf), _p(faces), _p(adj), ctypes.c_int(8),
            _p(X), *[_p(a) for a in ps], ct(L), ctypes.c_int(pbc), ct(Kre), ct(Kat), ctypes.c_uint(mask),
-           ct(ks_grad), _p(F), _p(pr), _p(wn), _p(down), _p(adh))
+           ct(ks_grad), _p(F), _p(pr), _p(wn), _p(down), _p(adh), _p(ah))
     assert rc == 0
-    return {"forces": F, "contact": pr, "winding": wn, "down": down, "adhesion": adh}
+    return {"forces": F, "contact": pr, "winding": wn, "down": down, "adhesion": adh,
+            "adhesion_hash": ah}
 
 
 def run3d(X, faces, nv, Kv, Ka, Ks, v0, a0, L, Kre, nsteps, dt, Kat=0.0, Kb=0.0, pbc=1, mask=15,
@@ -240,22 +247,3 @@ REF_LIB = os.path.join(_HERE, "_ref", "libdpmref.so")
 
 def ref_available() -> bool:
     return os.path.exists(REF_LIB)
-
-
-def ref_cell3d(sp, calA, r0):
-    L = _lib(os.path.join("_ref", "libdpmref.so"))
-    v = np.zeros(162 * 3, np.float32)
-    f = np.zeros(320 * 3, np.int32)
-    s = np.zeros(7, np.float32)
-    spa = np.asarray(sp, np.float32)
-    L.refshim_cell3d(_p(spa), ctypes.c_float(calA), ctypes.c_float(r0), _p(v), _p(f), _p(s))
-    return v.reshape(162, 3), f.reshape(320, 3), s
-
-
-def ref_cell2d(x0, y0, calA, nv, r0):
-    L = _lib(os.path.join("_ref", "libdpmref.so"))
-    v = np.zeros(nv * 2, np.float32)
-    s = np.zeros(4, np.float32)
-    L.refshim_cell2d(ctypes.c_float(x0), ctypes.c_float(y0), ctypes.c_float(calA), ctypes.c_uint(nv),
-                     ctypes.c_float(r0), _p(v), _p(s))
-    return v.reshape(nv, 2), s

@@ -55,6 +55,37 @@ static inline void FN(normalize2)(R x, R y, R* ux, R* uy) {
   *ux = x * inv; *uy = y * inv;
 }
 
+/* Pair-set fingerprints (diagnostics, no reference counterpart): a partner
+ * vertex is identified by the bit patterns of its coordinates,
+ * key = f(x ^ f(y ^ f(z))) with f the 32-bit finaliser of MurmurHash3 (a
+ * bijection of 32-bit words; a double contributes f(low word) ^ high word),
+ * and a vertex's partner set by the wrapping 32-bit sum of its partners'
+ * keys, which does not depend on the order the pairs are found in. */
+static inline unsigned FN(fmix32)(unsigned h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+static inline unsigned FN(rbits)(R v) {
+  if (sizeof(R) == 4) {
+    unsigned b;
+    memcpy(&b, &v, 4);
+    return b;
+  }
+  unsigned long long b = 0;
+  memcpy(&b, &v, sizeof(R));
+  return FN(fmix32)((unsigned)b) ^ (unsigned)(b >> 32);
+}
+static inline unsigned FN(vkey2)(R x, R y) {
+  return FN(fmix32)(FN(rbits)(x) ^ FN(fmix32)(FN(rbits)(y)));
+}
+static inline unsigned FN(vkey3)(R x, R y, R z) {
+  return FN(fmix32)(FN(rbits)(x) ^ FN(fmix32)(FN(rbits)(y) ^ FN(fmix32)(FN(rbits)(z))));
+}
+
 /* ======================================================================= 2D
  * Layout: padded AoS like the reference, V[(ci*stride + vi)*2 + {0,1}]
  * (Tissue2D.cpp:117-140).  Only vi < NV[ci] is read or written.
@@ -109,12 +140,15 @@ static int FN(crossing_parity)(const R* V, int stride, const int* NV, int cj,
 int FN(ora2d_forces)(int ncells, int stride, const int* NV, const R* V,
                      const R* Ka, const R* Kl, const R* Kb, const R* a0,
                      const R* l0, const R* r0, R L, int PBC, R Kre, R Kat,
-                     unsigned mask, R* F, int* overlap, int* attr_count) {
+                     unsigned mask, R* F, int* overlap, int* attr_count,
+                     unsigned* attr_hash) {
   for (int k = 0; k < ncells * stride * 2; k++) F[k] = 0;
   if (overlap)
     for (int k = 0; k < ncells * stride; k++) overlap[k] = 0;
   if (attr_count)
     for (int k = 0; k < ncells * stride; k++) attr_count[k] = 0;
+  if (attr_hash)
+    for (int k = 0; k < ncells * stride; k++) attr_hash[k] = 0;
 
   ORA_PARALLEL_FOR
   for (int ci = 0; ci < ncells; ci++) {
@@ -212,6 +246,7 @@ int FN(ora2d_forces)(int ncells, int stride, const int* NV, const R* V,
               F[index * 2 + 0] += ftmp * ux;
               F[index * 2 + 1] += ftmp * uy;
               if (attr_count) attr_count[index]++;
+              if (attr_hash) attr_hash[index] += FN(vkey2)(X(j), Y(j));
             }
           }
         }
@@ -284,7 +319,7 @@ int FN(ora2d_run)(int ncells, int stride, const int* NV, R* V, const R* Ka,
   if (!F) return -1;
   for (int s = 0; s < nsteps; s++) {
     FN(ora2d_forces)(ncells, stride, NV, V, Ka, Kl, Kb, a0, l0, r0, L, PBC, Kre,
-                     Kat, mask, F, NULL, NULL);
+                     Kat, mask, F, NULL, NULL, NULL);
     for (int ci = 0; ci < ncells; ci++)
       for (int vi = 0; vi < NV[ci]; vi++) {
         int k = (ci * stride + vi) * 2;
@@ -382,7 +417,7 @@ int FN(ora2d_fire)(int ncells, int stride, const int* NV, R* V, R* Vel,
   int npos = 0, it = 0, conv = 0, cleared = 0;
   for (it = 0; it < ip[0]; it++) {
     FN(ora2d_forces)(ncells, stride, NV, V, Ka, Kl, Kb, a0, l0, r0, L, PBC, Kre,
-                     Kat, mask, F, NULL, NULL);
+                     Kat, mask, F, NULL, NULL, NULL);
     double P = 0.0, F2 = 0.0, v2 = 0.0;
     fmax = 0.0;
     for (int ci = 0; ci < ncells; ci++)
@@ -594,17 +629,23 @@ typedef struct {
  *   pairs[(ci*nv+v)*ncells + cj]
  * wnum (optional, same shape, R): winding numbers of every evaluated pair
  * down (optional, ncells*nf bytes): faces that pass the facing-down test
- * adh_count (optional, ncells*nv ints): adhesion partners (own side) */
+ * adh_count (optional, ncells*nv ints): adhesion partners (own side)
+ * adh_hash (optional, ncells*nv*2): per vertex a, the key sum (vkey3) of the
+ *   partners b its own loop accepts, then of the vertices b whose own loop
+ *   accepts a (the reverse pairs whose force lands on a) */
 int FN(ora3d_forces)(int ncells, int nv, int nf, const int* faces,
                      const int* adj, int maxdeg, const R* X, const R* Kv,
                      const R* Ka, const R* Ks, const R* Kb, const R* v0,
                      const R* a0, R L, int PBC, R Kre, R Kat, unsigned mask,
                      R ks_grad, R* F, unsigned char* pairs, R* wnum,
-                     unsigned char* down, int* adh_count) {
+                     unsigned char* down, int* adh_count,
+                     unsigned* adh_hash) {
   const size_t nvt = (size_t)ncells * nv;
   for (size_t k = 0; k < nvt * 3; k++) F[k] = 0;
   if (adh_count)
     for (size_t k = 0; k < nvt; k++) adh_count[k] = 0;
+  if (adh_hash)
+    for (size_t k = 0; k < 2 * nvt; k++) adh_hash[k] = 0;
   R* l0 = (R*)malloc(sizeof(R) * ncells);
   R* com = (R*)malloc(sizeof(R) * ncells * 3);
   if (!l0 || !com) return -1;
@@ -774,6 +815,10 @@ int FN(ora3d_forces)(int ncells, int nv, int nf, const int* faces,
                 F[b * 3 + k] += t;
               }
               if (adh_count) adh_count[a]++;
+              if (adh_hash) {
+                adh_hash[2 * a] += FN(vkey3)(X[b * 3], X[b * 3 + 1], X[b * 3 + 2]);
+                adh_hash[2 * b + 1] += FN(vkey3)(X[a * 3], X[a * 3 + 1], X[a * 3 + 2]);
+              }
             }
           }
         }
@@ -834,7 +879,7 @@ int FN(ora3d_run)(int ncells, int nv, int nf, const int* faces, const int* adj,
   for (int s = 0; s < nsteps; s++) {
     int rc = FN(ora3d_forces)(ncells, nv, nf, faces, adj, maxdeg, X, Kv, Ka, Ks,
                               Kb, v0, a0, L, PBC, Kre, Kat, mask, ks_grad, F,
-                              NULL, NULL, NULL, NULL);
+                              NULL, NULL, NULL, NULL, NULL);
     if (rc) { free(F); return rc; }
     for (size_t k = 0; k < n3; k++) X[k] = FMA(F[k], dt, X[k]);
   }

@@ -6,7 +6,8 @@ Bars (stated per test):
   * pair sets (3D contact vertex-cell pairs, facing-down faces, adhesion
     partners; 2D overlap and attraction partners): bit-exact against the
     float restatement ref32, whose decisions use the same IEEE operation
-    order as the kernels;
+    order as the kernels (partner sets as counts plus per-vertex key sums,
+    dpm3d_probe_adhesion / dpm2d_probe_attraction);
   * batching, graph replay and repeated runs: bit-identical to the plain path.
 """
 import numpy as np
@@ -78,6 +79,7 @@ def test_3d_forces_and_pairs(case, mask, kernel):
     np.testing.assert_array_equal(g["contact"], r32["contact"])
     np.testing.assert_array_equal(g["down"], r32["down"])
     np.testing.assert_array_equal(g["adhesion"], r32["adhesion"])
+    np.testing.assert_array_equal(g["adhesion_hash"], r32["adhesion_hash"])
     print(f"pairs: contact={int(g['contact'].sum())} down={int(g['down'].sum())} "
           f"adhesion={int(g['adhesion'].sum())}", flush=True)
     if case.startswith("compressed") and mask & capi.DPM3D_CONTACT:
@@ -100,6 +102,7 @@ def test_3d_pair_sets_across_the_separation_cutoffs(kernel):
         assert_forces(g["forces"], ref3d(m, m.X, capi.DPM3D_ALL, 64)["forces"], f"3d spacing={sp:.2f} kernel={kernel}")
         np.testing.assert_array_equal(g["contact"], r32["contact"])
         np.testing.assert_array_equal(g["adhesion"], r32["adhesion"])
+        np.testing.assert_array_equal(g["adhesion_hash"], r32["adhesion_hash"])
         print(f"spacing {sp:.2f}: contact={int(g['contact'].sum())} adhesion={int(g['adhesion'].sum())}", flush=True)
 
 
@@ -118,6 +121,7 @@ def test_3d_pairs_after_deformation(spacing, seed):
     np.testing.assert_array_equal(g["contact"], r32["contact"])
     np.testing.assert_array_equal(g["down"], r32["down"])
     np.testing.assert_array_equal(g["adhesion"], r32["adhesion"])
+    np.testing.assert_array_equal(g["adhesion_hash"], r32["adhesion_hash"])
     print(f"deformed pairs: contact={int(g['contact'].sum())}", flush=True)
     assert g["contact"].sum() > 0
 
@@ -143,7 +147,7 @@ def test_3d_wave_and_workgroup_kernels_agree(mask):
     print(f"kernels: mask={mask} force rel {r:.3e} contacts {int(g1['contact'].sum())} "
           f"adhesion {int(g1['adhesion'].sum())}", flush=True)
     assert r <= 1e-6
-    for key in ("contact", "down", "adhesion"):
+    for key in ("contact", "down", "adhesion", "adhesion_hash"):
         np.testing.assert_array_equal(g1[key], g2[key])
     out = []
     for kind in (1, 2):
@@ -413,11 +417,13 @@ def tissue2d_handle(p: W.Packing2D, mask=capi.DPM2D_ALL):
                           mask=mask)
 
 
-def ref2d(p: W.Packing2D, X, prec, mask=31):
+def ref2d(p: W.Packing2D, X, prec, mask=31, keys=False):
     P, stride = W.Packing2D(p.nv, X, p.a0, p.l0, p.r0, p.Ka, p.Kl, p.Kb, p.L).padded()
-    F, ov, at = oracle.forces2d(P, stride, p.nv, p.Ka, p.Kl, p.Kb, p.a0, p.l0, p.r0, p.L, p.Kre, p.Kat,
-                                mask=mask, prec=prec)
+    F, ov, at, ah = oracle.forces2d(P, stride, p.nv, p.Ka, p.Kl, p.Kb, p.a0, p.l0, p.r0, p.L, p.Kre, p.Kat,
+                                    mask=mask, prec=prec, attr_hash=True)
     keep = np.concatenate([np.arange(c * stride, c * stride + p.nv[c]) for c in range(p.C)])
+    if keys:
+        return F[keep], ov[keep], at[keep], ah[keep]
     return F[keep], ov[keep], at[keep]
 
 
@@ -437,10 +443,11 @@ def test_2d_forces_and_pairs(case):
     t.set_positions(p.X)
     g = t.probe()
     F64, _, _ = ref2d(p, p.X, 64)
-    F32, ov32, at32 = ref2d(p, p.X, 32)
+    F32, ov32, at32, ah32 = ref2d(p, p.X, 32, keys=True)
     assert_forces(g["forces"], F64, f"2d {case}")
     np.testing.assert_array_equal(g["overlap"], ov32)
     np.testing.assert_array_equal(g["attraction"], at32)
+    np.testing.assert_array_equal(g["attraction_hash"], ah32)
     print(f"pairs: overlap={int(ov32.sum())} attraction={int(at32.sum())}", flush=True)
 
 
@@ -559,6 +566,7 @@ def test_3d_cell_list_equals_scan(n_side, spacing, subdiv, kernel):
     np.testing.assert_array_equal(g0["forces"], g1["forces"])
     np.testing.assert_array_equal(g0["contact"], g1["contact"])
     np.testing.assert_array_equal(g0["adhesion"], g1["adhesion"])
+    np.testing.assert_array_equal(g0["adhesion_hash"], g1["adhesion_hash"])
     np.testing.assert_array_equal(X0, X1)
     np.testing.assert_array_equal(F0, F1)
     print(f"cell list {n_side}x{n_side} s={subdiv} kernel={kernel}: contacts {int(g0['contact'].sum())}, "

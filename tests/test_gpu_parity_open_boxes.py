@@ -1,0 +1,188 @@
+"""Oracle parity on the code paths the headline 3D workload runs.
+
+In a periodic box wider than about four cell diameters, neighbouring cells
+need no periodic image ("no_image" in the step kernels).  Only there do the
+step kernels skip the adhesion pair tests of candidate pairs whose
+centre-line separation exceeds 2 l0, and test vertex pairs without the
+image rounding.  The boxes here (6 x 6 and 8 x 8 monolayers, and the
+256-cell monolayer itself) take those paths; every result is compared with
+the oracle from the same float inputs:
+  * forces: ||F_gpu - F_ref64|| / ||F_ref64|| <= 1e-5 and every vertex within
+    1e-4 * max|F_ref64|;
+  * contact (vertex, cell) pairs and facing-down faces: bit-exact vs ref32;
+  * adhesion: per vertex the partner count and the key sums of the partners
+    its own loop accepts and of the vertices whose loop accepts it
+    (dpm3d_probe_adhesion), equal to ref32's.
+Both step forms, with the cell list on (bins active) and off.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from opencl_dpm_amd import capi
+from opencl_dpm_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FORCE_RTOL = 1e-5
+FORCE_VTOL = 1e-4
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def tissue3d(m, mask=capi.DPM3D_ALL):
+    return capi.Tissues3D(m.mesh.faces, [0, m.C], m.Kv, m.Ka, m.Ks, m.v0, m.a0, m.L, m.Kre, Kat=m.Kat, Kb=m.Kb,
+                          ks_grad=m.ks_grad, mask=mask, nv=m.mesh.nv)
+
+
+def oracle_pair(m, X, lib="liboracle.so"):
+    kw = dict(Kat=m.Kat, Kb=m.Kb, mask=capi.DPM3D_ALL, ks_grad=m.ks_grad, lib=lib)
+    args = (X, m.mesh.faces, m.mesh.nv, m.Kv, m.Ka, m.Ks, m.v0, m.a0, m.L, m.Kre)
+    log(f"  oracle ref32 ({m.C} cells) ...")
+    r32 = oracle.forces3d(*args, prec=32, pairs=True, **kw)
+    log(f"  oracle ref64 ({m.C} cells) ...")
+    r64 = oracle.forces3d(*args, prec=64, **kw)
+    return r32, r64
+
+
+def check(g, r32, r64, what):
+    F64 = r64["forces"]
+    Fg = np.asarray(g["forces"], np.float64)
+    r = float(np.linalg.norm(Fg - F64) / np.linalg.norm(F64))
+    vmax = float(np.abs(Fg - F64).max())
+    scale = float(np.abs(F64).max())
+    log(f"  {what}: rel={r:.3e} vmax={vmax:.3e} scale={scale:.3e} contact={int(g['contact'].sum())} "
+        f"down={int(g['down'].sum())} adhesion={int(g['adhesion'].sum())}")
+    assert r <= FORCE_RTOL, (what, r)
+    assert vmax <= FORCE_VTOL * scale, (what, vmax, scale)
+    np.testing.assert_array_equal(g["contact"], r32["contact"])
+    np.testing.assert_array_equal(g["down"], r32["down"])
+    np.testing.assert_array_equal(g["adhesion"], r32["adhesion"])
+    np.testing.assert_array_equal(g["adhesion_hash"], r32["adhesion_hash"])
+
+
+def coverage(m, X):
+    """Candidate cell pairs (centres within R_i + R_j + 2 l0) that need no
+    periodic image by the kernels' test, and those among them whose nearest
+    vertices are at least 2 l0 apart (adhesion pairs a separation bound may
+    skip)."""
+    nv = m.mesh.nv
+    P = np.asarray(X, np.float64).reshape(m.C, nv, 3)
+    C = P.mean(1)
+    R = np.sqrt(((P - C[:, None]) ** 2).sum(-1)).max(1)
+    l0 = np.sqrt(4 * np.asarray(np.broadcast_to(m.a0, (m.C,)), np.float64)) / math.sqrt(3)
+    L = float(m.L)
+    no_image = far = 0
+    for i in range(m.C):
+        for j in range(m.C):
+            if i == j:
+                continue
+            d = C[j] - C[i]
+            d -= L * np.round(d / L)
+            l0m = max(l0[i], l0[j])
+            if np.linalg.norm(d) > R[i] + R[j] + 2 * l0m:
+                continue
+            if np.all(np.abs(C[i] - C[j]) + (R[i] + R[j]) < 0.48 * L):
+                no_image += 1
+                q = P[j] - np.round((P[j] - P[i].mean(0)) / L) * L
+                dmin = np.sqrt(((P[i][:, None, :] - q[None, :, :]) ** 2).sum(-1)).min()
+                far += dmin >= 2 * l0m * 1.05
+    return no_image, far
+
+
+def expected_bins(m, X):
+    """Bins per axis the cell list uses (k3d_bin: floor(L / reach), reach the
+    largest candidate reach of the step, 0 below 3)."""
+    nv = m.mesh.nv
+    P = np.asarray(X, np.float64).reshape(m.C, nv, 3)
+    C = P.mean(1)
+    rmax = np.sqrt(((P - C[:, None]) ** 2).sum(-1)).max()
+    F = np.asarray(m.mesh.faces)
+    e = np.concatenate([P[:, F[:, 1]] - P[:, F[:, 0]], P[:, F[:, 2]] - P[:, F[:, 1]], P[:, F[:, 0]] - P[:, F[:, 2]]], 1)
+    emax = np.sqrt((e ** 2).sum(-1)).max()
+    l0max = math.sqrt(4 * float(np.max(m.a0))) / math.sqrt(3)
+    reach = max(2 * rmax + 0.4004 * emax, 2 * rmax + 2 * l0max) * 1.0001 * 1.001
+    nb = int(math.floor(float(m.L) / reach))
+    return nb if nb >= 3 else 0
+
+
+OPEN_BOXES = {
+    # name: (n_side, spacing, seed)
+    "6x6_spacing2.4": (6, 2.4, 60),
+    "8x8_spacing2.4": (8, 2.4, 61),
+    "8x8_spacing2.1": (8, 2.1, 62),  # squeezed: overlapping neighbours, contacts
+}
+
+
+@pytest.mark.parametrize("case", list(OPEN_BOXES))
+def test_open_box_start_and_after_200_steps(case):
+    n, sp, seed = OPEN_BOXES[case]
+    m = W.monolayer3d(n, sp, seed=seed)
+    run = tissue3d(m)
+    run.set_positions(m.X)
+    run.euler(200, 0.002)
+    states = {"start": m.X, "after 200 steps": run.positions()}
+    for label, X in states.items():
+        ni, far = coverage(m, X)
+        nb = expected_bins(m, X)
+        log(f"{case} {label}: candidate pairs without image {ni}, of which beyond 2 l0 {far}; bins {nb}")
+        assert ni > 0 and far > 0 and nb >= 3
+        r32, r64 = oracle_pair(m, X)
+        for kernel in (1, 2):
+            for cl in (0, 1):
+                t = tissue3d(m)
+                t.set_step_kernel(kernel)
+                t.set_cell_list(cl)
+                t.set_positions(X)
+                check(t.probe(), r32, r64, f"{case} {label} kernel={kernel} cell_list={cl}")
+        assert r32["adhesion"].sum() > 0
+        if sp < 2.4:
+            assert r32["contact"].sum() > 0
+
+
+@pytest.mark.timeout(1200)
+def test_headline_monolayer_start_and_after_200_steps():
+    # the 256-cell x 162-vertex monolayer of the benchmark, all terms, at its
+    # start and after 200 steps; the oracle runs all pairs (OpenMP build)
+    m = W.monolayer_config("cfg2", seed=0)
+    run = tissue3d(m)
+    run.set_positions(m.X)
+    run.euler(200, m.dt)
+    for label, X in (("start", m.X), ("after 200 steps", run.positions())):
+        ni, far = coverage(m, X)
+        log(f"cfg2 {label}: candidate pairs without image {ni}, beyond 2 l0 {far}; bins {expected_bins(m, X)}")
+        assert ni > 0 and far > 0
+        r32, r64 = oracle_pair(m, X, lib="liboracle_omp.so")
+        for kernel in (1, 2):
+            for cl in (0, 1):
+                t = tissue3d(m)
+                t.set_step_kernel(kernel)
+                t.set_cell_list(cl)
+                t.set_positions(X)
+                check(t.probe(), r32, r64, f"cfg2 {label} kernel={kernel} cell_list={cl}")
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_probe_instance_equals_production_step(kernel):
+    # probes of 162-vertex meshes run the step's probe instance (the same
+    # source with the diagnostic outputs compiled in); its forces are the
+    # production kernel's bit for bit (dpm_euler's last-step forces)
+    m = W.monolayer3d(6, 2.1, seed=63)
+    t = tissue3d(m)
+    t.set_step_kernel(kernel)
+    t.set_positions(m.X)
+    t.euler(50, 0.002)
+    X = t.positions()
+    for cl in (0, 1):
+        t.set_cell_list(cl)
+        t.set_positions(X)
+        g = t.probe()
+        t.set_positions(X)
+        t.euler(1, 0.002)
+        np.testing.assert_array_equal(g["forces"], t.forces())
+        log(f"kernel={kernel} cell_list={cl}: probe forces = step forces (contacts {int(g['contact'].sum())}, "
+            f"adhesion {int(g['adhesion'].sum())})")

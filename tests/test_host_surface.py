@@ -167,3 +167,49 @@ def test_library_fails_loudly_without_device():
     T = make_tissue3d()
     with pytest.raises(RuntimeError):
         T.CLEulerUpdate(2, 0.01)
+
+
+def test_short_verts_lists_raise_value_error_before_any_device_call():
+    # Verts is read-write from Python; the drop-in reads NV entries, so a
+    # shorter list is an error (the reference would read past its end)
+    T = clDPM.Tissue2D([clDPM.Cell2D(0, 0, 1.0, 16, 1.0), clDPM.Cell2D(3, 0, 1.0, 16, 1.0)], 0.8)
+    T.Cells[1].Verts = T.Cells[1].Verts[:10]
+    with pytest.raises(ValueError, match="fewer than NV"):
+        T.CLEulerUpdate(3, 0.005)
+    with pytest.raises(ValueError, match="fewer than NV"):
+        T.Disperse()
+    c = clDPM.Cell2D(0, 0, 1.0, 16, 1.0)
+    c.Verts = c.Verts[:10]
+    with pytest.raises(ValueError, match="fewer than NV"):
+        c.GetArea()
+    with pytest.raises(ValueError, match="fewer than NV"):
+        clDPM.Tissue2D([c], 0.8)
+    c3 = clDPM.Cell3D([0, 0, 0], 1.0, 1.0)
+    c3.Verts = c3.Verts[:100]
+    for f in (c3.GetPositions, c3.GetVolume, c3.GetSurfaceArea, c3.GetCOM, lambda: c3.GetVesselPositions(10.0)):
+        with pytest.raises(ValueError, match="fewer than NV"):
+            f()
+    c3 = clDPM.Cell3D([0, 0, 0], 1.0, 1.0)
+    c3.Forces = c3.Forces[:5]
+    with pytest.raises(ValueError, match="fewer than NV"):
+        c3.GetForces()
+    T3 = clDPM.Tissue3D([clDPM.Cell3D([0, 0, 0], 1.0, 1.0)], 0.35)
+    T3.Cells = [T3.Cells[0], clDPM.Cell3D([3, 0, 0], 1.0, 1.0)]
+    v = T3.Cells[1].Verts
+    T3.Cells[1].Verts = v[:50]
+    with pytest.raises(ValueError, match="fewer than NV"):
+        T3.Disperse2D()
+
+
+def test_longer_verts_list_is_packed_to_nv():
+    # a longer Verts list is packed to its first NV entries, as the reference
+    # packs (Tissue2D.cpp:137-139): without a device the call fails in the
+    # library, after packing, with the library's error
+    c = clDPM.Cell2D(0, 0, 1.0, 16, 1.0)
+    c.Verts = c.Verts + [[5.0, 5.0]] * 4
+    T = clDPM.Tissue2D([c], 0.8)
+    if capi.device_count() > 0:
+        pytest.skip("a GPU is visible (covered by the GPU drop-in tests)")
+    with pytest.raises(RuntimeError):
+        T.CLEulerUpdate(3, 0.005)
+    assert len(T.Cells[0].Verts) == 20

@@ -582,3 +582,141 @@ def test_euler_step_is_fused_update():
     np.testing.assert_array_equal(Fl, F)
     x64 = X.astype(np.float64) + F.astype(np.float64) * np.float64(np.float32(0.01))
     np.testing.assert_array_equal(Xn, x64.astype(np.float32))  # one rounding: fma
+
+
+def test_2d_euler_loop_is_fused_update_with_last_step_forces():
+    # ora2d_run (Tissue2D.cpp:215-229 with EulerUpdate, Cell2D_kernel.cl:270-281):
+    # each step evaluates the forces at the current positions and moves every
+    # vertex by x + F dt rounded once (fused); the forces handed back are the
+    # ones the last update used (read before EulerUpdate zeroes them).  A dense
+    # packing with attraction so that every term contributes.
+    p = W.tissue2d(6, 20, 1.4, 0.92, Kb=0.01, Kat=0.5, seed=11)
+    P, stride = p.padded()
+    P = P.astype(np.float32)
+    # a large step, so that F dt is comparable to the coordinates and one
+    # rounding (fused) and two roundings differ at many vertices
+    dt = np.float32(0.37)
+    args = (stride, p.nv, p.Ka, p.Kl, p.Kb, p.a0, p.l0, p.r0, p.L, p.Kre, p.Kat)
+
+    def forces(X):
+        F, ov, at = oracle.forces2d(X, *args, prec=32)
+        return F, ov, at
+
+    def fused(X, F):
+        return (X.astype(np.float64) + F.astype(np.float64) * np.float64(dt)).astype(np.float32)
+
+    real = np.concatenate([np.arange(c * stride, c * stride + p.nv[c]) for c in range(p.C)])
+    F0, ov, at = forces(P)
+    assert ov[real].sum() > 0 and at[real].sum() > 0  # repulsion and attraction pairs present
+    X1, Fl1 = oracle.run2d(P, *args, 1, dt, prec=32)
+    np.testing.assert_array_equal(Fl1[real], F0[real])
+    np.testing.assert_array_equal(X1[real], fused(P, F0)[real])
+    F1, _, _ = forces(X1)
+    assert not np.array_equal(F1[real], F0[real])
+    X2, Fl2 = oracle.run2d(P, *args, 2, dt, prec=32)
+    np.testing.assert_array_equal(Fl2[real], F1[real])  # forces at the pre-step positions of step 2
+    np.testing.assert_array_equal(X2[real], fused(X1, F1)[real])
+    # a two-rounding update (x + round(F dt)) differs somewhere: the check above has teeth
+    two = (P + (F0 * dt).astype(np.float32)).astype(np.float32)
+    assert not np.array_equal(two[real], fused(P, F0)[real])
+    X0, Fl0 = oracle.run2d(P, *args, 0, dt, prec=32)
+    np.testing.assert_array_equal(X0, P)
+    assert not Fl0.any()
+
+
+def fmix32(h):
+    h = np.asarray(h, np.uint32).copy()
+    h ^= h >> np.uint32(16)
+    h *= np.uint32(0x85EBCA6B)
+    h ^= h >> np.uint32(13)
+    h *= np.uint32(0xC2B2AE35)
+    h ^= h >> np.uint32(16)
+    return h
+
+
+def keys3(P):
+    b = np.ascontiguousarray(P, np.float32).view(np.uint32)
+    return fmix32(b[:, 0] ^ fmix32(b[:, 1] ^ fmix32(b[:, 2])))
+
+
+def keys2(P):
+    b = np.ascontiguousarray(P, np.float32).view(np.uint32)
+    return fmix32(b[:, 0] ^ fmix32(b[:, 1]))
+
+
+def test_fmix32_is_murmur3_finaliser():
+    # MurmurHash3's fmix32 (a bijection) on known values: fmix32(0) = 0 and
+    # the published constants' effect on 1
+    assert int(fmix32([0])[0]) == 0
+    h = 1
+    h ^= h >> 16
+    h = (h * 0x85EBCA6B) & 0xFFFFFFFF
+    h ^= h >> 13
+    h = (h * 0xC2B2AE35) & 0xFFFFFFFF
+    h ^= h >> 16
+    assert int(fmix32([1])[0]) == h
+    x = np.arange(1 << 16, dtype=np.uint32) * np.uint32(65537)
+    assert len(np.unique(fmix32(x))) == len(x)
+
+
+def test_adhesion_key_sums_match_brute_force_pairs():
+    # the oracle's adhesion pair sets (own and reverse, as key sums) against an
+    # independent float32 brute force of Cell3D_Kernel.cl:341-353: minimum
+    # image per component, |d| < 2 l0 of the own cell, |d| > 1e-12; a small
+    # periodic box, so that pairs across the boundary take an image
+    m = W.monolayer3d(3, 1.75, seed=12)
+    X = m.X.astype(np.float32)
+    nv, L = m.mesh.nv, np.float32(m.L)
+    r = oracle.forces3d(X, m.mesh.faces, nv, m.Kv, m.Ka, m.Ks, m.v0, m.a0, m.L, m.Kre, Kat=0.1, Kb=m.Kb,
+                        mask=16, prec=32, pairs=True)
+    l0 = (np.sqrt(np.float32(4) * np.asarray(m.a0, np.float32)) / np.sqrt(np.float32(3))).astype(np.float32)
+    cell = np.repeat(np.arange(m.C), nv)
+    key = keys3(X)
+    own = np.zeros(len(X), np.uint32)
+    rev = np.zeros(len(X), np.uint32)
+    cnt = np.zeros(len(X), np.int32)
+    crossing = 0
+    for a in range(len(X)):
+        d = X - X[a]
+        img = np.round(d / L).astype(np.float32)
+        crossing += int(np.any(img[cell != cell[a]] != 0))
+        d = (d - L * img).astype(np.float32)
+        dist = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+        lim = np.float32(2) * l0[cell[a]]
+        acc = (cell != cell[a]) & (dist < lim) & (dist > np.float32(1e-12))
+        cnt[a] = int(acc.sum())
+        own[a] = np.sum(key[acc], dtype=np.uint32)
+        np.add.at(rev, np.nonzero(acc)[0], key[a])
+    assert cnt.sum() > 0 and crossing > 0
+    np.testing.assert_array_equal(r["adhesion"], cnt)
+    np.testing.assert_array_equal(r["adhesion_hash"][:, 0], own)
+    np.testing.assert_array_equal(r["adhesion_hash"][:, 1], rev)
+    # a partner dropped from one vertex's set changes its key sum
+    assert np.uint32(own[np.argmax(cnt)] - key[0]) != own[np.argmax(cnt)]
+
+
+def test_attraction_key_sums_match_brute_force_pairs():
+    # Cell2D_kernel.cl:250-266: minimum image, d < l0 of the own cell
+    p = W.tissue2d(8, 20, 1.4, 0.92, Kb=0.01, Kat=0.5, seed=13)
+    P, stride = p.padded()
+    P = P.astype(np.float32)
+    F, ov, at, ah = oracle.forces2d(P, stride, p.nv, p.Ka, p.Kl, p.Kb, p.a0, p.l0, p.r0, p.L, p.Kre, p.Kat,
+                                    prec=32, attr_hash=True)
+    real = np.concatenate([np.arange(c * stride, c * stride + p.nv[c]) for c in range(p.C)])
+    X = P.reshape(-1, 2)[real]
+    cell = np.repeat(np.arange(p.C), p.nv)
+    L = np.float32(p.L)
+    key = keys2(X)
+    l0 = np.asarray(p.l0, np.float32)
+    cnt = np.zeros(len(X), np.int32)
+    own = np.zeros(len(X), np.uint32)
+    for a in range(len(X)):
+        d = X - X[a]
+        d = (d - L * np.round(d / L).astype(np.float32)).astype(np.float32)
+        dist = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1])
+        acc = (cell != cell[a]) & (dist < l0[cell[a]])
+        cnt[a] = int(acc.sum())
+        own[a] = np.sum(key[acc], dtype=np.uint32)
+    assert cnt.sum() > 0
+    np.testing.assert_array_equal(at[real], cnt)
+    np.testing.assert_array_equal(ah[real], own)
