@@ -16,7 +16,7 @@ resident in HBM; the plain rate is reported beside it.
 
 roofline: the only kernel of a batched step is k3d_wstep_f32_v162 (one wave
 per cell; the wave-form step compiled for 162-vertex, 320-face cells; the
-plain 256-cell tissue runs k3d_step_f32, one workgroup per cell, which fills
+plain 256-cell tissue runs k3d_step_f32_v162, one workgroup per cell, which fills
 the device better at that size).  Algorithmic
 bytes per vertex-step = 24 (position read + write, float xyz) + 24/162
 (per-cell state amortised) = 24.15 B; `achieved` = that x vertices per
@@ -94,15 +94,53 @@ def time_launches(t, hiptime, n, dt):
     return float(np.mean(d)), float(np.min(d))
 
 
+def host_cores():
+    """Physical cores among the CPUs this process may run on (one thread per
+    core), capped by OMP_NUM_THREADS when it is set; and the CPU model."""
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            pkg = open(base + "physical_package_id").read().strip()
+            core = open(base + "core_id").read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", str(c)))
+    n = max(1, len(cores))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return n, len(cpus), model
+
+
 def cpu_baseline(m, steps):
+    """The oracle's float restatement of the reference algorithm, OpenMP over
+    cells with one thread per physical core, threads bound close
+    (OMP_PROC_BIND=close), set before the OpenMP runtime loads."""
+    threads, ncpus, model = host_cores()
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ["OMP_PROC_BIND"] = "close"
+    os.environ["OMP_PLACES"] = "cores"
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
     oracle.run3d(m.X, m.mesh.faces, m.mesh.nv, m.Kv, m.Ka, m.Ks, m.v0, m.a0, m.L, m.Kre, steps, m.dt,
                  mask=15, prec=32, lib="liboracle_omp.so")
     sec = time.perf_counter() - t0
     nvt = m.C * m.mesh.nv
     return {"value": nvt * steps / sec, "unit": "vertex-steps/s", "cores": threads, "kind": "port",
+            "cpu": f"{model}; {ncpus} CPUs available, {threads} physical cores used, OMP_PROC_BIND=close",
             "sample": f"{steps} step(s) of the 256-cell x 162-vertex monolayer, the reference's four "
                       f"terms (volume, surface, substrate, all-pairs winding contact), float, "
                       f"OpenMP over cells: {sec:.2f} s"}
@@ -141,17 +179,21 @@ def plain2d_cpu(p, steps=1000, dt=0.005):
     return {"value": nvt * steps / sec, "unit": "vertex-steps/s", "cores": 1, "kind": "port", "seconds": sec}
 
 
-def measured_traffic_per_vertex_step():
+def measured_traffic_per_vertex_step(config: str):
     """HBM bytes per vertex-step of the batched step kernel from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (None if absent)."""
+    FETCH_SIZE/WRITE_SIZE summary, used only when that summary was taken on the same
+    configuration (its `traffic_config:` line equals `config`); None otherwise."""
     path = os.path.join(ROOT, "profiles", "k3d_wstep_monolayer_64x256x162.md")
+    value, cfg = None, None
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_vertex_step:"):
-                return float(line.split(":", 1)[1])
+                value = float(line.split(":", 1)[1])
+            elif line.startswith("traffic_config:"):
+                cfg = line.split(":", 1)[1].strip()
     except OSError:
-        pass
-    return None
+        return None
+    return value if cfg == config else None
 
 
 def main():
@@ -214,6 +256,7 @@ def main():
     b_kind = args.step_kernel or 2  # the automatic choice at 64 x 256 cells
     bt.set_step_kernel(b_kind)
     tuned = None
+    best = {"form": b_kind, "cells_per_workgroup": 0, "cell_list": -1}
     if not args.no_autotune and args.step_kernel is None:
         # the autotuner on the start state (it restores it), then the
         # warm-up and the timed steps
@@ -229,7 +272,9 @@ def main():
         f"launch {b_launch:.3f} ms)")
     achieved = B_ALG * nvb / (b_launch / 1e3) / 1e9
     windings = float(b_cnt[1]) / max(1, int(b_cnt[0]))
-    tpv = measured_traffic_per_vertex_step()
+    config_key = (f"batch={B} form={best['form']} cells_per_workgroup={best['cells_per_workgroup']} "
+                  f"cell_list={best['cell_list']}")
+    tpv = measured_traffic_per_vertex_step(config_key)
     traffic = None if tpv is None else tpv * nvb
 
     out = {
@@ -259,8 +304,8 @@ def main():
         "winding_evals_per_step": windings,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "k3d_wstep_f32_v162" if b_kind == 2 else "k3d_step_f32", "launch_ms": b_launch,
-                     "bytes_per_vertex_step": B_ALG},
+                     "kernel": "k3d_wstep_f32_v162" if b_kind == 2 else "k3d_step_f32_v162", "launch_ms": b_launch,
+                     "bytes_per_vertex_step": B_ALG, "config": config_key},
     }
     p2, out["reference_2d"] = plain2d_gpu(capi, hiptime, W)
     log(f"[bench] 2D reference case: {out['reference_2d']['ms']:.2f} ms for 1000 steps "

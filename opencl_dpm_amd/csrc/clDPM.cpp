@@ -22,7 +22,7 @@ PYBIND11_MODULE(clDPM, m) {
       .def_readwrite("Ka", &Cell2D::Ka)
       .def_readwrite("Kl", &Cell2D::Kl)
       .def_readwrite("Kb", &Cell2D::Kb)
-      .def_readwrite("Verts", &Cell2D::Verticies)
+      .def_property("Verts", [](const Cell2D& c) { return c.Verticies; }, &Cell2D::SetVerts)
       .def_readwrite("Forces", &Cell2D::Forces)
       // extensions
       .def_readonly("NV", &Cell2D::NV)
@@ -50,7 +50,7 @@ PYBIND11_MODULE(clDPM, m) {
       .def_readwrite("Kv", &Cell3D::Kv)
       .def_readwrite("Ka", &Cell3D::Ka)
       .def_readwrite("Ks", &Cell3D::Ks)
-      .def_readwrite("Verts", &Cell3D::Verts)
+      .def_property("Verts", [](const Cell3D& c) { return c.Verts; }, &Cell3D::SetVerts)
       .def("GetVolume", &Cell3D::GetVolume)
       .def("GetPositions", &Cell3D::GetPositions)
       .def("GetVesselPositions", &Cell3D::GetVesselPositions)

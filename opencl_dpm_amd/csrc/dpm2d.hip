@@ -147,35 +147,39 @@ __device__ __forceinline__ int cell_of(const int* vb, int nc, int k) {
 }
 
 // Per-cell quantities of the current positions: area strain (double area),
-// orientation, centre of mass, axis-aligned bounds.  One thread per cell.
+// orientation, centre of mass, axis-aligned bounds, of tissue-local cell c.
+template <typename R>
+__device__ __forceinline__ void cell_one(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Lds2D<R>& l, int c) {
+  const int b = l.vb[c], e = l.vb[c + 1], n = e - b;
+  double area = 0.0, sx = 0.0, sy = 0.0;
+  R x0 = l.x[b], x1 = l.x[b], y0 = l.y[b], y1 = l.y[b];
+  for (int j = 0; j < n; j++) {
+    const int k = b + j, km = (j == 0) ? (e - 1) : (k - 1);
+    const double xs = (double)l.x[km] + (double)l.x[k];
+    const double yd = (double)l.y[km] - (double)l.y[k];
+    area += xs * yd;
+    sx += (double)l.x[k];
+    sy += (double)l.y[k];
+    x0 = fmin(x0, l.x[k]);
+    x1 = fmax(x1, l.x[k]);
+    y0 = fmin(y0, l.y[k]);
+    y1 = fmax(y1, l.y[k]);
+  }
+  l.strain[c] = (R)(0.5 * fabs(area) / (double)g.a0[T.c0 + c] - 1.0);
+  // the shoelace sum in this order is negative for counter-clockwise cells
+  l.orient[c] = area < 0.0 ? R(-1) : R(1);
+  l.comx[c] = (R)(sx / n);
+  l.comy[c] = (R)(sy / n);
+  l.ax0[c] = x0;
+  l.ax1[c] = x1;
+  l.ay0[c] = y0;
+  l.ay1[c] = y1;
+}
+
+// cell_one for every cell of the tissue, one thread per cell.
 template <typename R>
 __device__ void cell_pass(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Lds2D<R>& l) {
-  for (int c = threadIdx.x; c < T.nc; c += blockDim.x) {
-    const int b = l.vb[c], e = l.vb[c + 1], n = e - b;
-    double area = 0.0, sx = 0.0, sy = 0.0;
-    R x0 = l.x[b], x1 = l.x[b], y0 = l.y[b], y1 = l.y[b];
-    for (int j = 0; j < n; j++) {
-      const int k = b + j, km = (j == 0) ? (e - 1) : (k - 1);
-      const double xs = (double)l.x[km] + (double)l.x[k];
-      const double yd = (double)l.y[km] - (double)l.y[k];
-      area += xs * yd;
-      sx += (double)l.x[k];
-      sy += (double)l.y[k];
-      x0 = fmin(x0, l.x[k]);
-      x1 = fmax(x1, l.x[k]);
-      y0 = fmin(y0, l.y[k]);
-      y1 = fmax(y1, l.y[k]);
-    }
-    l.strain[c] = (R)(0.5 * fabs(area) / (double)g.a0[T.c0 + c] - 1.0);
-    // the shoelace sum in this order is negative for counter-clockwise cells
-    l.orient[c] = area < 0.0 ? R(-1) : R(1);
-    l.comx[c] = (R)(sx / n);
-    l.comy[c] = (R)(sy / n);
-    l.ax0[c] = x0;
-    l.ax1[c] = x1;
-    l.ay0[c] = y0;
-    l.ay1[c] = y1;
-  }
+  for (int c = threadIdx.x; c < T.nc; c += blockDim.x) cell_one(g, T, l, c);
 }
 
 // Cell-level candidate lists: cell cj is a candidate of ci unless every
@@ -221,6 +225,20 @@ __device__ bool maybe_pair(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Ld
   return false;
 }
 
+// The candidate list of tissue-local cell ci, ascending, by one thread.
+template <typename R>
+__device__ __forceinline__ void cand_one(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Lds2D<R>& l, int ci) {
+  int n = 0;
+  for (int cj = 0; cj < T.nc; cj++) {
+    if (cj == ci) continue;
+    if (maybe_pair(g, T, l, ci, cj)) {
+      if (n < DPM2D_MAXCAND) l.cand[ci * DPM2D_MAXCAND + n] = (short)cj;
+      n++;
+    }
+  }
+  l.ncand[ci] = n;  // > MAXCAND: the vertex loops scan every cell
+}
+
 template <typename R>
 __device__ void cand_pass(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Lds2D<R>& l) {
   const int nc = T.nc, NT = blockDim.x;
@@ -251,17 +269,7 @@ __device__ void cand_pass(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Lds
     }
     return;
   }
-  for (int ci = threadIdx.x; ci < nc; ci += NT) {
-    int n = 0;
-    for (int cj = 0; cj < nc; cj++) {
-      if (cj == ci) continue;
-      if (maybe_pair(g, T, l, ci, cj)) {
-        if (n < DPM2D_MAXCAND) l.cand[ci * DPM2D_MAXCAND + n] = (short)cj;
-        n++;
-      }
-    }
-    l.ncand[ci] = n;  // > MAXCAND: the vertex loops scan every cell
-  }
+  for (int ci = threadIdx.x; ci < nc; ci += NT) cand_one(g, T, l, ci);
 }
 
 // True when no vertex of cell cj can lie within `cut` of point p under the
@@ -423,8 +431,8 @@ __device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DC
   const int ncl = scan_all ? T.nc : nc;
   const R half_l = R(0.49f) * L;
   // With Kat = 0 the attraction adds exact zeros: it is evaluated only where
-  // its partner count is asked for (n_attr).
-  if ((mask & M2_ATTRACTION) && (T.Kat != R(0) || n_attr)) {
+  // its partners are asked for (n_attr, attr_key).
+  if ((mask & M2_ATTRACTION) && (T.Kat != R(0) || n_attr || attr_key)) {
     const R l0 = (R)g.l0[gc];
     const R kat = T.Kat / (R)n;
     const R lim = l0 * l0 * R(1.00001f);

@@ -8,6 +8,7 @@
 // double).
 #include "dpm_host.hpp"
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -17,6 +18,11 @@
 #include "dpm.h"
 
 namespace dpmhost {
+
+uint64_t next_stamp() {
+  static std::atomic<uint64_t> counter{0};
+  return ++counter;
+}
 
 void HandleDeleter::operator()(dpm_tissue* h) const { dpm_destroy(h); }
 
@@ -333,11 +339,13 @@ void Tissue2D::Disperse() {
   std::vector<float> r(NCELLS), X, Y;
   for (int i = 0; i < NCELLS; i++) r[i] = cells[i].r0;
   relax_centres(r, 1.0f, L, X, Y, false);
-  for (int i = 0; i < NCELLS; i++)
+  for (int i = 0; i < NCELLS; i++) {
     for (unsigned int j = 0; j < cells[i].NV; j++) {
       cells[i].Verticies[j][0] = cells[i].r0 * (cos(2.0 * M_PI * (j + 1) / cells[i].NV)) + X[i];
       cells[i].Verticies[j][1] = cells[i].r0 * (sin(2.0 * M_PI * (j + 1) / cells[i].NV)) + Y[i];
     }
+    cells[i].verts_stamp = next_stamp();
+  }
 }
 
 void Tissue2D::CLEulerUpdate(int nsteps, float dt) {
@@ -372,6 +380,8 @@ void Tissue2D::CLEulerUpdate(int nsteps, float dt) {
   }
   const int cb[2] = {0, C};
   const int pbc = PBC ? 1 : 0;
+  bool resident = (bool)h_ && h_nv_ == nv && (int)synced_.size() == C;
+  for (int c = 0; c < C && resident; c++) resident = cells[c].verts_stamp == synced_[c];
   if (!h_ || h_nv_ != nv) {
     dpm2d_desc d{};
     d.ntissues = 1;
@@ -400,24 +410,30 @@ void Tissue2D::CLEulerUpdate(int nsteps, float dt) {
   }
   std::vector<float> xy(2 * V);
   size_t k = 0;
-  for (const auto& c : cells)
-    for (unsigned int vi = 0; vi < c.NV; vi++) {
-      xy[k++] = c.Verticies[vi][0];
-      xy[k++] = c.Verticies[vi][1];
-    }
-  check(dpm2d_set_positions(h_.get(), xy.data(), V));
+  if (!resident) {  // the device holds other positions: upload these
+    synced_.clear();
+    for (const auto& c : cells)
+      for (unsigned int vi = 0; vi < c.NV; vi++) {
+        xy[k++] = c.Verticies[vi][0];
+        xy[k++] = c.Verticies[vi][1];
+      }
+    check(dpm2d_set_positions(h_.get(), xy.data(), V));
+  }
   check(dpm_euler_async(h_.get(), nsteps, dt));
   check(dpm_synchronize(h_.get()));
   std::vector<float> f(2 * V);
   check(dpm2d_get_positions(h_.get(), xy.data(), V));
   check(dpm2d_get_forces(h_.get(), f.data(), V));
   k = 0;
-  for (auto& c : cells) {
+  synced_.resize(C);
+  for (int ci = 0; ci < C; ci++) {
+    Cell2D& c = cells[ci];
     if (c.Forces.size() < c.NV) c.Forces.resize(c.NV);
     for (unsigned int vi = 0; vi < c.NV; vi++, k++) {
       c.Verticies[vi] = {xy[2 * k], xy[2 * k + 1]};
       c.Forces[vi] = {f[2 * k], f[2 * k + 1]};
     }
+    c.verts_stamp = synced_[ci] = next_stamp();
   }
 }
 
@@ -446,6 +462,7 @@ void Tissue3D::Disperse2D() {
       Cells[i].Verts[j][0] -= X[i];
       Cells[i].Verts[j][1] -= Y[i];
     }
+    Cells[i].verts_stamp = next_stamp();
   }
 }
 
@@ -483,6 +500,10 @@ void Tissue3D::CLEulerUpdate(int nsteps, float dt) {
   std::vector<float> Kv(C), Ka(C), Ks(C), Kb(C), v0(C), a0(C);
   bool bending = false;
   std::vector<float> xyz((size_t)C * nv * 3);
+  // The device still holds every cell's positions when each cell carries the
+  // stamp of the last write-back (they were checked finite after that call).
+  bool resident = (bool)h_ && h_ncells_ == C && h_nv_ == nv && h_faces_ == faces_u && (int)synced_.size() == C;
+  for (int c = 0; c < C && resident; c++) resident = Cells[c].verts_stamp == synced_[c];
   for (int c = 0; c < C; c++) {
     const Cell3D& cell = Cells[c];
     if (cell.Kv <= 0 || cell.Ka <= 0 || cell.Ks <= 0) {
@@ -503,7 +524,7 @@ void Tissue3D::CLEulerUpdate(int nsteps, float dt) {
     bending = bending || cell.Kb != 0.0f;
     v0[c] = cell.v0;
     a0[c] = cell.a0;
-    for (int v = 0; v < nv; v++)
+    for (int v = 0; v < nv && !resident; v++)
       for (int k = 0; k < 3; k++) {
         const float val = cell.Verts[v][k];
         if (!std::isfinite(val)) {
@@ -550,7 +571,10 @@ void Tissue3D::CLEulerUpdate(int nsteps, float dt) {
                                 a0.data()));
     check(dpm3d_set_tissue_params(h_.get(), &L, &pbc, &Kre, &Kat, &KsGradient, mask));
   }
-  check(dpm3d_set_positions(h_.get(), xyz.data(), (size_t)C * nv));
+  if (!resident) {
+    synced_.clear();
+    check(dpm3d_set_positions(h_.get(), xyz.data(), (size_t)C * nv));
+  }
   const auto t0 = std::chrono::high_resolution_clock::now();
   check(dpm_euler_async(h_.get(), nsteps, dt));
   check(dpm_synchronize(h_.get()));
@@ -584,6 +608,7 @@ void Tissue3D::CLEulerUpdate(int nsteps, float dt) {
       cell.Verts[v] = {xyz[i], xyz[i + 1], xyz[i + 2]};
       cell.Forces[v] = {f[i], f[i + 1], f[i + 2]};
     }
+    cell.verts_stamp = next_stamp();
     cell.Volume = cell.GetVolume();
     cell.SurfaceArea = cell.GetSurfaceArea();
     if (!std::isfinite(cell.Volume) || cell.Volume <= 0)
@@ -592,6 +617,8 @@ void Tissue3D::CLEulerUpdate(int nsteps, float dt) {
       std::cerr << "[WARNING] Invalid surface area for cell " << c << ": " << cell.SurfaceArea
                 << std::endl;
   }
+  synced_.resize(C);
+  for (int c = 0; c < C; c++) synced_[c] = Cells[c].verts_stamp;
 }
 
 }  // namespace dpmhost

@@ -5,6 +5,7 @@
 // nothing here computes forces.
 #pragma once
 #include <array>
+#include <cstdint>
 #include <memory>
 #include <string>
 #include <vector>
@@ -12,6 +13,13 @@
 struct dpm_tissue;
 
 namespace dpmhost {
+
+// A process-wide unique number drawn at every change of a cell's vertex
+// positions (construction, assignment from Python, dispersal, a timestep's
+// write-back).  Two cells with the same stamp hold the same positions, so a
+// tissue whose cells all carry the stamps of its last write-back still has
+// those positions on the device and skips the upload.
+uint64_t next_stamp();
 
 struct HandleDeleter {
   void operator()(dpm_tissue* h) const;
@@ -25,6 +33,11 @@ struct Cell2D {
   float Ka = 0, Kb = 0, Kl = 0, Ks = 0;  // uninitialised in the reference; 0 here
   std::vector<std::array<float, 2>> Verticies;
   std::vector<std::array<float, 2>> Forces;
+  uint64_t verts_stamp = next_stamp();
+  void SetVerts(std::vector<std::array<float, 2>> v) {
+    Verticies = std::move(v);
+    verts_stamp = next_stamp();
+  }
   // The fifth argument is the circumradius (cell.cpp:12-16), although the
   // reference header names it a0.
   Cell2D(float x0, float y0, float calA, unsigned int numVerts, float r0);
@@ -44,6 +57,11 @@ struct Cell3D {
   std::vector<std::array<float, 3>> Verts;
   std::vector<std::array<float, 3>> Forces;
   std::vector<std::array<unsigned int, 3>> Faces;
+  uint64_t verts_stamp = next_stamp();
+  void SetVerts(std::vector<std::array<float, 3>> v) {
+    Verts = std::move(v);
+    verts_stamp = next_stamp();
+  }
 
   Cell3D(std::array<float, 3> starting_point, float calA, float r0, int subdiv = 2);
   float GetVolume() const;
@@ -74,6 +92,7 @@ struct Tissue2D {
  private:
   Handle h_;
   std::vector<int> h_nv_;
+  std::vector<uint64_t> synced_;  // cell stamps of the positions the device holds
 };
 
 // Tissue.hpp:13-33, Tissue3D.cpp.
@@ -98,6 +117,7 @@ struct Tissue3D {
   Handle h_;
   int h_ncells_ = -1, h_nv_ = -1;
   std::vector<unsigned int> h_faces_;
+  std::vector<uint64_t> synced_;  // cell stamps of the positions the device holds
 };
 
 // Throws std::invalid_argument for DPM_EINVAL, std::runtime_error otherwise.

@@ -94,6 +94,40 @@ def main():
     out["t3d_verts"] = verts3.reshape(n3, 162, 3)
     np.savez_compressed(os.path.join(HERE, "setup_goldens.npz"), **out)
     print("wrote", os.path.join(HERE, "setup_goldens.npz"))
+    drivers(L, refdir)
+
+
+def drivers(L, refdir):
+    """The set-up of the reference's own driver scripts, with an unseeded
+    drand48 stream (glibc's default state 0x1234ABCD330E):
+      test2D.py:9-21  [Cell2D(0,0,1.2,25,1.0), Cell2D(0,0,1.2,22,1.3)] * 200,
+                      Tissue2D(cells, 0.9), Disperse();
+      test3D.py:8-15  [Cell3D([0,0,0],1.0,1.0)] * 32, Tissue3D(cells, 0.35),
+                      Disperse2D().
+    Stored: box lengths and the dispersed vertices."""
+    out = {}
+    n2 = 400
+    calA = np.full(n2, 1.2, np.float32)
+    nv = np.array([25, 22] * (n2 // 2), np.uint32)
+    r0 = np.array([1.0, 1.3] * (n2 // 2), np.float32)
+    Lout = np.zeros(1, np.float32)
+    verts = np.zeros(int(nv.sum()) * 2, np.float32)
+    L.refshim_seed48(ctypes.c_ushort(0x330E), ctypes.c_ushort(0xABCD), ctypes.c_ushort(0x1234))
+    rc = L.refshim_tissue2d_disperse(n2, p(calA), p(nv), p(r0), ctypes.c_float(0.9), refdir, p(Lout), p(verts))
+    assert rc == 0, rc
+    out["d2_L"] = Lout.copy()
+    out["d2_verts"] = verts.reshape(-1, 2)
+    n3 = 32
+    calA3 = np.ones(n3, np.float32)
+    r03 = np.ones(n3, np.float32)
+    verts3 = np.zeros(n3 * 162 * 3, np.float32)
+    L.refshim_seed48(ctypes.c_ushort(0x330E), ctypes.c_ushort(0xABCD), ctypes.c_ushort(0x1234))
+    rc = L.refshim_tissue3d_disperse2d(n3, p(calA3), p(r03), ctypes.c_float(0.35), refdir, p(Lout), p(verts3))
+    assert rc == 0, rc
+    out["d3_L"] = Lout.copy()
+    out["d3_verts"] = verts3.reshape(n3, 162, 3)
+    np.savez_compressed(os.path.join(HERE, "driver_goldens.npz"), **out)
+    print("wrote", os.path.join(HERE, "driver_goldens.npz"))
 
 
 if __name__ == "__main__":

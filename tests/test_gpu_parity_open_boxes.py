@@ -67,16 +67,17 @@ def check(g, r32, r64, what):
 
 def coverage(m, X):
     """Candidate cell pairs (centres within R_i + R_j + 2 l0) that need no
-    periodic image by the kernels' test, and those among them whose nearest
-    vertices are at least 2 l0 apart (adhesion pairs a separation bound may
-    skip)."""
+    periodic image by the kernels' test, and those among them whose
+    centre-line separation (the kernels' bound D - h_i - h_j, h the largest
+    extent of each cell toward the other) is at least 2 l0: the pairs whose
+    adhesion tests the step skips."""
     nv = m.mesh.nv
     P = np.asarray(X, np.float64).reshape(m.C, nv, 3)
     C = P.mean(1)
     R = np.sqrt(((P - C[:, None]) ** 2).sum(-1)).max(1)
     l0 = np.sqrt(4 * np.asarray(np.broadcast_to(m.a0, (m.C,)), np.float64)) / math.sqrt(3)
     L = float(m.L)
-    no_image = far = 0
+    no_image = skip = 0
     for i in range(m.C):
         for j in range(m.C):
             if i == j:
@@ -84,14 +85,16 @@ def coverage(m, X):
             d = C[j] - C[i]
             d -= L * np.round(d / L)
             l0m = max(l0[i], l0[j])
-            if np.linalg.norm(d) > R[i] + R[j] + 2 * l0m:
+            D = float(np.linalg.norm(d))
+            if D > R[i] + R[j] + 2 * l0m:
                 continue
             if np.all(np.abs(C[i] - C[j]) + (R[i] + R[j]) < 0.48 * L):
                 no_image += 1
-                q = P[j] - np.round((P[j] - P[i].mean(0)) / L) * L
-                dmin = np.sqrt(((P[i][:, None, :] - q[None, :, :]) ** 2).sum(-1)).min()
-                far += dmin >= 2 * l0m * 1.05
-    return no_image, far
+                n = d / D
+                hi = float(((P[i] - C[i]) @ n).max())
+                hj = float(((C[j] - P[j]) @ n).max())
+                skip += (D - hi - hj) - 1e-4 * (R[i] + R[j] + D) >= 2 * l0m * 1.001
+    return no_image, skip
 
 
 def expected_bins(m, X):
@@ -119,18 +122,27 @@ OPEN_BOXES = {
 
 
 @pytest.mark.parametrize("case", list(OPEN_BOXES))
-def test_open_box_start_and_after_200_steps(case):
+def test_open_box_start_and_after_200_and_2000_steps(case):
+    # as the cells spread on the substrate their bounding radii grow and
+    # diagonal neighbours become candidates whose surfaces stay apart: the
+    # separation skip appears along the trajectory
     n, sp, seed = OPEN_BOXES[case]
     m = W.monolayer3d(n, sp, seed=seed)
     run = tissue3d(m)
     run.set_positions(m.X)
+    states = {"start": m.X}
     run.euler(200, 0.002)
-    states = {"start": m.X, "after 200 steps": run.positions()}
+    states["after 200 steps"] = run.positions()
+    run.euler(1800, 0.002)
+    states["after 2000 steps"] = run.positions()
+    skipped = adhesion = contact = 0
     for label, X in states.items():
-        ni, far = coverage(m, X)
+        ni, sk = coverage(m, X)
         nb = expected_bins(m, X)
-        log(f"{case} {label}: candidate pairs without image {ni}, of which beyond 2 l0 {far}; bins {nb}")
-        assert ni > 0 and far > 0 and nb >= 3
+        skipped += sk
+        log(f"{case} {label}: candidate pairs without image {ni}, adhesion tests skipped by separation {sk}; "
+            f"bins {nb}")
+        assert ni > 0 and nb >= 3
         r32, r64 = oracle_pair(m, X)
         for kernel in (1, 2):
             for cl in (0, 1):
@@ -139,9 +151,13 @@ def test_open_box_start_and_after_200_steps(case):
                 t.set_cell_list(cl)
                 t.set_positions(X)
                 check(t.probe(), r32, r64, f"{case} {label} kernel={kernel} cell_list={cl}")
-        assert r32["adhesion"].sum() > 0
-        if sp < 2.4:
-            assert r32["contact"].sum() > 0
+        adhesion += int(r32["adhesion"].sum())
+        contact += int(r32["contact"].sum())
+    # along the trajectory: adhesion pairs, the separation skip on pairs the
+    # oracle checked, and contacts where the cells start squeezed
+    assert adhesion > 0 and skipped > 0
+    if sp < 2.4:
+        assert contact > 0
 
 
 @pytest.mark.timeout(1200)
@@ -153,9 +169,10 @@ def test_headline_monolayer_start_and_after_200_steps():
     run.set_positions(m.X)
     run.euler(200, m.dt)
     for label, X in (("start", m.X), ("after 200 steps", run.positions())):
-        ni, far = coverage(m, X)
-        log(f"cfg2 {label}: candidate pairs without image {ni}, beyond 2 l0 {far}; bins {expected_bins(m, X)}")
-        assert ni > 0 and far > 0
+        ni, sk = coverage(m, X)
+        log(f"cfg2 {label}: candidate pairs without image {ni}, adhesion tests skipped by separation {sk}; "
+            f"bins {expected_bins(m, X)}")
+        assert ni > 0
         r32, r64 = oracle_pair(m, X, lib="liboracle_omp.so")
         for kernel in (1, 2):
             for cl in (0, 1):

@@ -213,3 +213,25 @@ def test_longer_verts_list_is_packed_to_nv():
     with pytest.raises(RuntimeError):
         T.CLEulerUpdate(3, 0.005)
     assert len(T.Cells[0].Verts) == 20
+
+
+def test_reference_driver_setups_match_reference_host_code():
+    # test2D.py:9-21 and test3D.py:8-15 with an unseeded drand48 stream
+    # (glibc's default state): the host dispersal reproduces the reference's
+    # own host code bit for bit (tests/golden/driver_goldens.npz)
+    g = np.load(os.path.join(ROOT, "tests", "golden", "driver_goldens.npz"))
+    c = clDPM.Cell2D(0, 0, 1.2, 25, 1.0)
+    c2 = clDPM.Cell2D(0, 0, 1.2, 22, 1.3)
+    T = clDPM.Tissue2D([c, c2] * 200, 0.9)
+    assert T.L == g["d2_L"][0] and T.NCELLS == 400
+    clDPM.seed48(0x330E, 0xABCD, 0x1234)
+    T.Disperse()
+    v = np.concatenate([np.asarray(x.Verts, np.float32) for x in T.Cells])
+    np.testing.assert_array_equal(bits(v), bits(g["d2_verts"]))
+    c3 = clDPM.Cell3D([0.0] * 3, 1.0, 1.0)
+    T3 = clDPM.Tissue3D([c3] * 32, 0.35)
+    assert T3.L == g["d3_L"][0]
+    clDPM.seed48(0x330E, 0xABCD, 0x1234)
+    T3.Disperse2D()
+    v3 = np.stack([np.asarray(x.Verts, np.float32) for x in T3.Cells])
+    np.testing.assert_array_equal(bits(v3), bits(g["d3_verts"]))
