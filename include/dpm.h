@@ -149,6 +149,19 @@ int dpm3d_probe(dpm_tissue* h, float* forces, unsigned char* contact,
  * be NULL. */
 int dpm3d_probe_adhesion(dpm_tissue* h, unsigned* own_keys, unsigned* rev_keys);
 
+/* dpm3d_probe without device allocation (no reference counterpart; for
+ * callers that cannot allocate device memory: tight serving loops, harnesses
+ * that forbid hipMalloc).  Same outputs as dpm3d_probe, bit for bit; the
+ * device tables live in a 64 MiB array of the library that every device
+ * holds from the library's load until the process ends.  Fails with
+ * DPM_ERANGE, before any device work, when the requested tables (contact and
+ * winding nverts * ncmax bytes and floats, down ncells * nf bytes, adhesion
+ * nverts ints, forces 3 * nverts floats, each rounded up to 256 bytes)
+ * exceed 64 MiB.  Every call on a device shares that device's array: calls
+ * that overlap on one device (other threads, other handles) race. */
+int dpm3d_probe_noalloc(dpm_tissue* h, float* forces, unsigned char* contact,
+                        float* winding, unsigned char* down, int* adhesion);
+
 /* Full double precision.  dpm3d_create_f64 makes a handle whose positions,
  * forces and per-cell statistics are double on the device and whose step
  * evaluates every formula in double (the float thresholds of the reference

@@ -33,7 +33,7 @@ EXPORTS = [
     "dpm_last_error", "dpm_version", "dpm_device_count",
     "dpm3d_create", "dpm3d_set_positions", "dpm3d_get_positions", "dpm3d_get_forces",
     "dpm3d_set_cell_params", "dpm3d_set_tissue_params", "dpm3d_cell_stats", "dpm3d_probe",
-    "dpm3d_probe_adhesion", "dpm2d_probe_attraction",
+    "dpm3d_probe_adhesion", "dpm2d_probe_attraction", "dpm3d_probe_noalloc",
     "dpm3d_create_f64", "dpm3d_set_positions_f64", "dpm3d_get_positions_f64",
     "dpm3d_get_forces_f64", "dpm3d_energy", "dpm3d_trajectory", "dpm3d_adjoint", "dpm3d_adjoint_f64",
     "dpm3d_set_compensated", "dpm3d_set_cell_list", "dpm3d_set_step_kernel", "dpm3d_autotune",
@@ -109,6 +109,7 @@ def load() -> ctypes.CDLL:
     lib.dpm3d_probe.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.dpm2d_probe.argtypes = [vp, vp, vp, vp]
     lib.dpm3d_probe_adhesion.argtypes = [vp, vp, vp]
+    lib.dpm3d_probe_noalloc.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.dpm2d_probe_attraction.argtypes = [vp, vp]
     _lib = lib
     return lib
@@ -384,6 +385,19 @@ class Tissues3D(_Handle):
         _check(load().dpm3d_probe_adhesion(self._h, _p(own), _p(rev)))
         out["adhesion_hash"] = np.stack([own, rev], 1)
         return out
+
+
+    def probe_noalloc(self, winding: bool = False, contact: bool = True):
+        """probe() through dpm3d_probe_noalloc (device tables in the library's
+        static scratch, no device allocation): forces as float, pair tables."""
+        ncmax = int(np.max(np.diff(self.cell_begin)))
+        f = np.zeros((self.nverts, 3), np.float32)
+        ct = np.zeros((self.nverts, ncmax), np.uint8) if contact else None
+        w = np.zeros((self.nverts, ncmax), np.float32) if winding else None
+        down = np.zeros((self.C, self.nf), np.uint8)
+        adh = np.zeros(self.nverts, np.int32)
+        _check(load().dpm3d_probe_noalloc(self._h, _p(f), _p(ct), _p(w), _p(down), _p(adh)))
+        return {"forces": f, "contact": ct, "winding": w, "down": down, "adhesion": adh}
 
 
 class Tissues2D(_Handle):

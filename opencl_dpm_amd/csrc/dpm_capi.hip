@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1134,6 +1135,118 @@ extern "C" int dpm3d_probe_adhesion(dpm_tissue* h, unsigned* own_keys, unsigned*
     if (own_keys) own_keys[i] = k[2 * i];
     if (rev_keys) rev_keys[i] = k[2 * i + 1];
   }
+  return DPM_OK;
+}
+
+// --------------------------------------------- probe without allocation
+// dpm3d_probe_noalloc keeps its device tables (pair flags, winding numbers,
+// facing-down flags, adhesion counts, the AoS force copy) in this array
+// instead of allocating them per call.  Each device holds its own copy at
+// its own address, resolved once per device by hipGetSymbolAddress; host
+// code uses only that address, never the array's name.  Every call on a
+// device shares the device's copy (calls that overlap on one device race).
+constexpr size_t kProbeScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kProbeScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(int dev, unsigned char** out) {
+  *out = nullptr;
+  if (dev < 0 || dev >= kMaxDevices) return fail(DPM_ERANGE, "device index beyond the probe scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int prev = 0;
+    HIPCHECK(hipGetDevice(&prev));
+    HIPCHECK(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess || !p)
+      return fail(DPM_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  return DPM_OK;
+}
+}  // namespace
+
+__global__ void k_fill_zero(unsigned* p, size_t nwords) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+template <typename R>
+__global__ void k_soa_to_aos_f32(const R* x, const R* y, const R* z, float* out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    out[3 * i] = (float)x[i];
+    out[3 * i + 1] = (float)y[i];
+    out[3 * i + 2] = (float)z[i];
+  }
+}
+
+extern "C" int dpm3d_probe_noalloc(dpm_tissue* h, float* forces, unsigned char* contact, float* winding,
+                                   unsigned char* down, int* adhesion) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  const size_t NVT = (size_t)h->C * h->nv;
+  const size_t npair = NVT * (size_t)h->ncmax;
+  auto al = [](size_t b) { return (b + 255u) & ~(size_t)255u; };
+  // table offsets in the scratch, each 256-byte aligned and a multiple of 4 bytes
+  const size_t o_contact = 0;
+  const size_t o_winding = o_contact + (contact ? al(npair) : 0);
+  const size_t o_down = o_winding + (winding ? al(npair * sizeof(float)) : 0);
+  const size_t o_adh = o_down + (down ? al((size_t)h->C * h->nf) : 0);
+  const size_t o_forces = o_adh + (adhesion ? al(NVT * sizeof(int)) : 0);
+  const size_t total = o_forces + (forces ? al(3 * NVT * sizeof(float)) : 0);
+  if (total > kProbeScratchBytes) {
+    char buf[200];
+    snprintf(buf, sizeof(buf),
+             "dpm3d_probe_noalloc: the requested tables need %zu bytes, more than its %zu-byte device scratch "
+             "(use dpm3d_probe, or request fewer tables)",
+             total, kProbeScratchBytes);
+    return fail(DPM_ERANGE, buf);
+  }
+  unsigned char* base = nullptr;
+  TRY(scratch_address(h->device, &base));
+  hipStream_t s = h->stream;
+  unsigned char* dc = contact ? base + o_contact : nullptr;
+  float* dw = winding ? reinterpret_cast<float*>(base + o_winding) : nullptr;
+  unsigned char* dd = down ? base + o_down : nullptr;
+  int* da = adhesion ? reinterpret_cast<int*>(base + o_adh) : nullptr;
+  float* df = forces ? reinterpret_cast<float*>(base + o_forces) : nullptr;
+  if (o_forces > 0) {  // the tables start at zero (the array keeps the last call's contents)
+    const size_t words = o_forces / 4;
+    k_fill_zero<<<(int)std::min<size_t>((words + 255) / 256, 4096), 256, 0, s>>>(reinterpret_cast<unsigned*>(base),
+                                                                                 words);
+    HIPCHECK(hipGetLastError());
+  }
+  TRY(ensure_stats(h));
+  TRY(launch3d(h, K_STEP, h->cur, 0.0f, F_WRITE_FORCES, [&](auto& a) {
+    a.counters = nullptr;
+    a.p_contact = dc;
+    a.p_winding = dw;
+    a.p_down = dd;
+    a.p_adh = da;
+  }));
+  h->forces_valid = true;
+  if (df) {
+    const int blocks = (int)std::min<size_t>((NVT + 255) / 256, 4096);
+    if (h->rs == 8)
+      k_soa_to_aos_f32<double><<<blocks, 256, 0, s>>>((const double*)h->fx, (const double*)h->fy,
+                                                      (const double*)h->fz, df, NVT);
+    else
+      k_soa_to_aos_f32<float><<<blocks, 256, 0, s>>>((const float*)h->fx, (const float*)h->fy,
+                                                     (const float*)h->fz, df, NVT);
+    HIPCHECK(hipGetLastError());
+  }
+  if (dc) HIPCHECK(hipMemcpyAsync(contact, dc, npair, hipMemcpyDeviceToHost, s));
+  if (dw) HIPCHECK(hipMemcpyAsync(winding, dw, npair * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (dd) HIPCHECK(hipMemcpyAsync(down, dd, (size_t)h->C * h->nf, hipMemcpyDeviceToHost, s));
+  if (da) HIPCHECK(hipMemcpyAsync(adhesion, da, NVT * sizeof(int), hipMemcpyDeviceToHost, s));
+  if (df) HIPCHECK(hipMemcpyAsync(forces, df, 3 * NVT * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
   return DPM_OK;
 }
 
