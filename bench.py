@@ -20,8 +20,13 @@ plain 256-cell tissue runs k3d_step_f32_v162, one workgroup per cell, which fill
 the device better at that size).  Algorithmic
 bytes per vertex-step = 24 (position read + write, float xyz) + 24/162
 (per-cell state amortised) = 24.15 B; `achieved` = that x vertices per
-launch / the average launch duration measured here with HIP events
-bracketing single launches on the library's own stream.
+launch / the average launch duration: a step is one launch of that kernel
+and nothing else, launched back to back, so the timed region's per-step time
+(HIP events on the library's own stream) is the launch duration with the
+launch gaps included (an upper bound on it).  `traffic` = measured HBM bytes
+per launch of the steps that do not write the forces (the last step of a
+call also writes them) from the committed rocprofv3 summary, when that
+summary was taken on the same form and cell-list choice.
 
 cpu_baseline: the reference algorithm restated (oracle ref32: all-pairs,
 serial float, the four terms the reference launches), OpenMP over cells on
@@ -79,19 +84,6 @@ def time_steps(t, hiptime, steps, warmup, dt):
     ms = e0.elapsed_ms(e1)
     wall = time.perf_counter() - wall0
     return ms, wall, t.counters()
-
-
-def time_launches(t, hiptime, n, dt):
-    """Average duration of single k3d_step launches, events around each."""
-    s = t.stream()
-    evs = [(hiptime.Event(), hiptime.Event()) for _ in range(n)]
-    for a, b in evs:
-        a.record(s)
-        t.euler_async(1, dt)
-        b.record(s)
-    t.synchronize()
-    d = [a.elapsed_ms(b) for a, b in evs]
-    return float(np.mean(d)), float(np.min(d))
 
 
 def host_cores():
@@ -202,8 +194,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--plain-steps", type=int, default=1000)
-    ap.add_argument("--launches", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=None,
+                    help="steps timed on the plain tissue (default: --steps, the batched window)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
@@ -239,14 +231,14 @@ def main():
     m = W.monolayer_config("cfg2", seed=0)
     nvt = m.C * m.mesh.nv
 
+    if args.plain_steps is None:
+        args.plain_steps = args.steps  # the same window of the trajectory as the batched rate
     log(f"[bench] plain: {m.C} cells x {m.mesh.nv} vertices, {args.plain_steps} steps")
     plain = build_tissue(capi, [m])
     plain.set_step_kernel(args.step_kernel or 0)
     p_ms, p_wall, p_cnt = time_steps(plain, hiptime, args.plain_steps, args.warmup, dt)
-    p_launch, p_launch_min = time_launches(plain, hiptime, args.launches, dt)
     plain_rate = nvt * args.plain_steps / (p_ms / 1e3)
-    log(f"[bench] plain {plain_rate:.4g} vertex-steps/s ({p_ms / args.plain_steps * 1e3:.1f} us/step, "
-        f"launch {p_launch * 1e3:.1f} us)")
+    log(f"[bench] plain {plain_rate:.4g} vertex-steps/s ({p_ms / args.plain_steps * 1e3:.1f} us/step)")
     del plain
 
     B = args.batch
@@ -265,15 +257,15 @@ def main():
         tuned = {"choice": best, "ms_per_step": times}
         log(f"[bench] autotune: {best}")
     b_ms, b_wall, b_cnt = time_steps(bt, hiptime, args.steps, args.warmup, dt)
-    b_launch, b_launch_min = time_launches(bt, hiptime, args.launches, dt)
+    b_launch = b_ms / args.steps  # one launch per step (see the module docstring)
     nvb = nvt * B
     rate = nvb * args.steps / (b_ms / 1e3)
-    log(f"[bench] batched {rate:.4g} vertex-steps/s ({b_ms / args.steps:.3f} ms/step, "
-        f"launch {b_launch:.3f} ms)")
+    log(f"[bench] batched {rate:.4g} vertex-steps/s ({b_launch:.3f} ms/step)")
     achieved = B_ALG * nvb / (b_launch / 1e3) / 1e9
     windings = float(b_cnt[1]) / max(1, int(b_cnt[0]))
-    config_key = (f"batch={B} form={best['form']} cells_per_workgroup={best['cells_per_workgroup']} "
-                  f"cell_list={best['cell_list']}")
+    # the measured traffic per vertex-step does not depend on the cells per
+    # workgroup (within 0.2 % in the profile); the key names what it does
+    config_key = f"batch={B} form={best['form']} cell_list={best['cell_list']}"
     tpv = measured_traffic_per_vertex_step(config_key)
     traffic = None if tpv is None else tpv * nvb
 
@@ -297,7 +289,7 @@ def main():
             "vertices": nvb, "per_cell_sums": "f64",
         },
         "plain": {"value": plain_rate, "unit": "vertex-steps/s", "steps": args.plain_steps,
-                  "ms_per_step": p_ms / args.plain_steps, "launch_ms": p_launch},
+                  "ms_per_step": p_ms / args.plain_steps},
         "batched_vs_plain": rate / plain_rate,
         "autotune": tuned,
         "hbm_gbps_algorithmic": B_ALG * rate / 1e9,

@@ -148,6 +148,13 @@ int dpm3d_probe(dpm_tissue* h, float* forces, unsigned char* contact,
  * (the pairs whose reaction lands on gv, Cell3D_Kernel.cl:359).  Either may
  * be NULL. */
 int dpm3d_probe_adhesion(dpm_tissue* h, unsigned* own_keys, unsigned* rev_keys);
+/* Diagnostic: the contact pair set of one force evaluation in compact form,
+ * for tissues too large for dpm3d_probe's vertex x cell table: per vertex
+ * the number of cells it is in contact with (|w| >= 1e-6,
+ * Cell3D_Kernel.cl:303) and the wrapping 32-bit sum of those cells' keys
+ * f(c + 0x9E3779B9), c the cell's index within its tissue and f the
+ * finaliser of dpm3d_probe_adhesion.  Either may be NULL. */
+int dpm3d_probe_contact(dpm_tissue* h, unsigned* count, unsigned* keys);
 
 /* dpm3d_probe without device allocation (no reference counterpart; for
  * callers that cannot allocate device memory: tight serving loops, harnesses
@@ -161,6 +168,16 @@ int dpm3d_probe_adhesion(dpm_tissue* h, unsigned* own_keys, unsigned* rev_keys);
  * that overlap on one device (other threads, other handles) race. */
 int dpm3d_probe_noalloc(dpm_tissue* h, float* forces, unsigned char* contact,
                         float* winding, unsigned char* down, int* adhesion);
+
+/* The reference's host reductions Cell3D::GetVolume, GetSurfaceArea and
+ * GetCOM (cell.cpp:196-230, 276-288) of every cell at the current
+ * positions, evaluated on the device in the host functions' float operation
+ * order (bit-identical to them; Tissue3D::CLEulerUpdate recomputes volume and
+ * surface area on the host after every call, Tissue3D.cpp:503-504).  The
+ * surface area is the host function's: the sum of |cross| over faces, twice
+ * the triangulated area.  volume[ncells], surface_area[ncells],
+ * com[ncells*3]; any may be NULL. */
+int dpm3d_cell_measures(dpm_tissue* h, float* volume, float* surface_area, float* com);
 
 /* Full double precision.  dpm3d_create_f64 makes a handle whose positions,
  * forces and per-cell statistics are double on the device and whose step
@@ -287,6 +304,17 @@ int dpm2d_probe(dpm_tissue* h, float* forces, int* overlap, int* attraction);
  * dpm3d_probe_adhesion).  Float tissues. */
 int dpm2d_probe_attraction(dpm_tissue* h, unsigned* keys);
 
+/* Step form of the 2D tissues (no reference counterpart; the reference
+ * launches its 2D kernels over an NDRange of NCELLS x maxNV work-items,
+ * Tissue2D.cpp:212): 1 one workgroup per tissue, the whole call in one launch
+ * with the tissue in LDS; 2 the wide form, three launches per step over the
+ * whole device on global memory (any tissue size, float tissues); 0
+ * (default) the first where every tissue fits one workgroup, else the
+ * second.  Both make the same decisions and forces bit for bit.  FIRE,
+ * jamming, energies and double tissues need tissues that fit one
+ * workgroup. */
+int dpm2d_set_step_kernel(dpm_tissue* h, int kind);
+
 /* Double-precision 2D tissue (positions, forces, velocities, box in double;
  * the same formulas), as dpm3d_create_f64.  Its Euler steps, FIRE and
  * jamming all run in double. */
@@ -369,6 +397,20 @@ int dpm2d_jam_step(dpm_tissue* h, const dpm_jam_params* p, int start, int stages
 int dpm2d_set_relax_threads(dpm_tissue* h, int threads);
 int dpm2d_relax_threads(dpm_tissue* h);
 int dpm2d_autotune(dpm_tissue* h, int iters, int* best, float* times_ms);
+
+/* ---------------------------------------------------------- dispersal ---
+ * The soft-disk relaxation of Tissue2D::Disperse and Tissue3D::Disperse2D
+ * (Tissue2D.cpp:42-105, Tissue3D.cpp:49-106) on the device, bit-identical to
+ * the reference's host loop: n centres (X, Y: in the starting centres, which
+ * the reference draws from drand48; out the relaxed ones) with contact radii
+ * r (the reference's radius times its scale, 1 in 2D and 2 in 3D) in a
+ * periodic square box of side L; pairs closer than r_i + r_j push apart,
+ * x += 0.01 F per iteration, until the float energy changes by at most 1e-6
+ * or after max_count + 1 iterations (the reference: max_count = 100000).
+ * result (may be NULL): [0] iterations run, [1] 1 if stopped by max_count.
+ * One workgroup, candidates from a periodic bin grid; n <= 65535 and the
+ * centres must fit one workgroup's LDS (about 4,000 centres). */
+int dpm_disperse(const float* r, int n, float L, float* X, float* Y, int max_count, int* result);
 
 /* ------------------------------------------------------------- common ---
  * Overdamped forward Euler, x += F*dt, nsteps times (the step loops of

@@ -33,7 +33,8 @@ EXPORTS = [
     "dpm_last_error", "dpm_version", "dpm_device_count",
     "dpm3d_create", "dpm3d_set_positions", "dpm3d_get_positions", "dpm3d_get_forces",
     "dpm3d_set_cell_params", "dpm3d_set_tissue_params", "dpm3d_cell_stats", "dpm3d_probe",
-    "dpm3d_probe_adhesion", "dpm2d_probe_attraction", "dpm3d_probe_noalloc",
+    "dpm3d_probe_adhesion", "dpm2d_probe_attraction", "dpm3d_probe_noalloc", "dpm3d_cell_measures",
+    "dpm_disperse", "dpm2d_set_step_kernel", "dpm3d_probe_contact",
     "dpm3d_create_f64", "dpm3d_set_positions_f64", "dpm3d_get_positions_f64",
     "dpm3d_get_forces_f64", "dpm3d_energy", "dpm3d_trajectory", "dpm3d_adjoint", "dpm3d_adjoint_f64",
     "dpm3d_set_compensated", "dpm3d_set_cell_list", "dpm3d_set_step_kernel", "dpm3d_autotune",
@@ -110,6 +111,10 @@ def load() -> ctypes.CDLL:
     lib.dpm2d_probe.argtypes = [vp, vp, vp, vp]
     lib.dpm3d_probe_adhesion.argtypes = [vp, vp, vp]
     lib.dpm3d_probe_noalloc.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.dpm3d_cell_measures.argtypes = [vp, vp, vp, vp]
+    lib.dpm_disperse.argtypes = [vp, i, fp, vp, vp, i, vp]
+    lib.dpm2d_set_step_kernel.argtypes = [vp, i]
+    lib.dpm3d_probe_contact.argtypes = [vp, vp, vp]
     lib.dpm2d_probe_attraction.argtypes = [vp, vp]
     _lib = lib
     return lib
@@ -364,6 +369,15 @@ class Tissues3D(_Handle):
         _check(load().dpm3d_cell_stats(self._h, _p(vol), _p(com)))
         return vol, com
 
+    def cell_measures(self):
+        """Per cell (volume, surface area, centre of mass) by the reference's
+        host reductions evaluated on the device (dpm3d_cell_measures)."""
+        vol = np.empty(self.C, np.float32)
+        sa = np.empty(self.C, np.float32)
+        com = np.empty((self.C, 3), np.float32)
+        _check(load().dpm3d_cell_measures(self._h, _p(vol), _p(sa), _p(com)))
+        return vol, sa, com
+
     def eval_forces(self, dtype=None) -> np.ndarray:
         """Forces at the current positions, positions unchanged (dpm3d_probe
         with no pair tables), in the handle's precision or `dtype`."""
@@ -386,6 +400,14 @@ class Tissues3D(_Handle):
         out["adhesion_hash"] = np.stack([own, rev], 1)
         return out
 
+
+    def probe_contact(self):
+        """Per vertex the number of contact partner cells and the key sum of
+        their indices (dpm3d_probe_contact); forces as probe()."""
+        n = np.zeros(self.nverts, np.uint32)
+        k = np.zeros(self.nverts, np.uint32)
+        _check(load().dpm3d_probe_contact(self._h, _p(n), _p(k)))
+        return n, k
 
     def probe_noalloc(self, winding: bool = False, contact: bool = True):
         """probe() through dpm3d_probe_noalloc (device tables in the library's
@@ -500,6 +522,10 @@ class Tissues2D(_Handle):
     def relax_threads(self) -> int:
         return int(load().dpm2d_relax_threads(self._h))
 
+    def set_step_kernel(self, kind: int) -> None:
+        """0 automatic, 1 one workgroup per tissue, 2 the wide form."""
+        _check(load().dpm2d_set_step_kernel(self._h, int(kind)))
+
     def autotune(self, iters: int = 50):
         best = ctypes.c_int(0)
         times = np.zeros(5, np.float32)
@@ -514,3 +540,15 @@ class Tissues2D(_Handle):
         keys = np.zeros(self.V, np.uint32)
         _check(load().dpm2d_probe_attraction(self._h, _p(keys)))
         return {"forces": f, "overlap": ov, "attraction": at, "attraction_hash": keys}
+
+
+def disperse(r, L, X, Y, max_count: int = 100000):
+    """dpm_disperse: the reference's dispersal relaxation on the device.
+    Returns the relaxed centres and (iterations, capped)."""
+    r = _f32(r)
+    n = len(r)
+    X = np.array(X, np.float32)
+    Y = np.array(Y, np.float32)
+    res = np.zeros(2, np.int32)
+    _check(load().dpm_disperse(_p(r), int(n), float(L), _p(X), _p(Y), int(max_count), _p(res)))
+    return X, Y, (int(res[0]), bool(res[1]))

@@ -110,6 +110,9 @@ struct Step3D {
   // per vertex: key sums (vertex_key) of the adhesion partners its own loop
   // accepts, then of the partners whose loop accepts it
   unsigned* p_adh_hash;
+  // per vertex: number of contact partner cells, key sum of their indices
+  // within the tissue (cell_key)
+  unsigned* p_contact_key;
   int ncmax;
   double* energy;  // k3d_energy output, per cell
   R* w_out;        // per vertex: sum of |w| over its contact partners (nullptr: not recorded)
@@ -233,8 +236,11 @@ __device__ __forceinline__ bool facing_down(V3<R> n) {
 // them, so they share their bytes with the tables of the later passes
 // (bending, neighbour cell, winding directions).
 struct Lds3DLayout {
-  size_t own, faces, clist, plist, glist, gath, cnt, red, nbr, bend, wnum, utab, fc, total;
+  size_t own, faces, clist, plist, glist, gath, cnt, red, cst, nbr, bend, wnum, utab, fc, total;
 };
+// Statistics of the first kCandStats candidates of a scan chunk are kept in
+// LDS (the rest are read from global memory).
+constexpr int kCandStats = 64;
 __host__ __device__ inline Lds3DLayout lds3d_layout(int nv, int nf, int nt, int ww, int rsize) {
   Lds3DLayout L;
   const size_t rs = (size_t)rsize;
@@ -249,7 +255,9 @@ __host__ __device__ inline Lds3DLayout lds3d_layout(int nv, int nf, int nt, int 
   L.gath = o;    o += 4u * 96u;         // 27 buckets: ids, starts, prefix
   L.cnt = o;     o += 4u * 48u;
   o = (o + 7u) & ~(size_t)7u;
-  L.red = o;     o += 8u * 64u;
+  L.red = o;     o += 8u * 192u;  // three 64-double regions (cell_stats alternates them)
+  o = (o + 31u) & ~(size_t)31u;
+  L.cst = o;     o += 8u * rs * (size_t)kCandStats;  // CellStat<R> of the candidates
   o = (o + 31u) & ~(size_t)31u;
   L.fc = o;  // union: shape pass | later passes
   const size_t fc_end = o + 9u * rs * (size_t)nf;
@@ -270,6 +278,7 @@ template <typename R>
 struct Lds3D {
   R *sx, *sy, *sz, *nx, *ny, *nz, *sb, *wb, *fcx, *fcy, *fcz;
   Q4<R>* utab;
+  CellStat<R>* cst;
   uint2* sf;
   int *clist, *plist, *glist, *gath, *cnt;
   double* red;
@@ -299,6 +308,7 @@ __device__ __forceinline__ Lds3D<R> carve3d(unsigned char* smem, int nv, int nf,
   l.fcy = l.fcx + 3 * nf;
   l.fcz = l.fcy + 3 * nf;
   l.red = reinterpret_cast<double*>(smem + L.red);
+  l.cst = reinterpret_cast<CellStat<R>*>(smem + L.cst);
   return l;
 }
 
@@ -340,7 +350,7 @@ __device__ void cell_stats(const Lds3D<R>& l, int nv, int nf, CellStat<R>* st_ou
     py += (double)l.sy[v];
     pz += (double)l.sz[v];
   }
-  block_sum3(px, py, pz, l.red);
+  block_sum3_1b(px, py, pz, l.red + 64);  // the two reductions alternate scratch regions
   const V3<R> C = {(R)(px / nv), (R)(py / nv), (R)(pz / nv)};
   R r = R(0);
   for (int v = tid; v < nv; v += NT) {
@@ -362,7 +372,7 @@ __device__ void cell_stats(const Lds3D<R>& l, int nv, int nf, CellStat<R>* st_ou
   {  // the radius, volume and longest edge in one reduction (max of
      // non-negative floats is exact in double)
     double rd = (double)r, e2d = (double)e2;
-    block_max_sum_max(rd, vol, e2d, l.red);
+    block_max_sum_max_1b(rd, vol, e2d, l.red + 128);
     r = (R)rd;
     e2 = (R)e2d;
   }
@@ -891,8 +901,9 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
       const int k = c0 + tid;
       const int cj = k < nsrc ? (src ? src[k] : cb + k) : ce;
       bool cand = false;
+      CellStat<R> stj{};
       if (cj < ce && cj != ci) {
-        const CellStat<R> stj = a.st_in[cj];
+        stj = a.st_in[cj];
         V3<R> sh = zero;
         if (pbc)
           sh = v3(image_shift(Ci.x - stj.cx, L, invL), image_shift(Ci.y - stj.cy, L, invL),
@@ -912,46 +923,42 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
         const R rr = reach * R(1.0001f) + R(1e-6f);
         cand = cand || vdot(d, d) <= rr * rr;
       }
-      const int ncand = block_compact(cand, cj, l.clist, l.cnt);
+      const int ncand = block_compact_with(cand, cj, l.clist, l.cnt, [&](int slot) {
+        if (slot < kCandStats) l.cst[slot] = stj;
+      });
       n_cand += ncand;
-      // each thread stages one vertex (NT >= nv) of a candidate; the next
-      // candidate's vertex is loaded while the current one is processed
-      V3<R> pre = zero;
-      if (ncand > 0 && own) {
-        const size_t b0 = (size_t)l.clist[0] * nv + tid;
-        pre = v3(a.xin[b0], a.yin[b0], a.zin[b0]);
-      }
-      for (int q = 0; q < ncand; q++) {
-        const int cj2 = l.clist[q];
-        const CellStat<R> stj = a.st_in[cj2];
-        V3<R> sh = zero;
-        if (pbc)
-          sh = v3(image_shift(Ci.x - stj.cx, L, invL), image_shift(Ci.y - stj.cy, L, invL),
-                  image_shift(Ci.z - stj.cz, L, invL));
-        if (own) {
-          l.nx[tid] = pre.x;
-          l.ny[tid] = pre.y;
-          l.nz[tid] = pre.z;
-        }
-        __syncthreads();
-        if (q + 1 < ncand && own) {
-          const size_t b1 = (size_t)l.clist[q + 1] * nv + tid;
-          pre = v3(a.xin[b1], a.yin[b1], a.zin[b1]);
-        }
-        // Separation along the centre line n = (C_j - C_i) / D (float
-        // steps): every vertex p of this cell has (p - C_i).n <= hi and every
-        // point x of cell j's (shifted) hull has (C_j - x).n <= hj, so
-        // |x - p| >= D - hi - hj, less slack for the float sums (1e-4 of the
-        // cells' extent, 1e-6 of the coordinates).  Contact needs a point
-        // closer than the culling margin m_j to a face of j (contact_margin),
-        // adhesion a vertex pair closer than 2 l0 — on pairs that take no
-        // periodic image (the unshifted centres and radii keep every
-        // component below 0.48 L; the shift is 0 then).  In the general
-        // double kernel the extra registers cost more than the skipped work
-        // (spills, measured on 4096 x 642 cells), and every candidate is
-        // evaluated.
-        bool skip_contact = false, skip_adh = false;
-        if constexpr (sizeof(R) == 4 || NTC > 0) {
+      // candidate q's statistics (wave-uniform q)
+      auto cand_stat = [&](int q) { return q < kCandStats ? l.cst[q] : a.st_in[l.clist[q]]; };
+      // Separation along the centre line n = (C_j - C_i) / D (float steps
+      // and the compile-time-sized instances): every vertex p of this cell
+      // has (p - C_i).n <= hi and every point x of cell j's (shifted) hull
+      // has (C_j - x).n <= hj, so |x - p| >= D - hi - hj, less slack for the
+      // float sums (1e-4 of the cells' extent, 1e-6 of the coordinates).
+      // Contact needs a point closer than the culling margin m_j to a face
+      // of j (contact_margin), adhesion a vertex pair closer than 2 l0 — on
+      // pairs that take no periodic image (the unshifted centres and radii
+      // keep every component below 0.48 L; the shift is 0 then).  The waves
+      // take the candidates in turn (maxima by wave shuffles, cell j's
+      // vertices read from the L2), so the bound costs no workgroup barrier
+      // per candidate; the candidates that need neither pass are dropped
+      // and the rest keep their two bits (1 contact, 2 adhesion) above bit
+      // 28 of their entry.  In the general double kernel the extra
+      // registers cost more than the skipped work (spills, measured on
+      // 4096 x 642 cells), and every candidate takes both passes.
+      // the candidates that take a pass: entries q | passes << 28 (q the
+      // index in l.clist, passes 1 contact, 2 adhesion)
+      int nkeep = ncand;
+      const int* keep = nullptr;
+      bool flagged = false;
+      if constexpr (sizeof(R) == 4 || NTC > 0) {
+        const int nwaves = NT / DPM_WAVE;
+        for (int q = wave; q < ncand; q += nwaves) {  // wave-uniform
+          const int cjq = l.clist[q];
+          const CellStat<R> stj = cand_stat(q);
+          V3<R> sh = zero;
+          if (pbc)
+            sh = v3(image_shift(Ci.x - stj.cx, L, invL), image_shift(Ci.y - stj.cy, L, invL),
+                    image_shift(Ci.z - stj.cz, L, invL));
           const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
           const R l0m = fmax(L0i, fmax(stj.l0, R(1e-12f)));
           const bool all_images = pbc && R(2) * (Ri + stj.r + R(2) * l0m) >= R(0.999f) * L;
@@ -962,18 +969,60 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
           const R D = sqrt(vdot(dC, dC));
           const V3<R> nC = D > R(0) ? vscale(dC, R(1) / D) : zero;
           R hi = R(-3.0e38f), hj = R(-3.0e38f);
-          if (own) {
-            hi = vdot(vsub(P, Ci), nC);
-            hj = vdot(v3(stj.cx - l.nx[tid], stj.cy - l.ny[tid], stj.cz - l.nz[tid]), nC);
+          const size_t bj = (size_t)cjq * nv;
+          for (int v = lane; v < nv; v += DPM_WAVE) {
+            hi = fmax(hi, vdot(vsub(lds_v(l.sx, l.sy, l.sz, v), Ci), nC));
+            hj = fmax(hj, vdot(v3(stj.cx - a.xin[bj + v], stj.cy - a.yin[bj + v], stj.cz - a.zin[bj + v]), nC));
           }
-          block_max2(hi, hj, reinterpret_cast<R*>(l.red));
+          for (int o = 32; o > 0; o >>= 1) {
+            hi = fmax(hi, __shfl_xor(hi, o, DPM_WAVE));
+            hj = fmax(hj, __shfl_xor(hj, o, DPM_WAVE));
+          }
+          bool skip_contact = false, skip_adh = false;
           const R Mc = ((fabs(Ci.x) + fabs(Ci.y)) + fabs(Ci.z)) + ((fabs(Cj.x) + fabs(Cj.y)) + fabs(Cj.z));
           if (D > R(0)) {
             const R gap = ((D - hi) - hj) - (R(1e-4f) * ((Ri + stj.r) + D) + R(1e-6f) * Mc);
             skip_contact = gap >= contact_margin(stj.r, stj.em);
             skip_adh = !all_images && no_image && gap >= R(2) * l0m;
           }
+          const int bits = ((do_contact && !skip_contact) ? 1 : 0) | ((do_adh && !skip_adh) ? 2 : 0);
+          if (lane == 0) l.plist[q] = q | (bits << 28);
         }
+        __syncthreads();
+        const int ent = tid < ncand ? l.plist[tid] : 0;
+        nkeep = block_compact(tid < ncand && (ent >> 28) != 0, ent, l.glist, l.cnt);
+        keep = l.glist;
+        flagged = true;
+      }
+      // each thread stages one vertex (NT >= nv) of a kept candidate; the
+      // next candidate's vertex is loaded while the current one is processed
+      auto entry = [&](int qq) { return flagged ? keep[qq] : (qq | (3 << 28)); };
+      V3<R> pre = zero;
+      if (nkeep > 0 && own) {
+        const size_t b0 = (size_t)l.clist[entry(0) & 0x0fffffff] * nv + tid;
+        pre = v3(a.xin[b0], a.yin[b0], a.zin[b0]);
+      }
+      for (int qq = 0; qq < nkeep; qq++) {
+        const int ent = entry(qq);
+        const int q = ent & 0x0fffffff;
+        const int cj2 = l.clist[q];
+        const int passes = ent >> 28;
+        const CellStat<R> stj = cand_stat(q);
+        V3<R> sh = zero;
+        if (pbc)
+          sh = v3(image_shift(Ci.x - stj.cx, L, invL), image_shift(Ci.y - stj.cy, L, invL),
+                  image_shift(Ci.z - stj.cz, L, invL));
+        if (own) {
+          l.nx[tid] = pre.x;
+          l.ny[tid] = pre.y;
+          l.nz[tid] = pre.z;
+        }
+        __syncthreads();
+        if (qq + 1 < nkeep && own) {
+          const size_t b1 = (size_t)l.clist[entry(qq + 1) & 0x0fffffff] * nv + tid;
+          pre = v3(a.xin[b1], a.yin[b1], a.zin[b1]);
+        }
+        const bool skip_contact = !(passes & 1), skip_adh = !(passes & 2);
         if (do_contact && !skip_contact) {  // block-uniform
           const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
           const R mj = contact_margin(stj.r, stj.em);
@@ -1023,6 +1072,10 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
                 npart++;
               }
               if (PROBE && a.p_contact) a.p_contact[pidx] = 1;
+              if (PROBE && a.p_contact_key) {  // count, then key sum of the partner cells
+                atomicAdd(&a.p_contact_key[2 * (base + tid)], 1u);
+                atomicAdd(&a.p_contact_key[2 * (base + tid) + 1], cell_key(cj2 - cb));
+              }
             }
           }
         }

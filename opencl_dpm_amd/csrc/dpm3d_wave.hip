@@ -712,6 +712,10 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
                   }
                 }
                 if (PROBE && a.p_contact) a.p_contact[pidx] = 1;
+                if (PROBE && a.p_contact_key) {  // count, then key sum of the partner cells
+                  atomicAdd(&a.p_contact_key[2 * (base + v)], 1u);
+                  atomicAdd(&a.p_contact_key[2 * (base + v) + 1], cell_key(cj2 - cb));
+                }
               }
             }
           }

@@ -17,6 +17,9 @@
 #include "dpm2d.hip"
 #include "dpm3d.hip"
 #include "dpm3d_wave.hip"
+#include "dpm2d_wide.hip"
+#include "dpm_disperse.hip"
+#include "dpm_measures.hip"
 
 namespace {
 
@@ -88,6 +91,7 @@ struct dpm_tissue {
   double* vol[2] = {};
   void* em[2] = {};  // per-cell longest edge
   double* energy = nullptr;
+  float* meas = nullptr;  // per cell: the host reductions (dpm3d_cell_measures)
   float *Kv = nullptr, *Ka = nullptr, *Ks = nullptr, *Kb = nullptr, *v0 = nullptr, *a0 = nullptr;
   int *cell_tissue = nullptr, *tissue_begin = nullptr;
   float *tL = nullptr, *tKre = nullptr, *tKat = nullptr, *tksg = nullptr;
@@ -150,6 +154,15 @@ struct dpm_tissue {
   std::vector<float> a0_h;
   hipGraphExec_t jam_graph = nullptr;
   std::vector<unsigned char> jam_key;
+  // tissues beyond one workgroup: 0 automatic, 1 one workgroup per tissue,
+  // 2 the wide path (three launches per step over the device)
+  int wide_mode = 0;
+  bool fits_wg = true;  // every tissue fits one workgroup (k2d_run)
+  float *xw = nullptr, *yw = nullptr, *cellq = nullptr;
+  int *ncand_w = nullptr, *cell_tissue2 = nullptr, *vbl = nullptr;
+  short* cand_w = nullptr;
+  hipGraphExec_t wide_graph = nullptr;
+  float wide_graph_dt = 0.0f;
 
   template <class T_>
   int alloc(T_** p, size_t n) {
@@ -166,6 +179,7 @@ struct dpm_tissue {
   ~dpm_tissue() {
     if (graph) (void)hipGraphExecDestroy(graph);
     if (jam_graph) (void)hipGraphExecDestroy(jam_graph);
+    if (wide_graph) (void)hipGraphExecDestroy(wide_graph);
     for (void* p : allocs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -803,7 +817,7 @@ static int launch3d(dpm_tissue* h, Kind3D kind, int in, float dt, int flags, Edi
     else
       k = kind == K_STEP ? k3d_step_f32 : kind == K_STATS ? k3d_stats_f32 : k3d_energy_f32;
     // diagnostic outputs requested: the instances that record them
-    const bool probe = a.p_contact || a.p_winding || a.p_down || a.p_adh || a.p_adh_hash;
+    const bool probe = a.p_contact || a.p_winding || a.p_down || a.p_adh || a.p_adh_hash || a.p_contact_key;
     if constexpr (sizeof(R) == 4)
       if (kind == K_STEP && h->ww == 4 && h->nv == 162 && h->nf == 320 && h->NT == 256)
         k = probe ? k3d_step_f32_v162_probe : k3d_step_f32_v162;
@@ -1055,6 +1069,35 @@ extern "C" int dpm3d_cell_stats(dpm_tissue* h, double* volume, float* com) {
   return DPM_OK;
 }
 
+extern "C" int dpm3d_cell_measures(dpm_tissue* h, float* volume, float* surface_area, float* com) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  const size_t lds = measures_lds_bytes(h->nv, h->nf);
+  if ((int)lds > max_lds_per_block(h->device)) return fail(DPM_ERANGE, "cell mesh does not fit the LDS of one workgroup");
+  if (!h->meas) TRY(h->alloc(&h->meas, 5 * (size_t)h->C));
+  const int c = h->cur;
+  auto go = [&](auto tag) -> int {
+    using R = decltype(tag);
+    Measures3D<R> a{(const R*)h->x[c], (const R*)h->y[c], (const R*)h->z[c], h->faces, h->nv, h->nf, h->meas};
+    void (*k)(Measures3D<R>) = nullptr;
+    if constexpr (sizeof(R) == 8) k = k3d_measures_f64; else k = k3d_measures_f32;
+    HIPCHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k, dim3(h->C), dim3(64), lds, h->stream, a);
+    HIPCHECK(hipGetLastError());
+    return DPM_OK;
+  };
+  TRY(h->rs == 8 ? go(0.0) : go(0.0f));
+  std::vector<float> m(5 * (size_t)h->C);
+  HIPCHECK(hipMemcpyAsync(m.data(), h->meas, m.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  for (int i = 0; i < h->C; i++) {
+    if (volume) volume[i] = m[5 * (size_t)i];
+    if (surface_area) surface_area[i] = m[5 * (size_t)i + 1];
+    if (com)
+      for (int k = 0; k < 3; k++) com[3 * (size_t)i + k] = m[5 * (size_t)i + 2 + k];
+  }
+  return DPM_OK;
+}
+
 extern "C" int dpm3d_energy(dpm_tissue* h, double* energy) {
   if (!h || h->dim != 3 || !energy) return fail(DPM_EINVAL, "bad arguments");
   TRY(launch3d(h, K_ENERGY, h->cur, 0.0f, 0, [](auto&) {}));
@@ -1134,6 +1177,35 @@ extern "C" int dpm3d_probe_adhesion(dpm_tissue* h, unsigned* own_keys, unsigned*
   for (size_t i = 0; i < NVT; i++) {
     if (own_keys) own_keys[i] = k[2 * i];
     if (rev_keys) rev_keys[i] = k[2 * i + 1];
+  }
+  return DPM_OK;
+}
+
+extern "C" int dpm3d_probe_contact(dpm_tissue* h, unsigned* count, unsigned* keys) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  if (!count && !keys) return DPM_OK;
+  TRY(ensure_stats(h));
+  const size_t NVT = (size_t)h->C * h->nv;
+  unsigned* dk = nullptr;
+  hipStream_t s = h->stream;
+  int rc = dev_alloc(&dk, 2 * NVT);
+  if (!rc) {
+    (void)hipMemsetAsync(dk, 0, 2 * NVT * sizeof(unsigned), s);
+    rc = launch3d(h, K_STEP, h->cur, 0.0f, F_WRITE_FORCES, [&](auto& a) {
+      a.counters = nullptr;
+      a.p_contact_key = dk;
+    });
+  }
+  std::vector<unsigned> k(rc ? 0 : 2 * NVT);
+  if (!rc)
+    rc = hipMemcpyAsync(k.data(), dk, 2 * NVT * sizeof(unsigned), hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(DPM_EDEVICE, "stream sync failed");
+  if (dk) (void)hipFree(dk);
+  if (rc) return rc;
+  h->forces_valid = true;
+  for (size_t i = 0; i < NVT; i++) {
+    if (count) count[i] = k[2 * i];
+    if (keys) keys[i] = k[2 * i + 1];
   }
   return DPM_OK;
 }
@@ -1510,15 +1582,15 @@ static int create2d(const dpm2d_desc* d, dpm_tissue** out, int rs) {
   // spreads (vertex, candidate) pairs over all of them
   h->NT2 = std::min(1024, std::max(64, (4 * maxv + 63) / 64 * 64));
   h->NTr = std::min(1024, std::max(64, (maxv / 4 + 63) / 64 * 64));
-  if (rs == 4 && maxv > DPM2D_VPT * h->NT2) {
-    delete h;
-    return fail(DPM_ERANGE, "tissue has more than 16384 vertices");
-  }
   h->lds2 = lds2d_bytes(maxv, maxc, rs);
-  if ((int)h->lds2 > max_lds_per_block(h->device)) {
+  // a tissue beyond one workgroup (LDS, or 16 vertices per thread) takes the
+  // wide path; float tissues only
+  h->fits_wg = maxv <= DPM2D_VPT * h->NT2 && (int)h->lds2 <= max_lds_per_block(h->device);
+  if (!h->fits_wg && rs == 8) {
     delete h;
-    return fail(DPM_ERANGE, "2D tissue does not fit the LDS of one workgroup");
+    return fail(DPM_ERANGE, "a double 2D tissue must fit the LDS of one workgroup");
   }
+  if (!h->fits_wg) h->lds2 = 0;
   h->cvb_h = cvb;
   h->tissue_begin_h.assign(d->cell_begin, d->cell_begin + T + 1);
   auto AR = [&](void** p, size_t n) { return h->alloc_real(p, n); };
@@ -1551,10 +1623,132 @@ static int create2d(const dpm2d_desc* d, dpm_tissue** out, int rs) {
   if (!rc) rc = dpm2d_set_cell_params(h, d->Ka, d->Kl, d->Kb, d->a0, d->l0, d->r0);
   if (!rc) rc = dpm2d_set_tissue_params(h, d->L, d->pbc, d->Kre, d->Kat, d->mask);
   if (rc) { delete h; return rc; }
-  (void)hipFuncSetAttribute((const void*)k2d_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds2);
-  (void)hipFuncSetAttribute(rs == 8 ? (const void*)k2d_relax_f64 : (const void*)k2d_relax_f32,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds2);
+  if (h->fits_wg) {
+    (void)hipFuncSetAttribute((const void*)k2d_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds2);
+    (void)hipFuncSetAttribute(rs == 8 ? (const void*)k2d_relax_f64 : (const void*)k2d_relax_f32,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds2);
+  }
   *out = h;
+  return DPM_OK;
+}
+
+// ---------------------------------------------------- 2D: the wide path
+
+static Run2D run2d_args(dpm_tissue* h, int nsteps, float dt);
+
+static bool use_wide(const dpm_tissue* h) {
+  return h->rs == 4 && (h->wide_mode == 2 || (h->wide_mode == 0 && !h->fits_wg));
+}
+
+static int ensure_wide(dpm_tissue* h) {
+  if (h->xw) return DPM_OK;
+  const int C = h->C, T = h->T;
+  int rc = 0;
+  rc |= h->alloc(&h->xw, h->V);
+  rc |= h->alloc(&h->yw, h->V);
+  rc |= h->alloc(&h->cellq, 8 * (size_t)C);
+  rc |= h->alloc(&h->ncand_w, C);
+  rc |= h->alloc(&h->cand_w, (size_t)C * DPM2D_MAXCAND);
+  rc |= h->alloc(&h->cell_tissue2, C);
+  rc |= h->alloc(&h->vbl, (size_t)C + T);
+  if (rc) return DPM_ENOMEM;
+  std::vector<int> ct(C), vbl((size_t)C + T);
+  for (int t = 0; t < T; t++) {
+    const int c0 = h->tissue_begin_h[t], c1 = h->tissue_begin_h[t + 1];
+    for (int c = c0; c < c1; c++) ct[c] = t;
+    for (int c = c0; c <= c1; c++) vbl[(size_t)c + t] = h->cvb_h[c] - h->cvb_h[c0];
+  }
+  TRY(upload(h->cell_tissue2, ct.data(), C, h->stream));
+  TRY(upload(h->vbl, vbl.data(), vbl.size(), h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return DPM_OK;
+}
+
+static Wide2D wide_args(dpm_tissue* h, int in, float dt, int flags) {
+  Wide2D a{};
+  a.g = run2d_args(h, 0, 0.0f).g;
+  float* X[2] = {(float*)h->x2, h->xw};
+  float* Y[2] = {(float*)h->y2, h->yw};
+  a.xin = X[in];
+  a.yin = Y[in];
+  a.xout = X[1 - in];
+  a.yout = Y[1 - in];
+  a.fx = (float*)h->fx2;
+  a.fy = (float*)h->fy2;
+  a.cell_tissue = h->cell_tissue2;
+  a.vbl = h->vbl;
+  a.cellq = h->cellq;
+  a.ncand = h->ncand_w;
+  a.cand = h->cand_w;
+  a.C = h->C;
+  a.V = h->V;
+  a.dt = dt;
+  a.flags = flags;
+  a.counters = h->counters;
+  return a;
+}
+
+static int launch_wide(dpm_tissue* h, const Wide2D& a) {
+  const int bc = (h->C + 255) / 256, bv = (h->V + 255) / 256;
+  hipLaunchKernelGGL(k2d_wide_cells, dim3(bc), dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL(k2d_wide_cand, dim3(bc), dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL(k2d_wide_step, dim3(bv), dim3(256), 0, h->stream, a);
+  HIPCHECK(hipGetLastError());
+  return DPM_OK;
+}
+
+// kGraphSteps steps from buffer 0 (ending on buffer 0), captured once per dt.
+static int build_wide_graph(dpm_tissue* h, float dt) {
+  if (h->wide_graph) {
+    (void)hipGraphExecDestroy(h->wide_graph);
+    h->wide_graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = DPM_OK;
+  for (int s = 0; s < kGraphSteps && !rc; s++) rc = launch_wide(h, wide_args(h, s & 1, dt, W2_MOVE));
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(DPM_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&h->wide_graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    h->wide_graph = nullptr;
+    return fail(DPM_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  h->wide_graph_dt = dt;
+  return DPM_OK;
+}
+
+// nsteps steps; the positions start and end in buffer 0 (x2, y2), the last
+// step writes the forces it used.
+static int euler2d_wide(dpm_tissue* h, int nsteps, float dt) {
+  TRY(ensure_wide(h));
+  if (!h->wide_graph || h->wide_graph_dt != dt) TRY(build_wide_graph(h, dt));
+  int cur = 0, left = nsteps - 1;
+  while (left >= kGraphSteps) {
+    HIPCHECK(hipGraphLaunch(h->wide_graph, h->stream));
+    left -= kGraphSteps;
+  }
+  for (; left > 0; left--, cur ^= 1) TRY(launch_wide(h, wide_args(h, cur, dt, W2_MOVE)));
+  TRY(launch_wide(h, wide_args(h, cur, dt, W2_MOVE | W2_FORCES)));
+  cur ^= 1;
+  if (cur == 1) {
+    HIPCHECK(hipMemcpyAsync(h->x2, h->xw, (size_t)h->V * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHECK(hipMemcpyAsync(h->y2, h->yw, (size_t)h->V * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+  }
+  return DPM_OK;
+}
+
+extern "C" int dpm2d_set_step_kernel(dpm_tissue* h, int kind) {
+  if (!h || h->dim != 2) return fail(DPM_EINVAL, "not a 2D tissue");
+  if (kind < 0 || kind > 2) return fail(DPM_EINVAL, "kind must be 0 (automatic), 1 (one workgroup) or 2 (wide)");
+  if (kind == 1 && !h->fits_wg) return fail(DPM_EINVAL, "a tissue does not fit one workgroup");
+  if (kind == 2 && h->rs != 4) return fail(DPM_EINVAL, "the wide 2D step takes float tissues");
+  h->wide_mode = kind;
   return DPM_OK;
 }
 
@@ -1791,6 +1985,8 @@ static int launch_relax(dpm_tissue* h, int mode, int threads, Edit edit) {
 }
 
 static int ensure_relax_buffers(dpm_tissue* h) {
+  if (!h->fits_wg)
+    return fail(DPM_ERANGE, "FIRE, jamming and energies need 2D tissues that fit one workgroup's LDS");
   if (h->vx2) return DPM_OK;
   int rc = h->alloc_real(&h->vx2, h->V);
   rc |= h->alloc_real(&h->vy2, h->V);
@@ -1815,6 +2011,7 @@ static int euler2d(dpm_tissue* h, int nsteps, float dt) {
       a.dt = dt;
     });
   }
+  if (use_wide(h)) return euler2d_wide(h, nsteps, dt);
   Run2D a = run2d_args(h, nsteps, dt);
   hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds2, h->stream, a);
   HIPCHECK(hipGetLastError());
@@ -1982,16 +2179,25 @@ extern "C" int dpm2d_probe(dpm_tissue* h, float* forces, int* overlap, int* attr
   int *dov = nullptr, *dat = nullptr;
   int rc = dev_alloc(&dov, h->V);
   rc |= dev_alloc(&dat, h->V);
+  if (!rc && use_wide(h)) rc = ensure_wide(h);
   if (!rc) {
     (void)hipMemsetAsync(dov, 0, h->V * sizeof(int), s);
     (void)hipMemsetAsync(dat, 0, h->V * sizeof(int), s);
-    Run2D a = run2d_args(h, 0, 0.0f);
-    a.counters = nullptr;
-    a.p_overlap = dov;
-    a.p_attr = dat;
-    hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds2, s, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) rc = fail(DPM_EDEVICE, std::string("k2d_run: ") + hipGetErrorString(e));
+    if (use_wide(h)) {
+      Wide2D a = wide_args(h, 0, 0.0f, W2_FORCES);
+      a.counters = nullptr;
+      a.p_overlap = dov;
+      a.p_attr = dat;
+      rc = launch_wide(h, a);
+    } else {
+      Run2D a = run2d_args(h, 0, 0.0f);
+      a.counters = nullptr;
+      a.p_overlap = dov;
+      a.p_attr = dat;
+      hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds2, s, a);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) rc = fail(DPM_EDEVICE, std::string("k2d_run: ") + hipGetErrorString(e));
+    }
   }
   if (!rc && overlap)
     rc = hipMemcpyAsync(overlap, dov, h->V * sizeof(int), hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
@@ -2012,20 +2218,93 @@ extern "C" int dpm2d_probe_attraction(dpm_tissue* h, unsigned* keys) {
   hipStream_t s = h->stream;
   unsigned* dk = nullptr;
   int rc = dev_alloc(&dk, h->V);
+  if (!rc && use_wide(h)) rc = ensure_wide(h);
   if (!rc) {
     (void)hipMemsetAsync(dk, 0, h->V * sizeof(unsigned), s);
-    Run2D a = run2d_args(h, 0, 0.0f);
-    a.counters = nullptr;
-    a.p_attr_key = dk;
-    hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds2, s, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) rc = fail(DPM_EDEVICE, std::string("k2d_run: ") + hipGetErrorString(e));
+    if (use_wide(h)) {
+      Wide2D a = wide_args(h, 0, 0.0f, 0);
+      a.counters = nullptr;
+      a.p_attr_key = dk;
+      rc = launch_wide(h, a);
+    } else {
+      Run2D a = run2d_args(h, 0, 0.0f);
+      a.counters = nullptr;
+      a.p_attr_key = dk;
+      hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds2, s, a);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) rc = fail(DPM_EDEVICE, std::string("k2d_run: ") + hipGetErrorString(e));
+    }
   }
   if (!rc)
     rc = hipMemcpyAsync(keys, dk, h->V * sizeof(unsigned), hipMemcpyDeviceToHost, s) ? DPM_EDEVICE : 0;
   if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(DPM_EDEVICE, "stream sync failed");
   if (dk) (void)hipFree(dk);
   return rc;
+}
+
+// ============================================================= dispersal ==
+
+extern "C" int dpm_disperse(const float* r, int n, float L, float* X, float* Y, int max_count, int* result) {
+  if (!r || !X || !Y || n < 1) return fail(DPM_EINVAL, "dpm_disperse: bad arguments");
+  if (n > 65535) return fail(DPM_ERANGE, "dpm_disperse: more than 65535 centres");
+  if (!(L > 0.0f) || !std::isfinite(L)) return fail(DPM_EINVAL, "dpm_disperse: box length must be positive");
+  if (max_count < 0) return fail(DPM_EINVAL, "dpm_disperse: max_count must be >= 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(DPM_EDEVICE, "no HIP device is visible");
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  float rmax = 0.0f;
+  for (int i = 0; i < n; i++) {
+    if (!(r[i] >= 0.0f) || !std::isfinite(r[i])) return fail(DPM_EINVAL, "dpm_disperse: radii must be finite and >= 0");
+    if (!std::isfinite(X[i]) || !std::isfinite(Y[i])) return fail(DPM_ENONFINITE, "dpm_disperse: non-finite centre");
+    rmax = std::max(rmax, r[i]);
+  }
+  // bins no narrower than the largest contact distance 2 rmax (with slack)
+  int nb = rmax > 0.0f ? (int)std::floor(L / (2.0f * rmax * 1.001f + 1e-6f)) : 0;
+  if (nb < 3 || nb > 256) nb = nb > 256 ? 256 : 0;
+  const int threads = std::min(kDispThreads, std::max(64, (n + 63) / 64 * 64));
+  size_t lds = disp_layout(nullptr, n, nb, threads).total;
+  const int maxlds = max_lds_per_block(dev);
+  if ((int)lds > maxlds && nb > 0) {
+    nb = 0;
+    lds = disp_layout(nullptr, n, nb, threads).total;
+  }
+  if ((int)lds > maxlds) return fail(DPM_ERANGE, "dpm_disperse: the centres do not fit the LDS of one workgroup");
+  const int term_cap = n * kDispMaxCand;
+  hipStream_t s = nullptr;
+  HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  float *dr = nullptr, *dx = nullptr, *dy = nullptr, *dt = nullptr;
+  int* dout = nullptr;
+  int rc = dev_alloc(&dr, n);
+  rc |= dev_alloc(&dx, n);
+  rc |= dev_alloc(&dy, n);
+  rc |= dev_alloc(&dout, 2);
+  rc |= dev_alloc(&dt, (size_t)term_cap);
+  int out[2] = {0, 0};
+  if (!rc) rc = upload(dr, r, n, s) | upload(dx, X, n, s) | upload(dy, Y, n, s);
+  if (!rc && hipFuncSetAttribute((const void*)k_disperse2d, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+                 hipSuccess)
+    rc = fail(DPM_EDEVICE, "hipFuncSetAttribute(k_disperse2d) failed");
+  if (!rc) {
+    Disperse2DArgs a{dr, dx, dy, n, L, max_count, nb, dout, dt, term_cap};
+    hipLaunchKernelGGL(k_disperse2d, dim3(1), dim3(threads), lds, s, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail(DPM_EDEVICE, std::string("k_disperse2d: ") + hipGetErrorString(e));
+  }
+  if (!rc && (hipMemcpyAsync(X, dx, n * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
+              hipMemcpyAsync(Y, dy, n * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
+              hipMemcpyAsync(out, dout, 2 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess))
+    rc = fail(DPM_EDEVICE, "dpm_disperse: copy back failed");
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(DPM_EDEVICE, "dpm_disperse: stream sync failed");
+  for (void* p : {(void*)dr, (void*)dx, (void*)dy, (void*)dout, (void*)dt})
+    if (p) (void)hipFree(p);
+  (void)hipStreamDestroy(s);
+  if (rc) return rc;
+  if (result) {
+    result[0] = out[0];
+    result[1] = out[1];
+  }
+  return DPM_OK;
 }
 
 // ================================================================ common ==

@@ -102,6 +102,9 @@ __device__ __forceinline__ unsigned vertex_key(R x, R y, R z) {
   return fmix32(rbits(x) ^ fmix32(rbits(y) ^ fmix32(rbits(z))));
 }
 
+// Key of a partner cell (its index within the tissue), for the same sums.
+__device__ __forceinline__ unsigned cell_key(int c) { return fmix32((unsigned)c + 0x9E3779B9u); }
+
 // ---------------------------------------------------------------- reductions
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, DPM_WAVE);
@@ -160,6 +163,60 @@ __device__ __forceinline__ void block_sum3(double& a, double& b, double& c, doub
   b = scratch[49];
   c = scratch[50];
   __syncthreads();
+}
+
+// block_sum3 with one workgroup barrier: every thread adds the waves'
+// partial sums itself, in wave order (the same result as one thread adding
+// them and broadcasting).  scratch (>= 48 doubles) must not be written by
+// another reduction until every thread has passed a later barrier: callers
+// alternate scratch regions.
+__device__ __forceinline__ void block_sum3_1b(double& a, double& b, double& c, double* scratch) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
+  const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  if (lane == 0) {
+    scratch[wave] = a;
+    scratch[16 + wave] = b;
+    scratch[32 + wave] = c;
+  }
+  __syncthreads();
+  double sa = 0.0, sb = 0.0, sc = 0.0;
+  for (int w = 0; w < nw; w++) {
+    sa += scratch[w];
+    sb += scratch[16 + w];
+    sc += scratch[32 + w];
+  }
+  a = sa;
+  b = sb;
+  c = sc;
+}
+
+// block_max_sum_max with one workgroup barrier (as block_sum3_1b).
+__device__ __forceinline__ void block_max_sum_max_1b(double& m1, double& s, double& m2, double* scratch) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
+  const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
+  for (int o = 32; o > 0; o >>= 1) {
+    m1 = fmax(m1, __shfl_down(m1, o, DPM_WAVE));
+    s += __shfl_down(s, o, DPM_WAVE);
+    m2 = fmax(m2, __shfl_down(m2, o, DPM_WAVE));
+  }
+  if (lane == 0) {
+    scratch[wave] = m1;
+    scratch[16 + wave] = s;
+    scratch[32 + wave] = m2;
+  }
+  __syncthreads();
+  double a = scratch[0], b = 0.0, c = scratch[32];
+  for (int w = 0; w < nw; w++) {
+    a = fmax(a, scratch[w]);
+    b += scratch[16 + w];
+    c = fmax(c, scratch[32 + w]);
+  }
+  m1 = a;
+  s = b;
+  m2 = c;
 }
 
 // max, sum, max of three per-thread values in one pass (every thread gets
@@ -251,21 +308,41 @@ __device__ __forceinline__ int block_compact(bool flag, int value, int* list, in
   const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
   const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
   const unsigned long long m = __ballot(flag);
-  const int before = __popcll(m & ((1ull << lane) - 1ull));
   if (lane == 0) counts[wave] = __popcll(m);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int s = 0;
-    for (int w = 0; w < nw; w++) {
-      int c = counts[w];
-      counts[w] = s;
-      s += c;
-    }
-    counts[16] = s;
+  // every thread adds the waves' counts itself (no serial pass, no third barrier)
+  int off = 0, total = 0;
+  for (int w = 0; w < nw; w++) {
+    const int c = counts[w];
+    off += w < wave ? c : 0;
+    total += c;
   }
+  if (flag) list[off + __popcll(m & ((1ull << lane) - 1ull))] = value;
+  __syncthreads();  // the list is complete; every read of counts is done
+  return total;
+}
+
+// block_compact that also calls also(slot) on every flagged thread, with
+// the thread's index in the list, before the closing barrier (so what it
+// writes to LDS is visible to every thread on return).
+template <typename Fn>
+__device__ __forceinline__ int block_compact_with(bool flag, int value, int* list, int* counts, Fn&& also) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
+  const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
+  const unsigned long long m = __ballot(flag);
+  if (lane == 0) counts[wave] = __popcll(m);
   __syncthreads();
-  if (flag) list[counts[wave] + before] = value;
-  const int total = counts[16];
+  int off = 0, total = 0;
+  for (int w = 0; w < nw; w++) {
+    const int c = counts[w];
+    off += w < wave ? c : 0;
+    total += c;
+  }
+  if (flag) {
+    const int slot = off + __popcll(m & ((1ull << lane) - 1ull));
+    list[slot] = value;
+    also(slot);
+  }
   __syncthreads();
   return total;
 }
@@ -284,25 +361,20 @@ __device__ __forceinline__ void block_compact2(bool fa, int va, int* la, bool fb
     counts[16 + wave] = __popcll(mb);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int sa = 0, sb = 0;
-    for (int w = 0; w < nw; w++) {
-      const int ca = counts[w], cb = counts[16 + w];
-      counts[w] = sa;
-      counts[16 + w] = sb;
-      sa += ca;
-      sb += cb;
-    }
-    counts[32] = sa;
-    counts[33] = sb;
+  int oa = 0, ob = 0, sa = 0, sb = 0;
+  for (int w = 0; w < nw; w++) {
+    const int ca = counts[w], cb = counts[16 + w];
+    oa += w < wave ? ca : 0;
+    ob += w < wave ? cb : 0;
+    sa += ca;
+    sb += cb;
   }
-  __syncthreads();
-  const int pa = counts[wave] + __popcll(ma & below);
-  slot_b = counts[16 + wave] + __popcll(mb & below);
+  const int pa = oa + __popcll(ma & below);
+  slot_b = ob + __popcll(mb & below);
   if (fa) la[pa] = va;
   if (fb) lb[slot_b] = vb;
-  na = counts[32];
-  nb = counts[33];
+  na = sa;
+  nb = sb;
   __syncthreads();
 }
 

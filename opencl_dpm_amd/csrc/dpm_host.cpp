@@ -273,63 +273,25 @@ Tissue2D::Tissue2D(std::vector<Cell2D> inputCells, float packingFraction) {
 }
 
 namespace {
-// O(N^2) soft-disk relaxation of cell centres in a periodic box, the
-// dispersal of Tissue2D.cpp:42-105 and Tissue3D.cpp:40-106 (contact
-// diameter `scale*(r_i + r_j)`), drawing the initial centres from drand48.
+// Dispersal of Tissue2D.cpp:42-105 and Tissue3D.cpp:40-106: starting centres
+// drawn from the process-wide drand48 stream (x then y per cell, as the
+// reference draws them), then the soft-disk relaxation with contact
+// diameter scale * (r_i + r_j) on the device (dpm_disperse, bit-identical
+// to the reference's host loop).
 void relax_centres(const std::vector<float>& radius, float scale, float L, std::vector<float>& X,
                    std::vector<float>& Y, bool warn) {
   const int n = (int)radius.size();
   X.assign(n, 0.0f);
   Y.assign(n, 0.0f);
-  std::vector<float> Fx(n), Fy(n);
+  std::vector<float> r(n);
   for (int i = 0; i < n; i++) {
     X[i] = drand48() * L;
     Y[i] = drand48() * L;
+    r[i] = radius[i] * scale;
   }
-  float oldU = 100, dU = 100;
-  int count = 0;
-  while (dU > 1e-6) {
-    float U = 0;
-    for (int i = 0; i < n; i++) {
-      Fx[i] = 0.0;
-      Fy[i] = 0.0;
-    }
-    for (int i = 0; i < n; i++) {
-      const float xi = X[i], yi = Y[i], ri = radius[i] * scale;
-      for (int j = 0; j < n; j++) {
-        if (j == i) continue;
-        const float rj = radius[j] * scale;
-        float dx = X[j] - xi;
-        dx -= L * roundf(dx / L);
-        float dy = Y[j] - yi;
-        dy -= L * roundf(dy / L);
-        float dist = sqrtf(dx * dx + dy * dy);
-        if (dist < 0.0) dist *= -1;
-        if (dist <= (ri + rj)) {
-          const float ux = dx / dist, uy = dy / dist;
-          const float ftmp = (1.0 - dist / (ri + rj)) / (ri + rj);
-          const float fx = ftmp * ux, fy = ftmp * uy;
-          Fx[i] -= fx;
-          Fy[i] -= fy;
-          Fy[j] += fy;
-          Fx[j] += fx;
-          U += 0.5 * (1 - (dist / (ri + rj)) * (1 - dist / (ri + rj)));
-        }
-      }
-    }
-    for (int i = 0; i < n; i++) {
-      X[i] += 0.01 * Fx[i];
-      Y[i] += 0.01 * Fy[i];
-    }
-    dU = U - oldU;
-    if (dU < 0.0) dU *= -1;
-    oldU = U;
-    count++;
-    if (count > 1e5) {
-      if (warn) std::cerr << "Warning: Max timesteps for dispersion reached" << std::endl;
-      break;
-    }
-  }
+  int res[2] = {0, 0};
+  check(dpm_disperse(r.data(), n, L, X.data(), Y.data(), 100000, res));
+  if (res[1] && warn) std::cerr << "Warning: Max timesteps for dispersion reached" << std::endl;
 }
 }  // namespace
 
@@ -585,9 +547,12 @@ void Tissue3D::CLEulerUpdate(int nsteps, float dt) {
     std::cout << "Average time per step: " << (ms.count() / static_cast<double>(nsteps)) << " ms"
               << std::endl;
   }
-  std::vector<float> f((size_t)C * nv * 3);
+  std::vector<float> f((size_t)C * nv * 3), vol(C), sa(C);
   check(dpm3d_get_positions(h_.get(), xyz.data(), (size_t)C * nv));
   check(dpm3d_get_forces(h_.get(), f.data(), (size_t)C * nv));
+  // GetVolume / GetSurfaceArea of every cell (Tissue3D.cpp:503-504), on the
+  // device in the host functions' float order (bit-identical)
+  check(dpm3d_cell_measures(h_.get(), vol.data(), sa.data(), nullptr));
   // Result checks and host reductions as Tissue3D.cpp:477-521.
   for (int c = 0; c < C; c++) {
     Cell3D& cell = Cells[c];
@@ -609,8 +574,8 @@ void Tissue3D::CLEulerUpdate(int nsteps, float dt) {
       cell.Forces[v] = {f[i], f[i + 1], f[i + 2]};
     }
     cell.verts_stamp = next_stamp();
-    cell.Volume = cell.GetVolume();
-    cell.SurfaceArea = cell.GetSurfaceArea();
+    cell.Volume = vol[c];
+    cell.SurfaceArea = sa[c];
     if (!std::isfinite(cell.Volume) || cell.Volume <= 0)
       std::cerr << "[WARNING] Invalid volume for cell " << c << ": " << cell.Volume << std::endl;
     if (!std::isfinite(cell.SurfaceArea) || cell.SurfaceArea <= 0)

@@ -194,3 +194,60 @@ def test_autograd_function():
     for k, name in enumerate(NAMES):
         np.testing.assert_allclose(tens[name].grad.numpy(), dth[:, k], rtol=1e-10, atol=1e-300)
     np.testing.assert_allclose(X0.grad.numpy(), l0, rtol=1e-10, atol=1e-10 * np.abs(l0).max())
+
+
+@pytest.mark.timeout(600)
+def test_differentiation_workload_gradients():
+    # the differentiation workload itself: 64 cells x 162 vertices on the
+    # substrate whose stiffness grows 10x across x, loss = mean |COM_200 -
+    # COM_0|, checkpoints every 20 steps.  Reference values: FP64 forward
+    # runs; per parameter the median of the difference quotients over the
+    # consecutive float values theta_0 + k ulp, k = -8..8 (the force law's
+    # cut-offs put isolated jumps in the loss, which a single central
+    # difference can straddle).  Bar: 1e-4 relative (gradient norm) for the
+    # double adjoint and for the float adjoint with compensated positions;
+    # the plain float adjoint is reported (its positions drop displacements
+    # below one float ulp of the coordinates, so its own loss drifts).
+    m = W.monolayer_config("cfg5")
+    th = params(m)
+    X0 = m.X.astype(np.float64)
+    nsteps, dt, every = 200, 0.002, 20
+    cells = range(4)
+    grads = {}
+    for prec in (64, "32c", 32):
+        t = tissue(m, 64 if prec == 64 else 32, th)
+        if prec == "32c":
+            t.set_compensated(True)
+        t.set_positions(X0 if prec == 64 else m.X)
+        t.trajectory(nsteps, dt, every)
+        L, lam = loss_and_lambda(t, m, X0)
+        grads[prec] = t.adjoint(lam)[0]
+        print(f"f{prec}: loss {L:.9e}", flush=True)
+
+    def loss_at(c, name, value):
+        tp = {q: a.copy() for q, a in th.items()}
+        tp[name][c] = value
+        t = tissue(m, 64, tp)
+        t.set_positions(X0)
+        t.euler(nsteps, dt)
+        return loss_and_lambda(t, m, X0)[0]
+
+    ref = np.zeros((len(cells), 4))
+    for c in cells:
+        for k, name in enumerate(NAMES):
+            x0 = np.float32(th[name][c])
+            xs = [float(x0)]
+            up = dn = x0
+            for _ in range(8):
+                up = np.nextafter(up, np.float32(np.inf))
+                dn = np.nextafter(dn, np.float32(-np.inf))
+                xs = [float(dn)] + xs + [float(up)]
+            Ls = [loss_at(c, name, x) for x in xs]
+            ref[c, k] = float(np.median(np.diff(Ls) / np.diff(xs)))
+        print(f"cell {c}: reference {ref[c]}", flush=True)
+    for prec, tol in ((64, 1e-4), ("32c", 1e-4), (32, None)):
+        g = grads[prec][:len(cells)]
+        e = float(np.linalg.norm(g - ref) / np.linalg.norm(ref))
+        print(f"f{prec} adjoint vs FP64 1-ulp difference quotients: relative error {e:.3e}", flush=True)
+        if tol is not None:
+            assert e <= tol

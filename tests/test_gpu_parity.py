@@ -36,6 +36,24 @@ def assert_forces(Fg, F64, what):
     assert vmax <= FORCE_VTOL * scale, (what, vmax, scale)
 
 
+def fmix32(h):
+    h = np.asarray(h, np.uint64) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(16)
+    h = (h * np.uint64(0x85EBCA6B)) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(13)
+    h = (h * np.uint64(0xC2B2AE35)) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(16)
+    return h
+
+
+def contact_signature(table):
+    """Per vertex the contact partner count and the 32-bit key sum of the
+    partner cells (the form dpm3d_probe_contact reports) from the table."""
+    keys = fmix32(np.arange(table.shape[1], dtype=np.uint64) + np.uint64(0x9E3779B9))
+    on = table.astype(bool)
+    return on.sum(1).astype(np.uint32), ((on * keys[None, :]).sum(1) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
 def tissue3d(m, mask=capi.DPM3D_ALL, Kat=None, Kre=None):
     return capi.Tissues3D(m.mesh.faces, [0, m.C], m.Kv, m.Ka, m.Ks, m.v0, m.a0, m.L,
                           m.Kre if Kre is None else Kre, Kat=m.Kat if Kat is None else Kat,
@@ -80,6 +98,10 @@ def test_3d_forces_and_pairs(case, mask, kernel):
     np.testing.assert_array_equal(g["down"], r32["down"])
     np.testing.assert_array_equal(g["adhesion"], r32["adhesion"])
     np.testing.assert_array_equal(g["adhesion_hash"], r32["adhesion_hash"])
+    n, k = t.probe_contact()  # the compact form of the same contact set
+    n32, k32 = contact_signature(r32["contact"])
+    np.testing.assert_array_equal(n, n32)
+    np.testing.assert_array_equal(k, k32)
     print(f"pairs: contact={int(g['contact'].sum())} down={int(g['down'].sum())} "
           f"adhesion={int(g['adhesion'].sum())}", flush=True)
     if case.startswith("compressed") and mask & capi.DPM3D_CONTACT:

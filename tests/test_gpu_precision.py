@@ -10,9 +10,10 @@ Bars:
   * energy (both handles): within 1e-13 relative of the oracle's double
     energy at the handle's positions;
   * the float production step against the FP64 step at identical positions,
-    at the full 4096-cell x 642-vertex precision workload: <= 1e-5 relative
-    force error (the production bar), and the float step's energy stays
-    within a stated band of the FP64 run's over a trajectory.
+    at 11 checkpoints of a 5,000-step run of the full 4096-cell x 642-vertex
+    precision workload: <= 1e-5 relative force error over the vertices whose
+    cut-off decisions agree, flip counts reported; the float step's energy
+    stays within a stated band of the FP64 run's over a trajectory.
 """
 import numpy as np
 import pytest
@@ -120,25 +121,95 @@ def test_energy_vs_oracle(precision):
     np.testing.assert_allclose(t.energy(), Eb, rtol=1e-13)
 
 
-def test_f32_step_vs_f64_step_full_precision_workload():
-    # 4096 cells x 642 vertices (icosphere s=3): float forces with double
-    # per-cell sums against the all-double step, at identical positions
+def flipped_vertices(m, X, t32, t64):
+    """Forces of the float and the double step at the same positions X, and
+    the vertices where a cut-off decision of the force law differs between
+    them: contact partner sets (count and key sum, dpm3d_probe_contact),
+    facing-down faces (their corners), the substrate band |z| < 2 l0 at the
+    corners of facing-down faces (Cell3D_Kernel.cl:237; l0 =
+    sqrt(4 a0 / sqrt 3) is rounded to float in the float step, so a corner
+    between the two thresholds takes the spreading push in one step only),
+    own-side adhesion partner counts (the vertex and every vertex of another
+    cell within 2 l0 of it, which receives the pair's reaction)."""
+    t32.set_positions(X)
+    t64.set_positions(np.asarray(X, np.float64))
+    n32, k32 = t32.probe_contact()
+    n64, k64 = t64.probe_contact()
+    g32 = t32.probe(contact=False)
+    g64 = t64.probe(contact=False)
+    F32 = np.asarray(g32["forces"], np.float64)
+    F64 = t64.forces(np.float64)
+    nv = m.mesh.nv
+    flip = np.zeros(len(X), bool)
+    fc = (n32 != n64) | (k32 != k64)
+    flip |= fc
+    cells_f, faces_f = np.nonzero(g32["down"] != g64["down"])
+    faces = np.asarray(m.mesh.faces)
+    for c, f in zip(cells_f, faces_f):
+        flip[c * nv + faces[f]] = True
+    # the substrate band, each step with its own l0 (cell_l0: sqrt(4 a0) / sqrt(3))
+    a0 = np.broadcast_to(np.asarray(m.a0, np.float32), (m.C,))
+    band32 = (np.sqrt(np.float32(4) * a0) / np.sqrt(np.float32(3))) * np.float32(2)
+    band64 = (np.sqrt(4.0 * a0.astype(np.float64)) / np.sqrt(3.0)) * 2.0
+    z = np.abs(np.asarray(X, np.float32)[:, 2]).reshape(m.C, nv)
+    differs = (z < band32[:, None]) != (z.astype(np.float64) < band64[:, None])
+    corner = np.zeros((m.C, nv), bool)
+    cells_d, faces_d = np.nonzero((g32["down"] != 0) | (g64["down"] != 0))
+    corner[cells_d[:, None], faces[faces_d]] = True
+    fb = (differs & corner).ravel()
+    flip |= fb
+    fa = g32["adhesion"] != g64["adhesion"]
+    flip |= fa
+    if fa.any():
+        from scipy.spatial import cKDTree
+        L = float(m.L)
+        P = np.asarray(X, np.float64).copy()
+        P[:, :2] = np.mod(P[:, :2], L)
+        zmin = P[:, 2].min()
+        P[:, 2] -= zmin
+        tree = cKDTree(P, boxsize=[L, L, P[:, 2].max() + 10.0])
+        l0 = np.sqrt(4 * float(np.max(m.a0))) / np.sqrt(3)
+        for i in np.nonzero(fa)[0]:
+            flip[tree.query_ball_point(P[i], 2 * l0 * 1.01)] = True
+    counts = {"contact": int(fc.sum()), "down": int(len(cells_f)), "substrate band": int(fb.sum()),
+              "adhesion": int(fa.sum())}
+    return F32, F64, flip, counts
+
+
+@pytest.mark.timeout(900)
+def test_f32_step_vs_f64_step_along_the_precision_workload():
+    # 4096 cells x 642 vertices (icosphere s=3), all terms: the float step
+    # (double per-cell sums) against the all-double step at the same
+    # positions, at 11 checkpoints of a 5,000-step float run.  The force law
+    # is discontinuous at its cut-offs (contact onset, facing-down faces,
+    # adhesion reach): where float and double take a decision differently the
+    # force moves by O(K) at that vertex, above any rounding bar.  Bar: at
+    # every checkpoint, <= 1e-5 relative force error over the vertices whose
+    # decisions agree (and no more than 1e-3 of the vertices flipped); the
+    # error over all vertices and the flip counts are reported.  (Measured:
+    # after 1,500 steps one corner lies between the float and the double
+    # substrate band edge, 1.2e-8 apart, and alone moves the error over all
+    # vertices to 5.9e-3; every other vertex is within 9e-6 absolute.)
     m = W.monolayer_config("cfg3")
+    t = tissue3d(m, capi.DPM3D_ALL, 32)
     t32 = tissue3d(m, capi.DPM3D_ALL, 32)
     t64 = tissue3d(m, capi.DPM3D_ALL, 64)
-    t32.set_positions(m.X)
-    t32.euler(50, 0.002)
-    X = t32.positions()
-    t64.set_positions(X.astype(np.float64))
-    # one more step of each: the forces it reports are those at X
-    t32.euler(1, 0.002)
-    t64.euler(1, 0.002)
-    F32, F64 = t32.forces(), t64.forces()
-    e = rel(F32, F64)
-    vmax = float(np.abs(F32 - F64).max())
-    print(f"cfg3 force error f32 vs f64: rel={e:.3e} vmax={vmax:.3e} scale={np.abs(F64).max():.3e}", flush=True)
-    assert e <= 1e-5
-    assert vmax <= 1e-4 * float(np.abs(F64).max())
+    t.set_positions(m.X)
+    worst = 0.0
+    for cp in range(11):
+        if cp:
+            t.euler(500, 0.002)
+        X = t.positions()
+        F32, F64, flip, counts = flipped_vertices(m, X, t32, t64)
+        keep = ~flip
+        e_all = float(np.linalg.norm(F32 - F64) / np.linalg.norm(F64))
+        e = float(np.linalg.norm(F32[keep] - F64[keep]) / np.linalg.norm(F64[keep]))
+        worst = max(worst, e)
+        print(f"step {500 * cp}: error {e:.3e} over {int(keep.sum())} vertices with equal decisions "
+              f"({e_all:.3e} over all); flipped: {counts}, {int(flip.sum())} vertices affected", flush=True)
+        assert e <= 1e-5
+        assert flip.sum() <= 1e-3 * len(flip)
+    print(f"worst over checkpoints: {worst:.3e}", flush=True)
 
 
 def test_energy_drift_f32_vs_f64():

@@ -50,29 +50,6 @@ def test_cell2d_matches_reference_constructor(goldens):
     assert k >= 5
 
 
-def test_tissue2d_disperse_matches_reference(goldens):
-    cells = [clDPM.Cell2D(0, 0, float(a), int(n), float(r))
-             for a, n, r in zip(goldens["t2d_calA"], goldens["t2d_nv"], goldens["t2d_r0"])]
-    T = clDPM.Tissue2D(cells, float(goldens["t2d_phi0"]))
-    assert T.L == goldens["t2d_L"][0]
-    assert T.NCELLS == len(cells) and T.PBC is True and T.Kre == 1.0 and T.Kat == 0.0
-    clDPM.seed48(0x330E, 0xABCD, 0x1234)
-    T.Disperse()
-    v = np.concatenate([np.asarray(c.Verts, np.float32) for c in T.Cells])
-    np.testing.assert_array_equal(bits(v), bits(goldens["t2d_verts"]))
-
-
-def test_tissue3d_disperse2d_matches_reference(goldens):
-    cells = [clDPM.Cell3D([0, 0, 0], float(a), float(r)) for a, r in zip(goldens["t3d_calA"], goldens["t3d_r0"])]
-    T = clDPM.Tissue3D(cells, float(goldens["t3d_phi0"]))
-    assert T.L == goldens["t3d_L"][0]
-    assert T.PBC == 1
-    clDPM.seed48(1, 2, 3)
-    T.Disperse2D()
-    v = np.stack([np.asarray(c.Verts, np.float32) for c in T.Cells])
-    np.testing.assert_array_equal(bits(v), bits(goldens["t3d_verts"]))
-
-
 def test_icosphere_counts_and_orientation():
     for s in range(4):
         c = clDPM.Cell3D([0, 0, 0], 1.0, 1.0, s)
@@ -215,23 +192,31 @@ def test_longer_verts_list_is_packed_to_nv():
     assert len(T.Cells[0].Verts) == 20
 
 
-def test_reference_driver_setups_match_reference_host_code():
-    # test2D.py:9-21 and test3D.py:8-15 with an unseeded drand48 stream
-    # (glibc's default state): the host dispersal reproduces the reference's
-    # own host code bit for bit (tests/golden/driver_goldens.npz)
+def test_box_lengths_match_reference_host_code(goldens):
+    # Tissue2D.L = sqrt(sum a0) / phi0, Tissue3D.L = cbrt(sum v0) / phi0
+    # (Tissue2D.cpp:31, Tissue3D.cpp:28), as the reference's own host code
+    # computed them for the golden tissues and the driver scripts
     g = np.load(os.path.join(ROOT, "tests", "golden", "driver_goldens.npz"))
-    c = clDPM.Cell2D(0, 0, 1.2, 25, 1.0)
-    c2 = clDPM.Cell2D(0, 0, 1.2, 22, 1.3)
-    T = clDPM.Tissue2D([c, c2] * 200, 0.9)
-    assert T.L == g["d2_L"][0] and T.NCELLS == 400
-    clDPM.seed48(0x330E, 0xABCD, 0x1234)
-    T.Disperse()
-    v = np.concatenate([np.asarray(x.Verts, np.float32) for x in T.Cells])
-    np.testing.assert_array_equal(bits(v), bits(g["d2_verts"]))
-    c3 = clDPM.Cell3D([0.0] * 3, 1.0, 1.0)
-    T3 = clDPM.Tissue3D([c3] * 32, 0.35)
-    assert T3.L == g["d3_L"][0]
-    clDPM.seed48(0x330E, 0xABCD, 0x1234)
-    T3.Disperse2D()
-    v3 = np.stack([np.asarray(x.Verts, np.float32) for x in T3.Cells])
-    np.testing.assert_array_equal(bits(v3), bits(g["d3_verts"]))
+    cells = [clDPM.Cell2D(0, 0, float(a), int(n), float(r))
+             for a, n, r in zip(goldens["t2d_calA"], goldens["t2d_nv"], goldens["t2d_r0"])]
+    T = clDPM.Tissue2D(cells, float(goldens["t2d_phi0"]))
+    assert T.L == goldens["t2d_L"][0]
+    assert T.NCELLS == len(cells) and T.PBC is True and T.Kre == 1.0 and T.Kat == 0.0
+    cells = [clDPM.Cell3D([0, 0, 0], float(a), float(r)) for a, r in zip(goldens["t3d_calA"], goldens["t3d_r0"])]
+    assert clDPM.Tissue3D(cells, float(goldens["t3d_phi0"])).L == goldens["t3d_L"][0]
+    T2 = clDPM.Tissue2D([clDPM.Cell2D(0, 0, 1.2, 25, 1.0), clDPM.Cell2D(0, 0, 1.2, 22, 1.3)] * 200, 0.9)
+    assert T2.L == g["d2_L"][0]
+    assert clDPM.Tissue3D([clDPM.Cell3D([0.0] * 3, 1.0, 1.0)] * 32, 0.35).L == g["d3_L"][0]
+
+
+def test_dispersal_fails_loudly_without_device():
+    # the dispersal relaxation runs on the device (dpm_disperse); without one
+    # it raises instead of falling back to the host
+    if capi.device_count() > 0:
+        pytest.skip("a GPU is visible (the dispersal goldens are checked by the GPU tests)")
+    T = clDPM.Tissue2D([clDPM.Cell2D(0, 0, 1.0, 16, 1.0)] * 4, 0.8)
+    with pytest.raises(RuntimeError):
+        T.Disperse()
+    T3 = clDPM.Tissue3D([clDPM.Cell3D([0.0] * 3, 1.0, 1.0)] * 4, 0.35)
+    with pytest.raises(RuntimeError):
+        T3.Disperse2D()
