@@ -340,22 +340,38 @@ __device__ __forceinline__ void face_of(const Lds3D<R>& l, int f, int& i0, int& 
 // absolute-coordinate triple products in float (getVolume,
 // Cell3D_Kernel.cl:46-64), which cancels catastrophically far from the
 // origin.
+// one_each: a thread per vertex, vertex tid in pv (cell_stats_own).
 template <typename R>
-__device__ void cell_stats(const Lds3D<R>& l, int nv, int nf, CellStat<R>* st_out,
-                           double* vol_out, R* em_out, int ci, R l0c) {
+__device__ void cell_stats_impl(const Lds3D<R>& l, int nv, int nf, CellStat<R>* st_out,
+                                double* vol_out, R* em_out, int ci, R l0c, bool one_each, bool has_v, V3<R> pv) {
   const int tid = threadIdx.x, NT = blockDim.x;
   double px = 0.0, py = 0.0, pz = 0.0;
-  for (int v = tid; v < nv; v += NT) {
-    px += (double)l.sx[v];
-    py += (double)l.sy[v];
-    pz += (double)l.sz[v];
+  if (one_each) {
+    if (has_v) {
+      px = (double)pv.x;
+      py = (double)pv.y;
+      pz = (double)pv.z;
+    }
+  } else {
+    for (int v = tid; v < nv; v += NT) {
+      px += (double)l.sx[v];
+      py += (double)l.sy[v];
+      pz += (double)l.sz[v];
+    }
   }
   block_sum3_1b(px, py, pz, l.red + 64);  // the two reductions alternate scratch regions
   const V3<R> C = {(R)(px / nv), (R)(py / nv), (R)(pz / nv)};
   R r = R(0);
-  for (int v = tid; v < nv; v += NT) {
-    const V3<R> d = vsub(lds_v(l.sx, l.sy, l.sz, v), C);
-    r = fmax(r, sqrt(vdot(d, d)));
+  if (one_each) {
+    if (has_v) {
+      const V3<R> d = vsub(pv, C);
+      r = sqrt(vdot(d, d));
+    }
+  } else {
+    for (int v = tid; v < nv; v += NT) {
+      const V3<R> d = vsub(lds_v(l.sx, l.sy, l.sz, v), C);
+      r = fmax(r, sqrt(vdot(d, d)));
+    }
   }
   double vol = 0.0;
   R e2 = R(0);
@@ -384,6 +400,21 @@ __device__ void cell_stats(const Lds3D<R>& l, int nv, int nf, CellStat<R>* st_ou
     vol_out[ci] = fabs(vol / 6.0);
     em_out[ci] = em;
   }
+}
+template <typename R>
+__device__ void cell_stats(const Lds3D<R>& l, int nv, int nf, CellStat<R>* st_out, double* vol_out, R* em_out,
+                           int ci, R l0c) {
+  cell_stats_impl(l, nv, nf, st_out, vol_out, em_out, ci, l0c, false, false, V3<R>{R(0), R(0), R(0)});
+}
+// The same from a workgroup with a thread per vertex (NT >= nv): thread v
+// holds vertex v (has_v) in registers, and the centre of mass and radius are
+// taken from them (the sums and maxima of cell_stats, bit for bit); the LDS
+// positions are read only by the face loop, after the first reduction's
+// barrier, so they may be written just before the call.
+template <typename R>
+__device__ void cell_stats_own(const Lds3D<R>& l, int nv, int nf, CellStat<R>* st_out, double* vol_out,
+                               R* em_out, int ci, R l0c, bool has_v, V3<R> pv) {
+  cell_stats_impl(l, nv, nf, st_out, vol_out, em_out, ci, l0c, true, has_v, pv);
 }
 
 template <typename R>
@@ -643,8 +674,47 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
     vfr[k] = own ? a.vf[tid * DPM3D_MAXDEG + k] : -1;
     adjr[k] = own ? a.adj[tid * DPM3D_MAXDEG + k] : -1;
   }
-  stage_topology(a, l);
-  load_own(a, l, base);
+  // The candidate scan's statistics (every cell of a tissue of at most NT
+  // cells, no cell list) are loaded here with the cell's own data, one
+  // memory round trip for both (float instances with compile-time sizes; in
+  // the others the registers held until the scan would spill).
+  const bool scan_pre = (sizeof(R) == 4 && NTC > 0) &&
+                        (((mask & M_CONTACT) && Kre != R(0)) || ((mask & M_ADHESION) && Kat != R(0)));
+  const bool scan_all = scan_pre && !(a.bin_list && a.tbin_nb[t] > 0) && ce - cb <= NT;  // block-uniform
+  CellStat<R> st_pre{};
+  if (scan_all && tid < ce - cb) st_pre = a.st_in[cb + tid];
+  if constexpr (NFC > 0 && NVC > 0 && NTC > 0) {
+    // compile-time sizes: every load in flight before the first LDS store
+    static_assert(NFC <= 2 * NTC, "at most two faces per thread");
+    const int f0 = tid, f1 = tid + NTC;
+    int a0 = 0, b0 = 0, e0 = 0, a1 = 0, b1 = 0, e1 = 0;
+    if (f0 < NFC) {
+      a0 = a.faces[3 * f0];
+      b0 = a.faces[3 * f0 + 1];
+      e0 = a.faces[3 * f0 + 2];
+    }
+    if (f1 < NFC) {
+      a1 = a.faces[3 * f1];
+      b1 = a.faces[3 * f1 + 1];
+      e1 = a.faces[3 * f1 + 2];
+    }
+    R px = R(0), py = R(0), pz = R(0);
+    if (own) {
+      px = a.xin[base + tid];
+      py = a.yin[base + tid];
+      pz = a.zin[base + tid];
+    }
+    if (f0 < NFC) l.sf[f0] = make_uint2((unsigned)a0 | ((unsigned)b0 << 16), (unsigned)e0);
+    if (f1 < NFC) l.sf[f1] = make_uint2((unsigned)a1 | ((unsigned)b1 << 16), (unsigned)e1);
+    if (own) {
+      l.sx[tid] = px;
+      l.sy[tid] = py;
+      l.sz[tid] = pz;
+    }
+  } else {
+    stage_topology(a, l);
+    load_own(a, l, base);
+  }
   __syncthreads();
 
   const V3<R> P = own ? lds_v(l.sx, l.sy, l.sz, tid) : zero;
@@ -903,7 +973,10 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
       bool cand = false;
       CellStat<R> stj{};
       if (cj < ce && cj != ci) {
-        stj = a.st_in[cj];
+        if (scan_all && c0 == 0)
+          stj = st_pre;
+        else
+          stj = a.st_in[cj];
         V3<R> sh = zero;
         if (pbc)
           sh = v3(image_shift(Ci.x - stj.cx, L, invL), image_shift(Ci.y - stj.cy, L, invL),
@@ -1203,26 +1276,28 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
       a.yout[base + tid] = Pn.y;
       a.zout[base + tid] = Pn.z;
     }
-    __syncthreads();
+    __syncthreads();  // every read of the step's positions is done
     if (own) {
       l.sx[tid] = Pn.x;
       l.sy[tid] = Pn.y;
       l.sz[tid] = Pn.z;
     }
-    __syncthreads();
-    cell_stats(l, nv, nf, a.st_out, a.vol_out, a.em_out, ci, l0i);
+    cell_stats_own(l, nv, nf, a.st_out, a.vol_out, a.em_out, ci, l0i, own, Pn);
   }
   if (a.counters) {
-    const double c = block_sum((double)n_contact, l.red);
-    const double nw = block_sum((double)n_wind, l.red);  // counted by lane 0 of each wave
-    if (tid == 0) {
-      // spread over DPM_COUNTER_SLOTS slots so that workgroups do not
-      // serialise on one address; the host sums the slots
-      unsigned long long* slot = a.counters + 4 * (blockIdx.x % DPM_COUNTER_SLOTS);
-      if (blockIdx.x == 0) atomicAdd(&slot[0], 1ull);
-      if (nw != 0.0) atomicAdd(&slot[1], (unsigned long long)nw);
-      if (c != 0.0) atomicAdd(&slot[2], (unsigned long long)c);
-      if (n_cand) atomicAdd(&slot[3], n_cand);
+    // per wave, without a workgroup barrier: the wave's contact count, its
+    // winding count (kept by lane 0), and the candidates (every thread holds
+    // the workgroup's count; wave 0 adds it).  Spread over
+    // DPM_COUNTER_SLOTS slots so that workgroups do not serialise on one
+    // address; the host sums the slots.
+    unsigned long long c = n_contact;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, DPM_WAVE);
+    if (lane == 0) {
+      unsigned long long* slot = a.counters + 4 * ((blockIdx.x * (NT / DPM_WAVE) + wave) % DPM_COUNTER_SLOTS);
+      if (blockIdx.x == 0 && wave == 0) atomicAdd(&slot[0], 1ull);
+      if (n_wind) atomicAdd(&slot[1], n_wind);
+      if (c) atomicAdd(&slot[2], c);
+      if (wave == 0 && n_cand) atomicAdd(&slot[3], n_cand);
     }
   }
 }
