@@ -47,11 +47,37 @@ enum : int {
   F_MOVE = 2,
 };
 
+// Per-cell statistics of a step's input positions (written by the previous
+// step's epilogue or k3d_stats): centre of mass, bounding radius, longest
+// edge, target edge length l0, and 1 / max(l0, 1e-12) in double (the
+// adhesion term's, divided once per cell instead of once per pair).  Eight
+// reals either way (32 bytes in float, 64 in double).
 template <typename R>
-struct CellStat {
-  R cx, cy, cz, r;  // centre of mass, bounding radius
-  R em, l0, pad0, pad1;  // longest edge, target edge length l0
+struct CellStat;
+template <>
+struct CellStat<float> {
+  float cx, cy, cz, r;
+  float em, l0;
+  double inv_l0;
 };
+template <>
+struct CellStat<double> {
+  double cx, cy, cz, r;
+  double em, l0;
+  double inv_l0, pad;
+};
+template <typename R>
+__host__ __device__ inline CellStat<R> make_cell_stat(R cx, R cy, R cz, R r, R em, R l0) {
+  CellStat<R> s{};
+  s.cx = cx;
+  s.cy = cy;
+  s.cz = cz;
+  s.r = r;
+  s.em = em;
+  s.l0 = l0;
+  s.inv_l0 = 1.0 / (double)(l0 > R(1e-12f) ? l0 : R(1e-12f));
+  return s;
+}
 
 template <typename R>
 struct alignas(4 * sizeof(R)) Q4 {
@@ -200,7 +226,7 @@ __device__ __forceinline__ bool outside_by_margin(const R* xs, const R* ys, cons
   R h = R(-3.0e38f);
   for (int v = lane; v < nv; v += DPM_WAVE)
     h = fmax(h, ((xs[v] - cx) * d.x + (ys[v] - cy) * d.y) + (zs[v] - cz) * d.z);
-  for (int o = 32; o > 0; o >>= 1) h = fmax(h, __shfl_xor(h, o, DPM_WAVE));
+  h = wave_max_all(h);
   const R dn = sqrt(vdot(d, d));
   const R M = ((fabs(cx) + fabs(cy)) + fabs(cz)) + ((fabs(Cj.x) + fabs(Cj.y)) + fabs(Cj.z));
   return dn > R(0) && dn * dn - h >= ((m + R(1e-4f) * (dn + Rj)) + R(1e-6f) * M) * dn;
@@ -396,7 +422,7 @@ __device__ void cell_stats_impl(const Lds3D<R>& l, int nv, int nf, CellStat<R>* 
     // Slack of a few ulp so that the culling tests are never tighter than the
     // largest distance a vertex can have from C, or the longest edge.
     const R em = sqrt(e2) * R(1.000001f);
-    st_out[ci] = CellStat<R>{C.x, C.y, C.z, r * R(1.0000005f) + R(1e-30f), em, l0c, R(0), R(0)};
+    st_out[ci] = make_cell_stat<R>(C.x, C.y, C.z, r * R(1.0000005f) + R(1e-30f), em, l0c);
     vol_out[ci] = fabs(vol / 6.0);
     em_out[ci] = em;
   }
@@ -770,19 +796,21 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
       bool bad = false;
       V3<R> Fr = zero;
       if (own) {
+        // (Force arithmetic, its products fused into FMAs; the short-edge
+        // test keeps a factor 2 of margin to the reference's 1e-12 cut.)
         const V3<R> d0 = vsub(lds_v(l.sx, l.sy, l.sz, adjr[0]), P);
         V3<R> dq = d0, Fv = zero, Fs = zero, sb = zero;
 #pragma unroll
         for (int q = 0; q < DPM3D_MAXDEG; q++) {
           if (surf_on) {
-            const R d2 = vdot(dq, dq);
+            const R d2 = vdot_f(dq, dq);
             bad = bad || d2 < R(4e-24f);
-            Fs = vadd(Fs, vscale(dq, spring_factor(d2, inv_l0)));
+            Fs = vmadd_f(dq, spring_factor(d2, inv_l0), Fs);
           }
           sb = vadd(sb, dq);
           const int un = q + 1 < DPM3D_MAXDEG ? adjr[q + 1] : -1;
           const V3<R> dn = un < 0 ? d0 : vsub(lds_v(l.sx, l.sy, l.sz, un), P);
-          if (vol_on) Fv = vadd(Fv, vcross(dq, dn));
+          if (vol_on) Fv = vcross_add_f(dq, dn, Fv);
           if (un < 0) break;
           dq = dn;
         }
@@ -1047,10 +1075,8 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
             hi = fmax(hi, vdot(vsub(lds_v(l.sx, l.sy, l.sz, v), Ci), nC));
             hj = fmax(hj, vdot(v3(stj.cx - a.xin[bj + v], stj.cy - a.yin[bj + v], stj.cz - a.zin[bj + v]), nC));
           }
-          for (int o = 32; o > 0; o >>= 1) {
-            hi = fmax(hi, __shfl_xor(hi, o, DPM_WAVE));
-            hj = fmax(hj, __shfl_xor(hj, o, DPM_WAVE));
-          }
+          hi = wave_max_all(hi);
+          hj = wave_max_all(hj);
           bool skip_contact = false, skip_adh = false;
           const R Mc = ((fabs(Ci.x) + fabs(Ci.y)) + fabs(Ci.z)) + ((fabs(Cj.x) + fabs(Cj.y)) + fabs(Cj.z));
           if (D > R(0)) {
@@ -1187,7 +1213,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
               const R half_l = R(0.49f) * L;
               const R lim_own = (L0i * R(2)) * (L0i * R(2)) * R(1.00001f);
               const R lim_rev = (L0j * R(2)) * (L0j * R(2)) * R(1.00001f);
-              const double kh = 0.5 * (double)Kat, inv_l0i = 1.0 / (double)L0i, inv_l0j = 1.0 / (double)L0j;
+              const double kh = 0.5 * (double)Kat, inv_l0i = sti.inv_l0, inv_l0j = stj.inv_l0;
               const int m1 = min(nnear, (sl + 1) * per);
               for (int m = sl * per; m < m1; m++) {
                 const int vj = jl[m];

@@ -229,10 +229,8 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   const R sc = Kai * sqrt(a0i) / l0i * R(0.3f);
   const R inv_l0 = R(1) / l0i;
   // Face pre-pass: the facing-down faces (Cell3D_Kernel.cl:206-246), one bit
-  // each, and the shortest edge of the cell.
-  bool degenerate = false;
-  if (shape_on && (sub_on || surf_on)) {  // wave-uniform
-    R e2min = R(3.0e38f);
+  // each.
+  if (shape_on && sub_on) {  // wave-uniform
     #pragma unroll 1
     for (int f0 = 0; f0 < nf; f0 += DPM_WAVE) {
       const int f = f0 + lane;
@@ -241,29 +239,25 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         int i0, i1, i2;
         wface(sf, f, i0, i1, i2);
         const V3<R> P0 = lds_v(sx, sy, sz, i0), P1 = lds_v(sx, sy, sz, i1), P2 = lds_v(sx, sy, sz, i2);
-        const V3<R> l0v = vsub(P1, P0), l2v = vsub(P0, P2);
-        if (sub_on) {
-          dn = facing_down(vcross(l0v, vsub(P2, P0)));
-          if (PROBE && dn && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
-        }
-        const V3<R> l1v = vsub(P2, P1);
-        e2min = fmin(e2min, fmin(vdot(l0v, l0v), fmin(vdot(l1v, l1v), vdot(l2v, l2v))));
+        dn = facing_down(vcross(vsub(P1, P0), vsub(P2, P0)));
+        if (PROBE && dn && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
       }
-      if (sub_on) {
-        const unsigned long long m = __ballot(dn);
-        if (lane == 0) {
-          down[f0 / 32] = (unsigned)m;
-          if (f0 / 32 + 1 < (nf + 31) / 32) down[f0 / 32 + 1] = (unsigned)(m >> 32);
-        }
+      const unsigned long long m = __ballot(dn);
+      if (lane == 0) {
+        down[f0 / 32] = (unsigned)m;
+        if (f0 / 32 + 1 < (nf + 31) / 32) down[f0 / 32 + 1] = (unsigned)(m >> 32);
       }
     }
-    // an edge shorter than 1e-12 makes the reference skip its faces' edge
-    // springs; such cells take the per-corner form below
-    degenerate = wmin_all((double)e2min) < 4e-24;
     wave_lds_sync();
   }
   const bool bend_on = (mask & M_BENDING) && Kbi != R(0);
-  const bool ring_pass = shape_on && !degenerate && a.closed;  // wave-uniform
+  // The ring walk, unless an edge is shorter than 2e-12 (an edge below 1e-12
+  // makes the reference skip its faces' edge springs): the walk checks every
+  // ring edge, and a cell with such an edge is taken again, every vertex by
+  // the per-corner form (its springs skip those faces as the reference does).
+  bool ring_pass = shape_on && a.closed;  // wave-uniform
+  for (int pass = 0; pass < 2; pass++) {
+  bool short_edge = false;
   #pragma unroll 1
   for (int v = lane; v < nv; v += DPM_WAVE) {
     const V3<R> P = lds_v(sx, sy, sz, v);
@@ -280,15 +274,20 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       //   s(d) = d (|d| / l0 - 1) / |d| and a, b its other corners, and every
       //   ring edge lies on two of the vertex's faces: 2 sc sum_q s(d_q);
       // bending's first sum, sum_q d_q, kept in nx for its second pass.
+      // (Force arithmetic only, no cut-off: the products fuse into FMAs.)
       const uint4 arow = rows[LW.adj / 16 + v];
       const V3<R> d0 = vsub(lds_v(sx, sy, sz, row16(arow, 0)), P);
       V3<R> dq = d0, Fv = zero, Fs = zero, sb = zero;
       for (int q = 0; q < DPM3D_MAXDEG; q++) {
-        if (surf_on) Fs = vadd(Fs, vscale(dq, spring_factor(vdot(dq, dq), inv_l0)));
+        if (surf_on) {
+          const R d2 = vdot_f(dq, dq);
+          short_edge = short_edge || d2 < R(4e-24f);
+          Fs = vmadd_f(dq, spring_factor(d2, inv_l0), Fs);
+        }
         sb = vadd(sb, dq);
         const int un = q + 1 < DPM3D_MAXDEG ? row16(arow, q + 1) : 0xffff;
         const V3<R> dn = un == 0xffff ? d0 : vsub(lds_v(sx, sy, sz, un), P);
-        if (vol_on) Fv = vadd(Fv, vcross(dq, dn));
+        if (vol_on) Fv = vcross_add_f(dq, dn, Fv);
         if (un == 0xffff) break;
         dq = dn;
       }
@@ -345,6 +344,9 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
     fx[v] = F.x;
     fy[v] = F.y;
     fz[v] = F.z;
+  }
+  if (!(ring_pass && surf_on && __ballot(short_edge) != 0ull)) break;  // wave-uniform
+  ring_pass = false;
   }
 
   // --------------------------------------------------------------- bending
@@ -573,8 +575,8 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
           nnear += __popcll(mn);
           nmine += __popcll(mm);
         }
-        hp = wmax_r(hp);
-        for (int o = 32; o > 0; o >>= 1) hq = fmin(hq, __shfl_xor(hq, o, DPM_WAVE));
+        hp = wave_max_all(hp);
+        hq = wave_min_all(hq);
         // the separation less slack for the float sums (relative 1e-4 of the
         // cells' extent, 1e-6 of the coordinates); -1 when it is not defined
         const R Mc = ((fabs(Ci.x) + fabs(Ci.y)) + fabs(Ci.z)) + ((fabs(Cj.x) + fabs(Cj.y)) + fabs(Cj.z));
@@ -590,7 +592,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
           const R half_l = R(0.49f) * L;
           const R lim_own = (L0i * R(2)) * (L0i * R(2)) * R(1.00001f);
           const R lim_rev = (L0j * R(2)) * (L0j * R(2)) * R(1.00001f);
-          const double kh = 0.5 * (double)Kat, inv_l0i = 1.0 / (double)L0i, inv_l0j = 1.0 / (double)L0j;
+          const double kh = 0.5 * (double)Kat, inv_l0i = sti.inv_l0, inv_l0j = stj.inv_l0;
           for (int tw = lane; tw < S * nmine; tw += DPM_WAVE) {
             const int iv = tw % nmine, sl = tw / nmine;
             const V3<R> Pv = lds_v(sx, sy, sz, (int)il[iv]);
@@ -787,11 +789,11 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       const V3<R> e0 = vsub(P1, P0), e1 = vsub(P2, P1), ee = vsub(P0, P2);
       e2 = fmax(e2, fmax(vdot(e0, e0), fmax(vdot(e1, e1), vdot(ee, ee))));
     }
-    const R rd = wmax_r(r), e2d = wmax_r(e2);  // maxima are exact in either precision
+    const R rd = wave_max_all(r), e2d = wave_max_all(e2);  // maxima are exact in either precision
     vol = wsum_all(vol);
     if (lane == 0) {
       const R em = sqrt((R)e2d) * R(1.000001f);
-      a.st_out[ci] = CellStat<R>{C.x, C.y, C.z, (R)rd * R(1.0000005f) + R(1e-30f), em, l0i, R(0), R(0)};
+      a.st_out[ci] = make_cell_stat<R>(C.x, C.y, C.z, (R)rd * R(1.0000005f) + R(1e-30f), em, l0i);
       a.vol_out[ci] = fabs(vol / 6.0);
       a.em_out[ci] = em;
     }

@@ -66,6 +66,21 @@ template <typename R>
 __device__ __forceinline__ V3<R> vcross(V3<R> a, V3<R> b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
+// Fused forms for force arithmetic that decides no cut-off (the sources are
+// compiled without contraction so that every decision keeps the reference's
+// rounding; these fuse where it does not matter).
+template <typename R>
+__device__ __forceinline__ R vdot_f(V3<R> a, V3<R> b) { return fma(a.x, b.x, fma(a.y, b.y, a.z * b.z)); }
+// c + a s
+template <typename R>
+__device__ __forceinline__ V3<R> vmadd_f(V3<R> a, R s, V3<R> c) {
+  return {fma(a.x, s, c.x), fma(a.y, s, c.y), fma(a.z, s, c.z)};
+}
+// c + a x b
+template <typename R>
+__device__ __forceinline__ V3<R> vcross_add_f(V3<R> a, V3<R> b, V3<R> c) {
+  return {fma(a.y, b.z, fma(-a.z, b.y, c.x)), fma(a.z, b.x, fma(-a.x, b.z, c.y)), fma(a.x, b.y, fma(-a.y, b.x, c.z))};
+}
 template <typename R>
 __device__ __forceinline__ V3<R> vnormalize(V3<R> a) {
   const R d = vdot(a, a);
@@ -114,6 +129,46 @@ __device__ __forceinline__ float wave_max(float v) {
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_down(v, o, DPM_WAVE));
   return v;
 }
+
+// Wave-wide maximum and minimum of a float, every lane receiving the result
+// (a wave-uniform value), by DPP row operations: quad swaps, the half-row
+// and row mirrors, then the row broadcasts, which leave the full result in
+// lane 63 — no LDS round trip per step, as a shuffle (ds_bpermute) takes.
+// Call with every lane of the wave active.  Maxima and minima are exact, so
+// the result is the shuffle reduction's bit for bit.  Doubles take the
+// shuffles.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+__device__ __forceinline__ float wave_max_f(float v) {
+  v = fmaxf(v, dpp_f<0xB1, 0xF>(v));   // quad_perm(1, 0, 3, 2)
+  v = fmaxf(v, dpp_f<0x4E, 0xF>(v));   // quad_perm(2, 3, 0, 1)
+  v = fmaxf(v, dpp_f<0x141, 0xF>(v));  // row_half_mirror
+  v = fmaxf(v, dpp_f<0x140, 0xF>(v));  // row_mirror
+  v = fmaxf(v, dpp_f<0x142, 0xA>(v));  // row_bcast15 into rows 1 and 3
+  v = fmaxf(v, dpp_f<0x143, 0xC>(v));  // row_bcast31 into rows 2 and 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ float wave_min_f(float v) {
+  v = fminf(v, dpp_f<0xB1, 0xF>(v));
+  v = fminf(v, dpp_f<0x4E, 0xF>(v));
+  v = fminf(v, dpp_f<0x141, 0xF>(v));
+  v = fminf(v, dpp_f<0x140, 0xF>(v));
+  v = fminf(v, dpp_f<0x142, 0xA>(v));
+  v = fminf(v, dpp_f<0x143, 0xC>(v));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double wave_max_all(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, DPM_WAVE));
+  return v;
+}
+__device__ __forceinline__ float wave_max_all(float v) { return wave_max_f(v); }
+__device__ __forceinline__ double wave_min_all(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, DPM_WAVE));
+  return v;
+}
+__device__ __forceinline__ float wave_min_all(float v) { return wave_min_f(v); }
 
 // Block-wide sum of one double per thread; every thread gets the result.
 // scratch: >= 17 doubles of LDS.  Contains __syncthreads: call from all threads.

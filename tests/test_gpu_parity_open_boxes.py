@@ -8,7 +8,10 @@ image rounding.  The boxes here (6 x 6 and 8 x 8 monolayers, and the
 256-cell monolayer itself) take those paths; every result is compared with
 the oracle from the same float inputs:
   * forces: ||F_gpu - F_ref64|| / ||F_ref64|| <= 1e-5 and every vertex within
-    1e-4 * max|F_ref64|;
+    1e-4 * max|F_ref64|, over the vertices where the float and the double
+    restatement take every cut-off decision alike (at most 32 or 1e-3 of the
+    vertices are set aside: the float step decides as ref32, and where ref64
+    decides otherwise its force moves by O(Kat) or O(Ks));
   * contact (vertex, cell) pairs and facing-down faces: bit-exact vs ref32;
   * adhesion: per vertex the partner count and the key sums of the partners
     its own loop accepts and of the vertices whose loop accepts it
@@ -49,14 +52,56 @@ def oracle_pair(m, X, lib="liboracle.so"):
     return r32, r64
 
 
-def check(g, r32, r64, what):
+def oracle_flips(m, X, r32, r64):
+    """Vertices where the float and the double restatement take a cut-off
+    decision of the force law differently at the same positions: adhesion
+    partner counts (the vertex, and every vertex of another cell within 2 l0
+    of it, which receives the pair's reaction), facing-down faces (their
+    corners), and the substrate band |z| < 2 l0 at the corners of
+    facing-down faces (l0 = sqrt(4 a0) / sqrt(3) rounded to float in one and
+    to double in the other).  The float step decides as ref32 does (the pair
+    sets below are compared bit for bit); at such a vertex its force moves
+    by O(Kat) or O(Ks) from ref64's."""
+    nv = m.mesh.nv
+    faces = np.asarray(m.mesh.faces)
+    flip = np.zeros(len(X), bool)
+    fa = np.asarray(r32["adhesion"]) != np.asarray(r64["adhesion"])
+    if fa.any():
+        from scipy.spatial import cKDTree
+        L = float(m.L)
+        P = np.asarray(X, np.float64).copy()
+        P[:, :2] = np.mod(P[:, :2], L)
+        P[:, 2] -= P[:, 2].min()
+        tree = cKDTree(P, boxsize=[L, L, P[:, 2].max() + 10.0])
+        l0 = np.sqrt(4 * float(np.max(m.a0))) / np.sqrt(3)
+        for i in np.nonzero(fa)[0]:
+            flip[tree.query_ball_point(P[i], 2 * l0 * 1.01)] = True
+    d32, d64 = np.asarray(r32["down"]) != 0, np.asarray(r64["down"]) != 0
+    for c, f in zip(*np.nonzero(d32 != d64)):
+        flip[c * nv + faces[f]] = True
+    a0 = np.broadcast_to(np.asarray(m.a0, np.float32), (m.C,))
+    band32 = (np.sqrt(np.float32(4) * a0) / np.sqrt(np.float32(3))) * np.float32(2)
+    band64 = (np.sqrt(4.0 * a0.astype(np.float64)) / np.sqrt(3.0)) * 2.0
+    z = np.abs(np.asarray(X, np.float32)[:, 2]).reshape(m.C, nv)
+    differs = (z < band32[:, None]) != (z.astype(np.float64) < band64[:, None])
+    corner = np.zeros((m.C, nv), bool)
+    cd, fd = np.nonzero(d32 | d64)
+    corner[cd[:, None], faces[fd]] = True
+    flip |= (differs & corner).ravel()
+    return flip
+
+
+def check(g, r32, r64, what, flip):
     F64 = r64["forces"]
     Fg = np.asarray(g["forces"], np.float64)
-    r = float(np.linalg.norm(Fg - F64) / np.linalg.norm(F64))
-    vmax = float(np.abs(Fg - F64).max())
+    keep = ~flip
+    r = float(np.linalg.norm(Fg[keep] - F64[keep]) / np.linalg.norm(F64[keep]))
+    vmax = float(np.abs(Fg[keep] - F64[keep]).max())
     scale = float(np.abs(F64).max())
     log(f"  {what}: rel={r:.3e} vmax={vmax:.3e} scale={scale:.3e} contact={int(g['contact'].sum())} "
-        f"down={int(g['down'].sum())} adhesion={int(g['adhesion'].sum())}")
+        f"down={int(g['down'].sum())} adhesion={int(g['adhesion'].sum())}; "
+        f"vertices at a float/double cut-off flip {int(flip.sum())}")
+    assert flip.sum() <= max(32, 1e-3 * len(flip)), (what, int(flip.sum()))
     assert r <= FORCE_RTOL, (what, r)
     assert vmax <= FORCE_VTOL * scale, (what, vmax, scale)
     np.testing.assert_array_equal(g["contact"], r32["contact"])
@@ -144,13 +189,14 @@ def test_open_box_start_and_after_200_and_2000_steps(case):
             f"bins {nb}")
         assert ni > 0 and nb >= 3
         r32, r64 = oracle_pair(m, X)
+        flip = oracle_flips(m, X, r32, r64)
         for kernel in (1, 2):
             for cl in (0, 1):
                 t = tissue3d(m)
                 t.set_step_kernel(kernel)
                 t.set_cell_list(cl)
                 t.set_positions(X)
-                check(t.probe(), r32, r64, f"{case} {label} kernel={kernel} cell_list={cl}")
+                check(t.probe(), r32, r64, f"{case} {label} kernel={kernel} cell_list={cl}", flip)
         adhesion += int(r32["adhesion"].sum())
         contact += int(r32["contact"].sum())
     # along the trajectory: adhesion pairs, the separation skip on pairs the
@@ -174,13 +220,14 @@ def test_headline_monolayer_start_and_after_200_steps():
             f"bins {expected_bins(m, X)}")
         assert ni > 0
         r32, r64 = oracle_pair(m, X, lib="liboracle_omp.so")
+        flip = oracle_flips(m, X, r32, r64)
         for kernel in (1, 2):
             for cl in (0, 1):
                 t = tissue3d(m)
                 t.set_step_kernel(kernel)
                 t.set_cell_list(cl)
                 t.set_positions(X)
-                check(t.probe(), r32, r64, f"cfg2 {label} kernel={kernel} cell_list={cl}")
+                check(t.probe(), r32, r64, f"cfg2 {label} kernel={kernel} cell_list={cl}", flip)
 
 
 @pytest.mark.parametrize("kernel", [1, 2])
