@@ -159,6 +159,46 @@ def plain2d_gpu(capi, hiptime, W, steps=1000, dt=0.005):
                "steps": steps, "ms": ms, "value": nvt * steps / (ms / 1e3), "unit": "vertex-steps/s"}
 
 
+def driver2d_gpu(capi, hiptime, steps=1000, dt=0.005):
+    """The reference's test2D.py tissue (test2D.py:9-31): 200 pairs of
+    Cell2D(0, 0, 1.2, 25, 1.0) and Cell2D(0, 0, 1.2, 22, 1.3), Ka 0.1, Kl 1,
+    Kb 0.05, Tissue2D(cells, 0.9), Kre 1, Kat 0.5, Disperse (on the device,
+    drand48 seeded as an unseeded reference run), then Euler steps of
+    dt = 0.005: 400 cells, 9,400 vertices.  Both 2D step forms, timed with HIP
+    events after a warm-up run from the same start."""
+    from opencl_dpm_amd import clDPM
+    c, c2 = clDPM.Cell2D(0, 0, 1.2, 25, 1.0), clDPM.Cell2D(0, 0, 1.2, 22, 1.3)
+    for x in (c, c2):
+        x.Ka, x.Kl, x.Kb = 0.1, 1.0, 0.05
+    T = clDPM.Tissue2D([c, c2] * 200, 0.9)
+    T.Kre, T.Kat = 1.0, 0.5
+    clDPM.seed48(0x330E, 0xABCD, 0x1234)
+    T.Disperse()
+    X0 = np.concatenate([np.asarray(x.Verts, np.float32) for x in T.Cells])
+    nvs = np.array([x.NV for x in T.Cells], np.int32)
+    a0 = np.array([x.a0 for x in T.Cells], np.float32)
+    l0 = np.array([x.l0 for x in T.Cells], np.float32)
+    r0 = np.array([x.r0 for x in T.Cells], np.float32)
+    out = {"workload": "test2D.py's tissue: 400 cells (25 and 22 vertices), packing 0.9, dispersed, "
+                       f"{steps} Euler steps, dt={dt}", "steps": steps, "vertices": int(nvs.sum())}
+    for kind, name in ((1, "one_workgroup"), (2, "wide")):
+        t = capi.Tissues2D([0, len(nvs)], nvs, 0.1, 1.0, 0.05, a0, l0, r0, T.L, Kre=1.0, Kat=0.5)
+        t.set_step_kernel(kind)
+        t.set_positions(X0)
+        t.euler(steps, dt)  # warm-up (captures the wide form's step graph)
+        t.set_positions(X0)
+        e0, e1 = hiptime.Event(), hiptime.Event()
+        s = t.stream()
+        e0.record(s)
+        t.euler_async(steps, dt)
+        e1.record(s)
+        ms = e0.elapsed_ms(e1)
+        out[name] = {"ms": ms, "us_per_step": ms * 1e3 / steps,
+                     "value": float(nvs.sum()) * steps / (ms / 1e3), "unit": "vertex-steps/s"}
+        del t
+    return out
+
+
 def plain2d_cpu(p, steps=1000, dt=0.005):
     """The same 2D run by the oracle's float restatement of the reference kernels,
     serial on one host core."""
@@ -302,6 +342,9 @@ def main():
     p2, out["reference_2d"] = plain2d_gpu(capi, hiptime, W)
     log(f"[bench] 2D reference case: {out['reference_2d']['ms']:.2f} ms for 1000 steps "
         f"({out['reference_2d']['value']:.4g} vertex-steps/s)")
+    out["driver_2d"] = driver2d_gpu(capi, hiptime)
+    log(f"[bench] test2D.py tissue (400 cells): one workgroup {out['driver_2d']['one_workgroup']['us_per_step']:.1f} "
+        f"us/step, wide {out['driver_2d']['wide']['us_per_step']:.1f} us/step")
     if not args.no_cpu:
         log(f"[bench] cpu baseline: {args.cpu_steps} step(s)")
         out["cpu_baseline"] = cpu_baseline(m, args.cpu_steps)

@@ -66,20 +66,8 @@ __device__ __forceinline__ void wface(const unsigned* sf, int f, int& i0, int& i
   i2 = (int)(w >> 20);
 }
 
-// Butterfly reductions: every lane receives the result.
-__device__ __forceinline__ double wsum_all(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, DPM_WAVE);
-  return v;
-}
-__device__ __forceinline__ double wmin_all(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, DPM_WAVE));
-  return v;
-}
-template <typename R>
-__device__ __forceinline__ R wmax_r(R v) {  // in the value's own precision
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, DPM_WAVE));
-  return v;
-}
+// Wave sums: every lane receives the result (DPP tree, fixed order).
+__device__ __forceinline__ double wsum_all(double v) { return wave_sum_d(v); }
 
 // Set bits of a ballot in the lanes below this one (v_mbcnt).
 __device__ __forceinline__ int lanes_below(unsigned long long m) {

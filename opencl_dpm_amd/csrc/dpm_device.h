@@ -170,6 +170,31 @@ __device__ __forceinline__ double wave_min_all(double v) {
 }
 __device__ __forceinline__ float wave_min_all(float v) { return wave_min_f(v); }
 
+// Wave-wide sum of a double by the same DPP steps (each moves the two halves;
+// a row that a row broadcast does not write adds 0), the result in every
+// lane: lane 63's, read back as a wave-uniform value.  The additions are
+// grouped by the DPP tree (quads, eights, rows, then rows 0+1, 2+3, all):
+// a fixed order, so the sum is deterministic.  Every lane active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_d(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+  v += dpp_d<0xB1, 0xF>(v);
+  v += dpp_d<0x4E, 0xF>(v);
+  v += dpp_d<0x141, 0xF>(v);
+  v += dpp_d<0x140, 0xF>(v);
+  v += dpp_d<0x142, 0xA>(v);
+  v += dpp_d<0x143, 0xC>(v);
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Block-wide sum of one double per thread; every thread gets the result.
 // scratch: >= 17 doubles of LDS.  Contains __syncthreads: call from all threads.
 __device__ __forceinline__ double block_sum(double v, double* scratch) {
