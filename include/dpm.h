@@ -309,8 +309,9 @@ int dpm2d_probe_attraction(dpm_tissue* h, unsigned* keys);
  * Tissue2D.cpp:212): 1 one workgroup per tissue, the whole call in one launch
  * with the tissue in LDS; 2 the wide form, three launches per step over the
  * whole device on global memory (any tissue size, float tissues); 0
- * (default) the first where every tissue fits one workgroup, else the
- * second.  Both make the same decisions and forces bit for bit.  FIRE,
+ * (default) the second for tissues beyond one workgroup and for handles of
+ * at most 64 tissues whose largest has 128 cells or more, else the first.
+ * Both make the same decisions and forces bit for bit.  FIRE,
  * jamming, energies and double tissues need tissues that fit one
  * workgroup. */
 int dpm2d_set_step_kernel(dpm_tissue* h, int kind);

@@ -360,12 +360,13 @@ __device__ void over_pass(const Tissue2DCtx<R>& T, const Lds2D<R>& l, unsigned* 
 // accumulates this vertex's share of the conservative energy (perimeter edge
 // k -> k+1, bending at k, the cell's area energy at its first vertex) and
 // with `e_int` half of each vertex-repulsion pair energy.  With `over_bits`
-// the crossing parity comes from over_pass instead of the loop here.
+// the crossing parity comes from over_pass instead of the loop here, with
+// over_state (0 or 1) from k2d_wide_over.
 template <typename R>
 __device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Lds2D<R>& l,
                                           int k, R& Fx, R& Fy, int* n_over, int* n_attr, double* energy,
                                           double* e_int, const unsigned* over_bits = nullptr,
-                                          unsigned* attr_key = nullptr) {
+                                          unsigned* attr_key = nullptr, int over_state = -1) {
   const int ci = cell_of(l.vb, T.nc, k);
   const int gc = T.c0 + ci;
   const int b = l.vb[ci], e = l.vb[ci + 1], n = e - b, vi = k - b;
@@ -496,8 +497,11 @@ __device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DC
     }
   }
   if (mask & M2_REPULSION) {
-    bool over = over_bits && ((over_bits[k >> 5] >> (k & 31)) & 1u);
-    for (int q = 0; q < ncl && !over && !over_bits; q++) {
+    // the crossing parity given (over_state 0 or 1, or bit k of over_bits),
+    // else found by the loop
+    const bool given = over_state >= 0 || over_bits;
+    bool over = over_state >= 0 ? over_state != 0 : (over_bits && ((over_bits[k >> 5] >> (k & 31)) & 1u));
+    for (int q = 0; q < ncl && !over && !given; q++) {
       const int cj = scan_all ? q : (int)l.cand[ci * DPM2D_MAXCAND + q];
       if (cj == ci || outside_box_even(l, cj, px, py, T.pbc, L)) continue;
       over = crossing_odd(l, cj, px, py, T.pbc, L);

@@ -8,9 +8,14 @@
 // functions, so every decision and every force is k2d_run's):
 //   k2d_wide_cells  one thread per cell: strain, orientation, centre of mass,
 //                   bounds of the input positions;
-//   k2d_wide_cand   one thread per cell: its candidate cells, ascending;
+//   k2d_wide_cand   one wave per cell: its candidate cells, ascending (the
+//                   lanes test 64 cells at a time, a ballot keeps the order);
+//   k2d_wide_over   one thread per (vertex, candidate slot): the crossing
+//                   parity of the vertex against that candidate, ORed into
+//                   a bit per vertex (k2d_run's over_pass over the device;
+//                   an OR, so the same bit as the per-vertex loop's);
 //   k2d_wide_step   one thread per vertex: all force terms (the crossing
-//                   parity by the per-vertex loop), then x + F dt into the
+//                   parity from the bits), then x + F dt into the
 //                   other position buffer (positions are double-buffered, so
 //                   every vertex reads the step's input positions).
 #pragma once
@@ -35,6 +40,8 @@ struct Wide2D {
   int* p_overlap;          // diagnostics per vertex (nullptr normally)
   int* p_attr;
   unsigned* p_attr_key;
+  unsigned* over_bits;     // [ceil(V / 32)] crossing parity per vertex (k2d_wide_over)
+  const int* vcell;        // [V] cell of each vertex
 };
 
 enum : int {
@@ -67,6 +74,9 @@ __device__ __forceinline__ Lds2D<float> wide_view(const Wide2D& a, const Tissue2
 
 extern "C" __global__ void __launch_bounds__(256) k2d_wide_cells(const Wide2D a) {
   const int gc = blockIdx.x * blockDim.x + threadIdx.x;
+  // the crossing-parity bits of this step start at zero (a kernel, not a
+  // memset: the step is captured in a graph)
+  for (int w = gc, nw = (a.V + 31) / 32; w < nw; w += gridDim.x * blockDim.x) a.over_bits[w] = 0u;
   if (a.counters && gc == 0) atomicAdd(&a.counters[0], 1ull);
   if (gc >= a.C) return;
   const int t = a.cell_tissue[gc];
@@ -74,26 +84,67 @@ extern "C" __global__ void __launch_bounds__(256) k2d_wide_cells(const Wide2D a)
   cell_one(a.g, T, wide_view(a, T, t, a.xin, a.yin), gc - T.c0);
 }
 
+// The list cand_one makes (maybe_pair over the tissue's cells, ascending,
+// the count past DPM2D_MAXCAND meaning "scan every cell"), by one wave.
 extern "C" __global__ void __launch_bounds__(256) k2d_wide_cand(const Wide2D a) {
-  const int gc = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gc >= a.C) return;
+  const int lane = threadIdx.x & (DPM_WAVE - 1);
+  const int gc = blockIdx.x * (blockDim.x / DPM_WAVE) + (int)(threadIdx.x / DPM_WAVE);
+  if (gc >= a.C) return;  // whole wave
   const int t = a.cell_tissue[gc];
   const Tissue2DCtx<float> T = tissue_ctx(a.g, t);
-  cand_one(a.g, T, wide_view(a, T, t, a.xin, a.yin), gc - T.c0);
+  const Lds2D<float> l = wide_view(a, T, t, a.xin, a.yin);
+  const int ci = gc - T.c0;
+  int n = 0;
+  for (int c0 = 0; c0 < T.nc; c0 += DPM_WAVE) {  // wave-uniform
+    const int cj = c0 + lane;
+    const bool hit = cj < T.nc && cj != ci && maybe_pair(a.g, T, l, ci, cj);
+    const unsigned long long m = __ballot(hit);
+    const int slot = n + (int)__popcll(m & ((1ull << lane) - 1ull));
+    if (hit && slot < DPM2D_MAXCAND) l.cand[ci * DPM2D_MAXCAND + slot] = (short)cj;
+    n += (int)__popcll(m);
+  }
+  if (lane == 0) l.ncand[ci] = n;
+}
+
+// Threads (slot q, vertex gv), q-major: the lanes of a wave take consecutive
+// vertices (mostly one cell's, with one candidate list) and the same slot.
+// A cell with more than DPM2D_MAXCAND candidates scans every cell: slot q
+// then takes cells q, q + DPM2D_MAXCAND, ....  Launched with repulsion only.
+extern "C" __global__ void __launch_bounds__(256) k2d_wide_over(const Wide2D a) {
+  const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= (long long)a.V * DPM2D_MAXCAND) return;
+  const int q = (int)(w / a.V), gv = (int)(w % a.V);
+  const int gc = a.vcell[gv];
+  const int t = a.cell_tissue[gc];
+  const Tissue2DCtx<float> T = tissue_ctx(a.g, t);
+  const Lds2D<float> l = wide_view(a, T, t, a.xin, a.yin);
+  const int ci = gc - T.c0, k = gv - T.v0;
+  const int ncd = l.ncand[ci];
+  const float px = l.x[k], py = l.y[k];
+  bool odd = false;
+  if (ncd > DPM2D_MAXCAND) {
+    for (int cj = q; cj < T.nc && !odd; cj += DPM2D_MAXCAND)
+      if (cj != ci && !outside_box_even(l, cj, px, py, T.pbc, T.L)) odd = crossing_odd(l, cj, px, py, T.pbc, T.L);
+  } else if (q < ncd) {
+    const int cj = l.cand[ci * DPM2D_MAXCAND + q];
+    if (cj != ci && !outside_box_even(l, cj, px, py, T.pbc, T.L)) odd = crossing_odd(l, cj, px, py, T.pbc, T.L);
+  }
+  if (odd) atomicOr(&a.over_bits[gv >> 5], 1u << (gv & 31));
 }
 
 extern "C" __global__ void __launch_bounds__(256) k2d_wide_step(const Wide2D a) {
   const int gv = blockIdx.x * blockDim.x + threadIdx.x;
   if (gv >= a.V) return;
-  const int gc = cell_of(a.g.cell_vert_begin, a.C, gv);
+  const int gc = a.vcell[gv];
   const int t = a.cell_tissue[gc];
   const Tissue2DCtx<float> T = tissue_ctx(a.g, t);
   const Lds2D<float> l = wide_view(a, T, t, a.xin, a.yin);
   const int k = gv - T.v0;
   float fx = 0.0f, fy = 0.0f;
   int no = 0, na = 0;
+  const int over = (a.g.mask & M2_REPULSION) ? (int)((a.over_bits[gv >> 5] >> (gv & 31)) & 1u) : -1;
   force_vertex(a.g, T, l, k, fx, fy, &no, a.p_attr ? &na : nullptr, (double*)nullptr, (double*)nullptr,
-               (const unsigned*)nullptr, a.p_attr_key ? a.p_attr_key + gv : nullptr);
+               (const unsigned*)nullptr, a.p_attr_key ? a.p_attr_key + gv : nullptr, over);
   if (a.p_overlap) a.p_overlap[gv] = no;
   if (a.p_attr) a.p_attr[gv] = na;
   if (a.flags & W2_FORCES) {

@@ -155,12 +155,15 @@ struct dpm_tissue {
   hipGraphExec_t jam_graph = nullptr;
   std::vector<unsigned char> jam_key;
   // tissues beyond one workgroup: 0 automatic, 1 one workgroup per tissue,
-  // 2 the wide path (three launches per step over the device)
+  // 2 the wide path (three or four launches per step over the device)
   int wide_mode = 0;
   bool fits_wg = true;  // every tissue fits one workgroup (k2d_run)
+  int max_cells = 0;    // cells of the largest tissue
   float *xw = nullptr, *yw = nullptr, *cellq = nullptr;
   int *ncand_w = nullptr, *cell_tissue2 = nullptr, *vbl = nullptr;
   short* cand_w = nullptr;
+  unsigned* over_w = nullptr;  // crossing-parity bits of the wide path
+  int* vcell_w = nullptr;      // cell of each vertex
   hipGraphExec_t wide_graph = nullptr;
   float wide_graph_dt = 0.0f;
 
@@ -1591,6 +1594,7 @@ static int create2d(const dpm2d_desc* d, dpm_tissue** out, int rs) {
     return fail(DPM_ERANGE, "a double 2D tissue must fit the LDS of one workgroup");
   }
   if (!h->fits_wg) h->lds2 = 0;
+  h->max_cells = maxc;
   h->cvb_h = cvb;
   h->tissue_begin_h.assign(d->cell_begin, d->cell_begin + T + 1);
   auto AR = [&](void** p, size_t n) { return h->alloc_real(p, n); };
@@ -1636,8 +1640,17 @@ static int create2d(const dpm2d_desc* d, dpm_tissue** out, int rs) {
 
 static Run2D run2d_args(dpm_tissue* h, int nsteps, float dt);
 
+// Automatic choice: the wide form for a tissue beyond one workgroup, and for
+// a few large tissues, which one workgroup each would keep on a few compute
+// units (the 400-cell tissue of the reference's test2D.py: 735 against 1,967
+// us per step); many tissues, or small ones, one workgroup per tissue (all
+// steps of a call in one launch).
+constexpr int kWideMinCells = 128;
+constexpr int kWideMaxTissues = 64;
 static bool use_wide(const dpm_tissue* h) {
-  return h->rs == 4 && (h->wide_mode == 2 || (h->wide_mode == 0 && !h->fits_wg));
+  if (h->rs != 4) return false;
+  if (h->wide_mode != 0) return h->wide_mode == 2;
+  return !h->fits_wg || (h->max_cells >= kWideMinCells && h->T <= kWideMaxTissues);
 }
 
 static int ensure_wide(dpm_tissue* h) {
@@ -1651,15 +1664,20 @@ static int ensure_wide(dpm_tissue* h) {
   rc |= h->alloc(&h->cand_w, (size_t)C * DPM2D_MAXCAND);
   rc |= h->alloc(&h->cell_tissue2, C);
   rc |= h->alloc(&h->vbl, (size_t)C + T);
+  rc |= h->alloc(&h->over_w, ((size_t)h->V + 31) / 32);
+  rc |= h->alloc(&h->vcell_w, (size_t)h->V);
   if (rc) return DPM_ENOMEM;
-  std::vector<int> ct(C), vbl((size_t)C + T);
+  std::vector<int> ct(C), vbl((size_t)C + T), vc((size_t)h->V);
   for (int t = 0; t < T; t++) {
     const int c0 = h->tissue_begin_h[t], c1 = h->tissue_begin_h[t + 1];
     for (int c = c0; c < c1; c++) ct[c] = t;
     for (int c = c0; c <= c1; c++) vbl[(size_t)c + t] = h->cvb_h[c] - h->cvb_h[c0];
   }
+  for (int c = 0; c < C; c++)
+    for (int v = h->cvb_h[c]; v < h->cvb_h[c + 1]; v++) vc[v] = c;
   TRY(upload(h->cell_tissue2, ct.data(), C, h->stream));
   TRY(upload(h->vbl, vbl.data(), vbl.size(), h->stream));
+  TRY(upload(h->vcell_w, vc.data(), vc.size(), h->stream));
   HIPCHECK(hipStreamSynchronize(h->stream));
   return DPM_OK;
 }
@@ -1685,13 +1703,19 @@ static Wide2D wide_args(dpm_tissue* h, int in, float dt, int flags) {
   a.dt = dt;
   a.flags = flags;
   a.counters = h->counters;
+  a.over_bits = h->over_w;
+  a.vcell = h->vcell_w;
   return a;
 }
 
 static int launch_wide(dpm_tissue* h, const Wide2D& a) {
-  const int bc = (h->C + 255) / 256, bv = (h->V + 255) / 256;
+  const int bc = (h->C + 255) / 256, bv = (h->V + 255) / 256, bw = (h->C + 3) / 4;
   hipLaunchKernelGGL(k2d_wide_cells, dim3(bc), dim3(256), 0, h->stream, a);
-  hipLaunchKernelGGL(k2d_wide_cand, dim3(bc), dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL(k2d_wide_cand, dim3(bw), dim3(256), 0, h->stream, a);  // a wave per cell
+  if (a.g.mask & M2_REPULSION) {  // (vertex, candidate slot) threads
+    const long long n = (long long)h->V * DPM2D_MAXCAND;
+    hipLaunchKernelGGL(k2d_wide_over, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, a);
+  }
   hipLaunchKernelGGL(k2d_wide_step, dim3(bv), dim3(256), 0, h->stream, a);
   HIPCHECK(hipGetLastError());
   return DPM_OK;

@@ -182,8 +182,10 @@ extern "C" __global__ void __launch_bounds__(kDispThreads) k_disperse2d(const Di
       __syncthreads();
     }
     // ---- forces on every centre, in the reference's accumulation order
+    int m_own = -2;  // this thread's gathered list, kept for the U terms when n <= NT
     for (int k = tid; k < n; k += NT) {
       const int m = disp_gather(l, k, n, nb, L, mycand);
+      m_own = m;
       const int ncand = m < 0 ? n : m;
       const float xk = l.x[k], yk = l.y[k], rk = l.r[k];
       float Fx = 0.0f, Fy = 0.0f;
@@ -235,7 +237,8 @@ extern "C" __global__ void __launch_bounds__(kDispThreads) k_disperse2d(const Di
     const bool stored = l.cnt[n] <= a.term_cap;  // block-uniform
     if (stored) {  // the own rows' U terms at their places in (i, j) order
       for (int k = tid; k < n; k += NT) {
-        const int m = disp_gather(l, k, n, nb, L, mycand);
+        // a thread with one centre still holds its list (positions unchanged)
+        const int m = n <= NT ? m_own : disp_gather(l, k, n, nb, L, mycand);
         const int ncand = m < 0 ? n : m;
         int o = l.cnt[k];
         for (int q = 0; q < ncand; q++) {
@@ -251,8 +254,17 @@ extern "C" __global__ void __launch_bounds__(kDispThreads) k_disperse2d(const Di
       // U += 0.5 * (1 - q (1 - q)), in the reference's (i, j) order
       float U = 0.0f;
       if (stored) {
+        // the terms in order, eight loads in flight ahead of their additions
         const int tot = l.cnt[n];
-        for (int q = 0; q < tot; q++) U = (float)((double)U + 0.5 * (double)a.terms[q]);
+        int q = 0;
+        for (; q + 8 <= tot; q += 8) {
+          float tb[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) tb[u] = a.terms[q + u];
+#pragma unroll
+          for (int u = 0; u < 8; u++) U = (float)((double)U + 0.5 * (double)tb[u]);
+        }
+        for (; q < tot; q++) U = (float)((double)U + 0.5 * (double)a.terms[q]);
       } else {
         for (int i = 0; i < n; i++)
           for (int j = 0; j < n; j++) {
