@@ -437,6 +437,7 @@ __device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DC
     const R l0 = (R)g.l0[gc];
     const R kat = T.Kat / (R)n;
     const R lim = l0 * l0 * R(1.00001f);
+    const double kk = (double)kat / (double)l0;  // ftmp/d = kat/l0 of an accepted pair
     int cnt = 0;
     for (int q = 0; q < ncl; q++) {
       const int cj = scan_all ? q : (int)l.cand[ci * DPM2D_MAXCAND + q];
@@ -461,7 +462,6 @@ __device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DC
           // digits in float
           const double ddx = ((double)l.x[j] - (double)px) - (double)L * (double)nx;
           const double ddy = ((double)l.y[j] - (double)py) - (double)L * (double)ny;
-          const double kk = (double)kat / (double)l0;  // ftmp/d = kat/l0
           fx += (R)(kk * ddx);
           fy += (R)(kk * ddy);
           cnt++;

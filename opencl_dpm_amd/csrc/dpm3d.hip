@@ -414,7 +414,7 @@ __device__ void cell_stats_impl(const Lds3D<R>& l, int nv, int nf, CellStat<R>* 
   {  // the radius, volume and longest edge in one reduction (max of
      // non-negative floats is exact in double)
     double rd = (double)r, e2d = (double)e2;
-    block_max_sum_max_1b(rd, vol, e2d, l.red + 128);
+    block_max_sum_max_1b<sizeof(R) == 4>(rd, vol, e2d, l.red + 128);
     r = (R)rd;
     e2 = (R)e2d;
   }
@@ -660,7 +660,7 @@ __device__ __forceinline__ R winding_wave(const Lds3D<R>& l, Q4<R>* ut, int nv, 
     const R num = vdot(u, vcross(v, w));
     part += R(2) * atan2(num, denom);
   }
-  const double tot = wave_sum((double)part);
+  const double tot = wave_sum_d((double)part);  // every lane (DPP tree)
   wave_lds_sync();  // this wave's table reads finish before it is rewritten
   return (R)(tot / (double)(R(4) * R(3.14159265358979323846)));
 }

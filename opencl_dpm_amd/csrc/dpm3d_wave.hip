@@ -139,9 +139,9 @@ __device__ __forceinline__ R winding_w(const R* nx, const R* ny, const R* nz, co
     const R num = vdot(u, vcross(v, w));
     part += R(2) * atan2(num, denom);
   }
-  const double tot = wave_sum((double)part);
+  const double tot = wave_sum_d((double)part);  // every lane (DPP tree)
   wave_lds_sync();
-  return (R)(__shfl(tot, 0, DPM_WAVE) / (double)(R(4) * R(3.14159265358979323846)));
+  return (R)(tot / (double)(R(4) * R(3.14159265358979323846)));
 }
 
 // NVC, NFC: vertex and face counts fixed at compile time (0: taken from the

@@ -253,9 +253,9 @@ __device__ __forceinline__ void block_sum3(double& a, double& b, double& c, doub
 __device__ __forceinline__ void block_sum3_1b(double& a, double& b, double& c, double* scratch) {
   const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
   const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
-  a = wave_sum(a);
-  b = wave_sum(b);
-  c = wave_sum(c);
+  a = wave_sum_d(a);  // DPP trees (no LDS round trip per step)
+  b = wave_sum_d(b);
+  c = wave_sum_d(c);
   if (lane == 0) {
     scratch[wave] = a;
     scratch[16 + wave] = b;
@@ -274,14 +274,20 @@ __device__ __forceinline__ void block_sum3_1b(double& a, double& b, double& c, d
 }
 
 // block_max_sum_max with one workgroup barrier (as block_sum3_1b).
+// FLOAT_MAX: the maxima hold float values (exact as floats: DPP maxima);
+// otherwise double maxima by shuffles.  The sum by the DPP tree.
+template <bool FLOAT_MAX>
 __device__ __forceinline__ void block_max_sum_max_1b(double& m1, double& s, double& m2, double* scratch) {
   const int lane = threadIdx.x & (DPM_WAVE - 1), wave = threadIdx.x / DPM_WAVE;
   const int nw = (blockDim.x + DPM_WAVE - 1) / DPM_WAVE;
-  for (int o = 32; o > 0; o >>= 1) {
-    m1 = fmax(m1, __shfl_down(m1, o, DPM_WAVE));
-    s += __shfl_down(s, o, DPM_WAVE);
-    m2 = fmax(m2, __shfl_down(m2, o, DPM_WAVE));
+  if constexpr (FLOAT_MAX) {
+    m1 = (double)wave_max_f((float)m1);
+    m2 = (double)wave_max_f((float)m2);
+  } else {
+    m1 = wave_max_all(m1);
+    m2 = wave_max_all(m2);
   }
+  s = wave_sum_d(s);
   if (lane == 0) {
     scratch[wave] = m1;
     scratch[16 + wave] = s;
