@@ -1581,9 +1581,20 @@ static int create2d(const dpm2d_desc* d, dpm_tissue** out, int rs) {
   h->T = T;
   h->C = C;
   h->V = cvb[C];
-  // four threads per vertex (up to 1024): the crossing-parity pass of k2d_run
-  // spreads (vertex, candidate) pairs over all of them
-  h->NT2 = std::min(1024, std::max(64, (4 * maxv + 63) / 64 * 64));
+  // Four threads per vertex (up to 1024) when some tissue can take the
+  // crossing-parity pass of k2d_run (its split_over predicate at that size;
+  // the mask is left out, as dpm2d_set_tissue_params may replace it), which
+  // spreads (vertex, candidate) pairs over all of them; otherwise one per
+  // vertex and DPM2D_CSPLIT per cell for the candidate pass.
+  const int nt4 = std::min(1024, std::max(64, (4 * maxv + 63) / 64 * 64));
+  bool any_pass = false;
+  for (int t = 0; t < T; t++) {
+    const int nc = d->cell_begin[t + 1] - d->cell_begin[t];
+    const int nvt = cvb[d->cell_begin[t + 1]] - cvb[d->cell_begin[t]];
+    if (nvt <= 32 * DPM2D_CSPLIT * nc && (long long)nvt * nc <= 16LL * nt4)
+      any_pass = true;
+  }
+  h->NT2 = any_pass ? nt4 : std::min(1024, std::max(64, (std::max(maxv, DPM2D_CSPLIT * maxc) + 63) / 64 * 64));
   h->NTr = std::min(1024, std::max(64, (maxv / 4 + 63) / 64 * 64));
   h->lds2 = lds2d_bytes(maxv, maxc, rs);
   // a tissue beyond one workgroup (LDS, or 16 vertices per thread) takes the

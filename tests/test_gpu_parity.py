@@ -473,6 +473,34 @@ def test_2d_forces_and_pairs(case):
     print(f"pairs: overlap={int(ov32.sum())} attraction={int(at32.sum())}", flush=True)
 
 
+def takes_parity_pass(p: W.Packing2D) -> bool:
+    """k2d_run's split_over predicate for a one-tissue float handle, at the
+    block size dpm2d_create gives it (four threads per vertex, up to 1024)."""
+    nvt, nc = int(np.sum(p.nv)), p.C
+    nt = min(1024, max(64, (4 * nvt + 63) // 64 * 64))
+    return nvt <= 256 * nc and nvt * nc <= 16 * nt
+
+
+@pytest.mark.parametrize("case", ["cfg1", "dense_attr"])
+def test_2d_parity_pass_matches_vertex_loop(case):
+    """The step's crossing-parity pass ((vertex, candidate) pairs over the
+    whole workgroup) and the relaxation kernel's per-vertex crossing loop
+    (its energy evaluation) give bit-identical forces on the same positions,
+    at the start state and after 300 steps."""
+    nc, nv, phi, Kb, Kat, seed, rng = CASES_2D[case]
+    p = W.tissue2d(nc, nv, 1.4, phi, Kb=Kb, Kat=Kat, seed=seed, nv_range=rng)
+    assert takes_parity_pass(p)
+    t = tissue2d_handle(p)
+    t.set_positions(p.X)
+    for steps in (0, 300):
+        if steps:
+            t.euler(steps, 0.005)
+        g = t.probe()
+        t.energy()
+        np.testing.assert_array_equal(t.forces(), g["forces"])
+        print(f"2d {case} after {steps} steps: overlapping vertices {int((g['overlap'] > 0).sum())}", flush=True)
+
+
 def test_2d_cfg1_1000_steps_vs_ref64():
     # cfg1 starts at mechanical rest (regular polygons at their own a0, l0),
     # so its forces are rounding-level; positions are the meaningful check.
