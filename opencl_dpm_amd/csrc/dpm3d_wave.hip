@@ -153,6 +153,8 @@ template <typename R, int NVC, int NFC, bool PROBE = true>
 __device__ void k3d_wstep_body(const Step3D<R>& a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nv = NVC > 0 ? NVC : a.nv, nf = NFC > 0 ? NFC : a.nf;
+  // 64-vertex chunks of a cell (a bound when the size is a run-time value)
+  constexpr int kChunks = NVC > 0 ? (NVC + DPM_WAVE - 1) / DPM_WAVE : kWaveMaxNv / DPM_WAVE;
   // wv, and every per-cell value derived from it, is wave-uniform: through
   // readfirstlane it lives in scalar registers (the cell's parameters and
   // statistics come by scalar loads), which keeps the step within 128 vector
@@ -452,14 +454,24 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       }
     }
     int c0 = 0;
+    // a chunk's source cell and statistics, loaded one chunk ahead (a gathered
+    // source is read before the compaction below can overwrite it: the
+    // compacted slots lie below the chunk being tested)
+    int cj_nx = lane < nsrc ? (gathered ? cl[lane] : cb + lane) : ce;
+    CellStat<R> st_nx{};
+    if (cj_nx < ce && cj_nx != ci) st_nx = a.st_in[cj_nx];
     while (c0 < nsrc) {  // wave-uniform: candidates in batches of at most kWaveCand
       int ncl = 0;
       while (c0 < nsrc && ncl + DPM_WAVE <= kWaveCand) {
-        const int k0 = c0 + lane;
-        const int cj = k0 < nsrc ? (gathered ? cl[k0] : cb + k0) : ce;
+        const int cj = cj_nx;
+        const CellStat<R> stj = st_nx;
+        {
+          const int k1 = c0 + DPM_WAVE + lane;
+          cj_nx = k1 < nsrc ? (gathered ? cl[k1] : cb + k1) : ce;
+          if (cj_nx < ce && cj_nx != ci) st_nx = a.st_in[cj_nx];
+        }
         bool cand = false;
         if (cj < ce && cj != ci) {
-          const CellStat<R> stj = a.st_in[cj];
           // a superset test: the nearest image by the rounded product with
           // 1/L (where that is ambiguous, near a half box length, both images
           // lie beyond any reach below 0.4995 L; longer reaches take every
@@ -493,12 +505,27 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       for (int qc = 0; qc < ncl; qc++) {
         const int cj2 = cl[qc];
         const CellStat<R> stj = a.st_in[cj2];
-        wave_stage_cell(nx, ny, nz, a.xin, a.yin, a.zin, (size_t)cj2 * nv, nv, lane);
+        // cell j's positions stay in registers through the list pass (lane l
+        // holds vertices l, l + 64, ..., as its chunks of the pass); every
+        // load is issued before the first use
+        R qx[kChunks], qy[kChunks], qz[kChunks];
+        {
+          const size_t bj = (size_t)cj2 * nv;
+#pragma unroll
+          for (int k = 0; k < kChunks; k++) {
+            const int v = lane + k * DPM_WAVE;
+            qx[k] = qy[k] = qz[k] = R(0);
+            if (v < nv) {
+              qx[k] = a.xin[bj + v];
+              qy[k] = a.yin[bj + v];
+              qz[k] = a.zin[bj + v];
+            }
+          }
+        }
         V3<R> sh = zero;
         if (pbc)
           sh = v3(image_shift(Ci.x - stj.cx, L, invL), image_shift(Ci.y - stj.cy, L, invL),
                   image_shift(Ci.z - stj.cz, L, invL));
-        wave_lds_sync();
         const V3<R> Cj = {stj.cx + sh.x, stj.cy + sh.y, stj.cz + sh.z};
         const R mj = contact_margin(stj.r, stj.em);
         const R rj = (stj.r + mj) * R(1.0001f) + R(1e-6f);
@@ -526,19 +553,18 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         const V3<R> Cis = vsub(Ci, sh);
         // One pass: the separation, whether an own vertex is within contact
         // reach, and the adhesion lists.  Cell j's vertices in reach are
-        // compacted in place to the front of nx/ny/nz (ascending; every slot
-        // written lies below the indices still to be read, and a chunk's loads
-        // precede its stores).
+        // compacted to the front of nx/ny/nz (ascending).
         bool any_need = false;
         int nnear = 0, nmine = 0;
-        #pragma unroll 1
-        for (int k0v = 0; k0v < nv; k0v += DPM_WAVE) {
+#pragma unroll
+        for (int k = 0; k < kChunks; k++) {
+          const int k0v = k * DPM_WAVE;
+          if (k0v >= nv) break;  // wave-uniform
           const int v = k0v + lane;
           bool need = false, near = false, mine = false;
-          V3<R> q = zero;
+          const V3<R> q = {qx[k], qy[k], qz[k]};
           if (v < nv) {
             const V3<R> P = lds_v(sx, sy, sz, v);
-            q = lds_v(nx, ny, nz, v);
             const V3<R> dp = vsub(P, Cj);
             const V3<R> ds = vsub(q, Cis);
             hp = fmax(hp, vdot(dp, nC));
@@ -645,12 +671,10 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         }
         // Contact: a point at least m_j from every face of cell j has winding
         // number 0 (contact_margin), so the pass is needed only below that
-        // separation; it restages cell j (the adhesion lists rewrote nx).
+        // separation; it stages cell j's whole mesh.
         if (do_contact && any_need && !(gap >= mj)) {  // wave-uniform
-          if (do_adh) {
-            wave_lds_sync();
-            wave_stage_cell(nx, ny, nz, a.xin, a.yin, a.zin, (size_t)cj2 * nv, nv, lane);
-          }
+          wave_lds_sync();
+          wave_stage_cell(nx, ny, nz, a.xin, a.yin, a.zin, (size_t)cj2 * nv, nv, lane);
           wave_lds_sync();
           #pragma unroll 1
           for (int k0v = 0; k0v < nv; k0v += DPM_WAVE) {
