@@ -460,6 +460,39 @@ __device__ __forceinline__ void load_own(const Step3D<R>& a, const Lds3D<R>& l, 
   }
 }
 
+// The reference's adhesion decision sqrt(d2) < 2 l0 && sqrt(d2) > 1e-12
+// (Cell3D_Kernel.cl:348-353), for float without the square root: with m the
+// midpoint between a positive float c and the float below it, RN(sqrt(x)) < c
+// exactly when x < m^2, and RN(sqrt(x)) > c exactly when x > m'^2 (m' the
+// midpoint with the float above).  m has at most 25 significant bits, so m^2
+// is exact in double, and sqrt(x) never equals m for a float x (m^2 would
+// need at least 49 significant bits), so no rounding tie arises.  Double
+// steps take the square root.
+template <typename R>
+struct AdhCut {
+  R c;
+  double hi, lo;
+  __device__ __forceinline__ explicit AdhCut(R two_l0) : c(two_l0), hi(0.0), lo(0.0) {
+    if constexpr (sizeof(R) == 4) {
+      const float below = __int_as_float(__float_as_int((float)two_l0) - 1);
+      const float tiny = 1e-12f, above = __int_as_float(__float_as_int(tiny) + 1);
+      const double m = 0.5 * ((double)below + (double)two_l0);
+      const double n = 0.5 * ((double)tiny + (double)above);
+      hi = m * m;
+      lo = n * n;
+    }
+  }
+  __device__ __forceinline__ bool operator()(R d2) const {
+    if constexpr (sizeof(R) == 4) {
+      const double x = (double)d2;
+      return x < hi && x > lo;
+    } else {
+      const R d = sqrt(d2);
+      return d < c && d > R(1e-12f);
+    }
+  }
+};
+
 // Kat/2 (d/L0 - 1) delta/d with delta = (b - a) - L n, the adhesion term of
 // Cell3D_Kernel.cl:354-357, evaluated in double.  Float steps take it as
 // Kat/2 (1/L0 - 1/d) delta with 1/d from the hardware reciprocal square root
@@ -1211,8 +1244,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
               V3<R> Fa = zero;
               int cnt_own = 0;
               const R half_l = R(0.49f) * L;
-              const R lim_own = (L0i * R(2)) * (L0i * R(2)) * R(1.00001f);
-              const R lim_rev = (L0j * R(2)) * (L0j * R(2)) * R(1.00001f);
+              const AdhCut<R> cut_own(L0i * R(2)), cut_rev(L0j * R(2));
               const double kh = 0.5 * (double)Kat, inv_l0i = sti.inv_l0, inv_l0j = stj.inv_l0;
               const int m1 = min(nnear, (sl + 1) * per);
               for (int m = sl * per; m < m1; m++) {
@@ -1230,29 +1262,21 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
                   dl = vsub(dl, v3(L * nl.x, L * nl.y, L * nl.z));
                   dr = vsub(dr, v3(L * nr.x, L * nr.y, L * nr.z));
                 }
-                // sqrt(d2) < c is only possible for d2 < c^2 (1 + 1e-5):
-                // the exact comparisons run where they can succeed.  The
-                // decisions are the reference's ones; the force of an
+                // The decisions are the reference's ones (AdhCut); the force of an
                 // accepted pair is evaluated in double on the same image,
                 // since across the periodic boundary the float difference
                 // of two near-L numbers keeps few digits of a short vector.
                 const R d2 = vdot(dl, dl);
-                if (d2 < lim_own) {
-                  const R ds = sqrt(d2);
-                  if (ds < L0i * R(2) && ds > R(1e-12f)) {
-                    Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i, kh, inv_l0i));
-                    cnt_own++;
-                    if (PROBE && a.p_adh_hash) atomicAdd(&a.p_adh_hash[2 * (base + il[iv])], vertex_key(pj.x, pj.y, pj.z));
-                  }
+                if (cut_own(d2)) {
+                  Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i, kh, inv_l0i));
+                  cnt_own++;
+                  if (PROBE && a.p_adh_hash) atomicAdd(&a.p_adh_hash[2 * (base + il[iv])], vertex_key(pj.x, pj.y, pj.z));
                 }
                 const R d2r = vdot(dr, dr);
-                if (d2r < lim_rev) {
-                  const R dsr = sqrt(d2r);
-                  if (dsr < L0j * R(2) && dsr > R(1e-12f)) {
-                    Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j, kh, inv_l0j));
-                    if (PROBE && a.p_adh_hash)
-                      atomicAdd(&a.p_adh_hash[2 * (base + il[iv]) + 1], vertex_key(pj.x, pj.y, pj.z));
-                  }
+                if (cut_rev(d2r)) {
+                  Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j, kh, inv_l0j));
+                  if (PROBE && a.p_adh_hash)
+                    atomicAdd(&a.p_adh_hash[2 * (base + il[iv]) + 1], vertex_key(pj.x, pj.y, pj.z));
                 }
               }
               acc[sl * nmine + iv] = Q4<R>{Fa.x, Fa.y, Fa.z, (R)cnt_own};

@@ -604,8 +604,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
           const int S = nmine >= DPM_WAVE ? 1 : max(1, min(nnear, DPM_WAVE / nmine));
           const int per = (nnear + S - 1) / S;
           const R half_l = R(0.49f) * L;
-          const R lim_own = (L0i * R(2)) * (L0i * R(2)) * R(1.00001f);
-          const R lim_rev = (L0j * R(2)) * (L0j * R(2)) * R(1.00001f);
+          const AdhCut<R> cut_own(L0i * R(2)), cut_rev(L0j * R(2));
           const double kh = 0.5 * (double)Kat, inv_l0i = sti.inv_l0, inv_l0j = stj.inv_l0;
           for (int tw = lane; tw < S * nmine; tw += DPM_WAVE) {
             const int iv = tw % nmine, sl = tw / nmine;
@@ -634,21 +633,15 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
                 d2r = vdot(dl, dl);
               }
               const R d2 = vdot(dl, dl);
-              if (d2 < lim_own) {
-                const R dsq = sqrt(d2);
-                if (dsq < L0i * R(2) && dsq > R(1e-12f)) {
-                  Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i, kh, inv_l0i));
-                  cnt_own++;
-                  if (PROBE && a.p_adh_hash) atomicAdd(&a.p_adh_hash[2 * (base + il[iv])], vertex_key(pj.x, pj.y, pj.z));
-                }
+              if (cut_own(d2)) {
+                Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i, kh, inv_l0i));
+                cnt_own++;
+                if (PROBE && a.p_adh_hash) atomicAdd(&a.p_adh_hash[2 * (base + il[iv])], vertex_key(pj.x, pj.y, pj.z));
               }
-              if (d2r < lim_rev) {
-                const R dsr = sqrt(d2r);
-                if (dsr < L0j * R(2) && dsr > R(1e-12f)) {
-                  Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j, kh, inv_l0j));
-                  if (PROBE && a.p_adh_hash)
-                    atomicAdd(&a.p_adh_hash[2 * (base + il[iv]) + 1], vertex_key(pj.x, pj.y, pj.z));
-                }
+              if (cut_rev(d2r)) {
+                Fa = vadd(Fa, adhesion_term(Pv, pj, nr, L, Kat, L0j, kh, inv_l0j));
+                if (PROBE && a.p_adh_hash)
+                  atomicAdd(&a.p_adh_hash[2 * (base + il[iv]) + 1], vertex_key(pj.x, pj.y, pj.z));
               }
             }
             ut[sl * nmine + iv] = Q4<R>{Fa.x, Fa.y, Fa.z, (R)cnt_own};
