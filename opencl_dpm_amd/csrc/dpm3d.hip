@@ -523,9 +523,13 @@ __device__ __forceinline__ V3<R> adhesion_term(V3<R> b, V3<R> a, V3<R> n, R L, R
 
 // Substrate force on one corner of a facing-down face
 // (Cell3D_Kernel.cl:229-245): a z < 0 penalty and the spreading push away
-// from the centre of mass within 2 l0 of the plane.
+// from the centre of mass within 2 l0 of the plane.  The decisions (z < 0,
+// |z| < 2 l0) are the reference's; the float force takes its distance to the
+// plane as |z| (the correctly rounded sqrt(z*z) is |z|), |z|/l0 as |z| times
+// sub_inv_l0 = 1/l0 in double and the unit vector by the hardware reciprocal
+// square root (force arithmetic only, relative error near 1e-7).
 template <typename R>
-__device__ __forceinline__ V3<R> substrate_corner(V3<R> pos, V3<R> C, R Ks, R ksg, R L, R l0) {
+__device__ __forceinline__ V3<R> substrate_corner(V3<R> pos, V3<R> C, R Ks, R ksg, R L, R l0, double sub_inv_l0) {
   V3<R> c = {R(0), R(0), R(0)};
   R ks = Ks;
   if (ksg != R(0)) {
@@ -536,10 +540,17 @@ __device__ __forceinline__ V3<R> substrate_corner(V3<R> pos, V3<R> C, R Ks, R ks
   const R height = fabs(pos.z);
   if (height < l0 * R(2)) {
     const V3<R> c2v = {pos.x - C.x, pos.y - C.y, R(0) - C.z};
-    const R dz = pos.z - R(0);
-    const R dist = sqrt((R(0) * R(0) + R(0) * R(0)) + dz * dz);
-    const R ftmp = (R)((double)ks * (1.0 - (double)(dist / l0)));
-    c = vadd(c, vscale(vnormalize(c2v), ftmp));
+    if constexpr (sizeof(R) == 4) {
+      const R ftmp = (R)((double)ks * (1.0 - (double)height * sub_inv_l0));
+      const R n2 = vdot(c2v, c2v);
+      const R s = n2 > R(0) ? __builtin_amdgcn_rsqf(n2) * ftmp : R(0);
+      c = vmadd_f(c2v, s, c);
+    } else {
+      const R dz = pos.z - R(0);
+      const R dist = sqrt((R(0) * R(0) + R(0) * R(0)) + dz * dz);
+      const R ftmp = (R)((double)ks * (1.0 - (double)(dist / l0)));
+      c = vadd(c, vscale(vnormalize(c2v), ftmp));
+    }
   }
   return c;
 }
@@ -795,6 +806,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
     const bool sub_on = (mask & M_SUBSTRATE) && !(Ksi < R(1e-12f));
     const R sc = Kai * sqrt(a0i) / l0i * R(0.3f);
     const R inv_l0 = R(1) / l0i;
+    const double sub_inv_l0 = 1.0 / (double)l0i;
     if ((sizeof(R) == 4 || NTC > 0) && a.closed) {  // block-uniform
       // Closed mesh, neighbour rows in ring order (face q is (v, u_q, u_(q+1))):
       // one walk around each vertex's ring, as the wave-per-cell step takes
@@ -858,7 +870,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
             const int f = e / 3;
             ndown += (dbits[f >> 5] >> (f & 31)) & 1u;
           }
-          if (ndown) Fr = vadd(Fr, vscale(substrate_corner(P, Ci, Ksi, ksg, L, l0i), (R)ndown));
+          if (ndown) Fr = vadd(Fr, vscale(substrate_corner(P, Ci, Ksi, ksg, L, l0i, sub_inv_l0), (R)ndown));
         }
       }
       ring_done = !__syncthreads_or(bad);  // also: every facing-down bit read
@@ -893,9 +905,9 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
         const V3<R> un = vnormalize(vcross(vsub(P1, P0), vsub(P2, P0)));
         if (un.z < R(-0.1f)) {
           if (PROBE && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
-          c0 = vadd(c0, substrate_corner(P0, Ci, Ksi, ksg, L, l0i));
-          c1 = vadd(c1, substrate_corner(P1, Ci, Ksi, ksg, L, l0i));
-          c2 = vadd(c2, substrate_corner(P2, Ci, Ksi, ksg, L, l0i));
+          c0 = vadd(c0, substrate_corner(P0, Ci, Ksi, ksg, L, l0i, sub_inv_l0));
+          c1 = vadd(c1, substrate_corner(P1, Ci, Ksi, ksg, L, l0i, sub_inv_l0));
+          c2 = vadd(c2, substrate_corner(P2, Ci, Ksi, ksg, L, l0i, sub_inv_l0));
         }
       }
       l.fcx[3 * f] = c0.x;

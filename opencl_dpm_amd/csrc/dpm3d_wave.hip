@@ -214,13 +214,23 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
   const R strain = (R)(a.vol_in[ci] / (double)v0i - 1.0);
   const R kvol = -Kvi * strain / R(6);
   const bool vol_on = (mask & M_VOLUME) && Kvi != R(0);
+  // the ring walk sums the volume products unconditionally and scales them
+  // by 0 when the term is off (a finite sum times 0 adds a signed zero)
+  const R kvol_on = vol_on ? kvol : R(0);
   const bool surf_on = (mask & M_SURFACE) && !(Kai < R(1e-8f));
   const bool sub_on = (mask & M_SUBSTRATE) && !(Ksi < R(1e-12f));
   const R sc = Kai * sqrt(a0i) / l0i * R(0.3f);
   const R inv_l0 = R(1) / l0i;
+  const double sub_inv_l0 = 1.0 / (double)l0i;
   // Face pre-pass: the facing-down faces (Cell3D_Kernel.cl:206-246), one bit
-  // each.
+  // each, and per vertex the number of facing-down faces it is a corner of
+  // (integer LDS atomics: the counts do not depend on their order; the array
+  // is free until the candidates).
+  unsigned* dcnt = reinterpret_cast<unsigned*>(slb + LW.ut);
   if (shape_on && sub_on) {  // wave-uniform
+    #pragma unroll 1
+    for (int v = lane; v < nv; v += DPM_WAVE) dcnt[v] = 0u;
+    wave_lds_sync();
     #pragma unroll 1
     for (int f0 = 0; f0 < nf; f0 += DPM_WAVE) {
       const int f = f0 + lane;
@@ -230,6 +240,11 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         wface(sf, f, i0, i1, i2);
         const V3<R> P0 = lds_v(sx, sy, sz, i0), P1 = lds_v(sx, sy, sz, i1), P2 = lds_v(sx, sy, sz, i2);
         dn = facing_down(vcross(vsub(P1, P0), vsub(P2, P0)));
+        if (dn) {
+          atomicAdd(&dcnt[i0], 1u);
+          atomicAdd(&dcnt[i1], 1u);
+          atomicAdd(&dcnt[i2], 1u);
+        }
         if (PROBE && dn && a.p_down) a.p_down[(size_t)ci * nf + f] = 1;
       }
       const unsigned long long m = __ballot(dn);
@@ -277,26 +292,19 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
         sb = vadd(sb, dq);
         const int un = q + 1 < DPM3D_MAXDEG ? row16(arow, q + 1) : 0xffff;
         const V3<R> dn = un == 0xffff ? d0 : vsub(lds_v(sx, sy, sz, un), P);
-        if (vol_on) Fv = vcross_add_f(dq, dn, Fv);
+        Fv = vcross_add_f(dq, dn, Fv);
         if (un == 0xffff) break;
         dq = dn;
       }
-      F = vadd(vscale(Fv, kvol), vscale(Fs, R(2) * sc));
+      F = vadd(vscale(Fv, kvol_on), vscale(Fs, R(2) * sc));
       if (bend_on) {
         nx[v] = sb.x;
         ny[v] = sb.y;
         nz[v] = sb.z;
       }
       if (sub_on) {  // Cell3D_Kernel.cl:206-246, per facing-down incident face
-        int ndown = 0;
-        const uint4 vrow = rows[LW.vf / 16 + v];
-        for (int q = 0; q < DPM3D_MAXDEG; q++) {
-          const int e = row16(vrow, q);  // face*3 + corner
-          if (e == 0xffff) break;
-          const int f = e / 3;
-          ndown += (down[f >> 5] >> (f & 31)) & 1u;
-        }
-        if (ndown) F = vadd(F, vscale(substrate_corner(P, Ci, Ksi, ksg, L, l0i), (R)ndown));
+        const unsigned ndown = dcnt[v];
+        if (ndown) F = vadd(F, vscale(substrate_corner(P, Ci, Ksi, ksg, L, l0i, sub_inv_l0), (R)ndown));
       }
     } else if (shape_on) {
       const uint4 vrow = rows[LW.vf / 16 + v];
@@ -327,7 +335,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
             cc = vadd(cc, vscale(vsub(ea, eb), sc));
           }
         }
-        if (sub_on && ((down[f >> 5] >> (f & 31)) & 1u)) cc = vadd(cc, substrate_corner(P, Ci, Ksi, ksg, L, l0i));
+        if (sub_on && ((down[f >> 5] >> (f & 31)) & 1u)) cc = vadd(cc, substrate_corner(P, Ci, Ksi, ksg, L, l0i, sub_inv_l0));
         F = vadd(F, cc);
       }
     }
