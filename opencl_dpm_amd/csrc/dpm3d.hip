@@ -521,6 +521,18 @@ __device__ __forceinline__ V3<R> adhesion_term(V3<R> b, V3<R> a, V3<R> n, R L, R
   return {(R)(k * dx), (R)(k * dy), (R)(k * dz)};
 }
 
+// A float step's adhesion term on a pair without a periodic image, added to
+// F: Kat/2 (1/L0 - 1/d) delta is subtracted, with delta = pj - p the float
+// difference of two nearby positions (relative error 2^-24) and y = 1/d from
+// the hardware reciprocal square root, for relative error near 1e-7 (the
+// partner's term of the same pair, Kat/2 (1/L0j - 1/d) (p - pj), is the same
+// call with its own 1/L0).  Force arithmetic only, fused.
+__device__ __forceinline__ V3<float> adhesion_term_near(V3<float> delta, float y, float kh, float inv_l0,
+                                                        V3<float> F) {
+  const float k = kh * (inv_l0 - y);
+  return vmadd_f(delta, -k, F);
+}
+
 // Substrate force on one corner of a facing-down face
 // (Cell3D_Kernel.cl:229-245): a z < 0 penalty and the spreading push away
 // from the centre of mass within 2 l0 of the plane.  The decisions (z < 0,

@@ -614,6 +614,7 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
           const R half_l = R(0.49f) * L;
           const AdhCut<R> cut_own(L0i * R(2)), cut_rev(L0j * R(2));
           const double kh = 0.5 * (double)Kat, inv_l0i = sti.inv_l0, inv_l0j = stj.inv_l0;
+          const float kh_f = (float)kh, inv_l0i_f = (float)inv_l0i, inv_l0j_f = (float)inv_l0j;
           for (int tw = lane; tw < S * nmine; tw += DPM_WAVE) {
             const int iv = tw % nmine, sl = tw / nmine;
             const V3<R> Pv = lds_v(sx, sy, sz, (int)il[iv]);
@@ -624,12 +625,14 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
               const V3<R> pj = lds_v(nx, ny, nz, m);
               // delta = pj - p and p - pj, minimum image per component
               // (Cell3D_Kernel.cl:341-346); decisions in the reference's
-              // arithmetic, accepted forces in double on the same image
-              // Without an image, p - pj is -(pj - p) exactly and the two
-              // squared distances are the same number.
+              // arithmetic.  Without an image, p - pj is -(pj - p) exactly and
+              // the two squared distances are the same number; a float step
+              // then takes the accepted terms in float (adhesion_term_near),
+              // pairs across the boundary in double on the same image.
               V3<R> dl = vsub(pj, Pv);
               V3<R> nl = zero, nr = zero;
               R d2r;
+              bool imaged = false;
               if (!no_image && !(fabs(dl.x) < half_l && fabs(dl.y) < half_l && fabs(dl.z) < half_l)) {
                 V3<R> dr = vsub(Pv, pj);
                 nl = v3(round(dl.x / L), round(dl.y / L), round(dl.z / L));
@@ -637,10 +640,31 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
                 dl = vsub(dl, v3(L * nl.x, L * nl.y, L * nl.z));
                 dr = vsub(dr, v3(L * nr.x, L * nr.y, L * nr.z));
                 d2r = vdot(dr, dr);
+                imaged = true;
               } else {
                 d2r = vdot(dl, dl);
               }
               const R d2 = vdot(dl, dl);
+              if constexpr (sizeof(R) == 4) {
+                if (!imaged) {
+                  const bool own_on = cut_own(d2), rev_on = cut_rev(d2);
+                  if (own_on || rev_on) {
+                    const R y = __builtin_amdgcn_rsqf(d2);
+                    if (own_on) {
+                      Fa = adhesion_term_near(dl, y, kh_f, inv_l0i_f, Fa);
+                      cnt_own++;
+                      if (PROBE && a.p_adh_hash)
+                        atomicAdd(&a.p_adh_hash[2 * (base + il[iv])], vertex_key(pj.x, pj.y, pj.z));
+                    }
+                    if (rev_on) {
+                      Fa = adhesion_term_near(dl, y, kh_f, inv_l0j_f, Fa);
+                      if (PROBE && a.p_adh_hash)
+                        atomicAdd(&a.p_adh_hash[2 * (base + il[iv]) + 1], vertex_key(pj.x, pj.y, pj.z));
+                    }
+                  }
+                  continue;
+                }
+              }
               if (cut_own(d2)) {
                 Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i, kh, inv_l0i));
                 cnt_own++;
