@@ -1270,6 +1270,7 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
               const R half_l = R(0.49f) * L;
               const AdhCut<R> cut_own(L0i * R(2)), cut_rev(L0j * R(2));
               const double kh = 0.5 * (double)Kat, inv_l0i = sti.inv_l0, inv_l0j = stj.inv_l0;
+              const float kh_f = (float)kh, inv_l0i_f = (float)inv_l0i, inv_l0j_f = (float)inv_l0j;
               const int m1 = min(nnear, (sl + 1) * per);
               for (int m = sl * per; m < m1; m++) {
                 const int vj = jl[m];
@@ -1280,17 +1281,42 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
                 V3<R> dl = vsub(pj, Pv);
                 V3<R> dr = vsub(Pv, pj);  // the partner's loop: delta = p - pj
                 V3<R> nl = zero, nr = zero;  // image indices
+                bool imaged = false;
                 if (pbc && !(fabs(dl.x) < half_l && fabs(dl.y) < half_l && fabs(dl.z) < half_l)) {
                   nl = v3(round(dl.x / L), round(dl.y / L), round(dl.z / L));
                   nr = v3(round(dr.x / L), round(dr.y / L), round(dr.z / L));
                   dl = vsub(dl, v3(L * nl.x, L * nl.y, L * nl.z));
                   dr = vsub(dr, v3(L * nr.x, L * nr.y, L * nr.z));
+                  imaged = true;
                 }
-                // The decisions are the reference's ones (AdhCut); the force of an
-                // accepted pair is evaluated in double on the same image,
-                // since across the periodic boundary the float difference
-                // of two near-L numbers keeps few digits of a short vector.
+                // The decisions are the reference's ones (AdhCut).  The force
+                // of an accepted pair across the periodic boundary is
+                // evaluated in double on the same image, since there the
+                // float difference of two near-L numbers keeps few digits of a
+                // short vector; without an image (dr = -dl exactly, the same
+                // squared distance) a float step takes it in float
+                // (adhesion_term_near).
                 const R d2 = vdot(dl, dl);
+                if constexpr (sizeof(R) == 4) {
+                  if (!imaged) {
+                    const bool own_on = cut_own(d2), rev_on = cut_rev(d2);
+                    if (own_on || rev_on) {
+                      const R y = __builtin_amdgcn_rsqf(d2);
+                      if (own_on) {
+                        Fa = adhesion_term_near(dl, y, kh_f, inv_l0i_f, Fa);
+                        cnt_own++;
+                        if (PROBE && a.p_adh_hash)
+                          atomicAdd(&a.p_adh_hash[2 * (base + il[iv])], vertex_key(pj.x, pj.y, pj.z));
+                      }
+                      if (rev_on) {
+                        Fa = adhesion_term_near(dl, y, kh_f, inv_l0j_f, Fa);
+                        if (PROBE && a.p_adh_hash)
+                          atomicAdd(&a.p_adh_hash[2 * (base + il[iv]) + 1], vertex_key(pj.x, pj.y, pj.z));
+                      }
+                    }
+                    continue;
+                  }
+                }
                 if (cut_own(d2)) {
                   Fa = vsub(Fa, adhesion_term(pj, Pv, nl, L, Kat, L0i, kh, inv_l0i));
                   cnt_own++;
