@@ -209,6 +209,15 @@ int dpm3d_energy(dpm_tissue* h, double* energy);
  * bit-identical results. */
 int dpm3d_set_cell_list(dpm_tissue* h, int mode);
 
+/* Bin width of the cell list over the step's largest candidate reach, a
+ * factor in [1, 4] (default 1, the narrowest bins the 27-bucket gather
+ * allows); wider bins mean fewer, fuller buckets.  Every factor gives
+ * bit-identical results.  No reference counterpart (the reference scans
+ * every cell pair, Cell3D_Kernel.cl:273-310, 325-364); dpm3d_autotune tunes
+ * it.  dpm3d_bin_width returns the factor (-1 for a bad handle). */
+int dpm3d_set_bin_width(dpm_tissue* h, float factor);
+float dpm3d_bin_width(dpm_tissue* h);
+
 /* Kernel form of the 3D step (no reference counterpart; the reference
  * launches one work-item per vertex per force term, Tissue3D.cpp:372-445):
  * 1 one workgroup per cell; 2 one 64-lane wave per cell (meshes of at most
@@ -225,11 +234,13 @@ int dpm3d_set_step_kernel(dpm_tissue* h, int kind);
  * Configurations, in the order of times_ms[8] (ms per step; -1 where a
  * configuration does not apply): workgroup form with the scan, workgroup
  * form with the cell list, then the wave form with 1, 2, 4 cells per
- * workgroup with the scan, and the same with the cell list.  best[3] (may
- * be NULL) receives {form, cells per workgroup (0 for the workgroup form),
- * cell list}; the handle keeps them (dpm3d_set_step_kernel and
- * dpm3d_set_cell_list override).  Results do not depend on the choice
- * beyond float rounding of the grouped sums. */
+ * workgroup with the scan, and the same with the cell list; each cell-list
+ * configuration is timed at bin widths 1, 1.5 and 2 (dpm3d_set_bin_width)
+ * and reports its fastest.  best[3] (may be NULL) receives {form, cells per
+ * workgroup (0 for the workgroup form), cell list}; the handle keeps them
+ * and the winning bin width (dpm3d_bin_width reads it; dpm3d_set_step_kernel,
+ * dpm3d_set_cell_list and dpm3d_set_bin_width override).  Results do not
+ * depend on the choice beyond float rounding of the grouped sums. */
 int dpm3d_autotune(dpm_tissue* h, int steps, float dt, int* best, float* times_ms);
 
 /* Differentiation form (no reference counterpart).  dpm3d_trajectory runs

@@ -37,7 +37,8 @@ EXPORTS = [
     "dpm_disperse", "dpm2d_set_step_kernel", "dpm3d_probe_contact",
     "dpm3d_create_f64", "dpm3d_set_positions_f64", "dpm3d_get_positions_f64",
     "dpm3d_get_forces_f64", "dpm3d_energy", "dpm3d_trajectory", "dpm3d_adjoint", "dpm3d_adjoint_f64",
-    "dpm3d_set_compensated", "dpm3d_set_cell_list", "dpm3d_set_step_kernel", "dpm3d_autotune",
+    "dpm3d_set_compensated", "dpm3d_set_cell_list", "dpm3d_set_bin_width", "dpm3d_bin_width",
+    "dpm3d_set_step_kernel", "dpm3d_autotune",
     "dpm2d_create", "dpm2d_set_positions", "dpm2d_get_positions", "dpm2d_get_forces",
     "dpm2d_set_cell_params", "dpm2d_set_tissue_params", "dpm2d_probe",
     "dpm2d_create_f64", "dpm2d_set_positions_f64", "dpm2d_get_positions_f64", "dpm2d_get_forces_f64",
@@ -84,6 +85,9 @@ def load() -> ctypes.CDLL:
     lib.dpm3d_trajectory.argtypes = [vp, i, fp, i]
     lib.dpm3d_set_compensated.argtypes = [vp, i]
     lib.dpm3d_set_cell_list.argtypes = [vp, i]
+    lib.dpm3d_set_bin_width.argtypes = [vp, fp]
+    lib.dpm3d_bin_width.argtypes = [vp]
+    lib.dpm3d_bin_width.restype = fp
     lib.dpm3d_set_step_kernel.argtypes = [vp, i]
     lib.dpm3d_autotune.argtypes = [vp, i, fp, vp, vp]
     lib.dpm3d_adjoint.argtypes = [vp, vp, vp, vp]
@@ -303,6 +307,14 @@ class Tissues3D(_Handle):
         """Candidate search: 0 scan, 1 binned cell list, -1 automatic."""
         _check(load().dpm3d_set_cell_list(self._h, int(mode)))
 
+    def set_bin_width(self, factor: float = 1.0) -> None:
+        """Cell-list bin width over the largest candidate reach, in [1, 4]
+        (dpm3d_set_bin_width)."""
+        _check(load().dpm3d_set_bin_width(self._h, ctypes.c_float(factor)))
+
+    def bin_width(self) -> float:
+        return float(load().dpm3d_bin_width(self._h))
+
     def set_step_kernel(self, kind: int = 0) -> None:
         """0 automatic, 1 one workgroup per cell, 2 one wave per cell
         (dpm3d_set_step_kernel)."""
@@ -318,7 +330,8 @@ class Tissues3D(_Handle):
         best = np.zeros(3, np.int32)
         t = np.zeros(8, np.float32)
         _check(load().dpm3d_autotune(self._h, int(steps), ctypes.c_float(dt), _p(best), _p(t)))
-        return ({"form": int(best[0]), "cells_per_workgroup": int(best[1]), "cell_list": int(best[2])},
+        return ({"form": int(best[0]), "cells_per_workgroup": int(best[1]), "cell_list": int(best[2]),
+                 "bin_width": self.bin_width()},
                 {k: float(v) for k, v in zip(self.AUTOTUNE_CONFIGS, t) if v >= 0})
 
     def set_compensated(self, on: bool = True) -> None:

@@ -611,6 +611,7 @@ struct Bin3D {
   const int* tbin_hoff;
   int* bin_start;
   int* bin_list;
+  float wscale;  // bin width over the largest candidate reach (>= 1)
 };
 
 // One workgroup per tissue; LDS: nbuck counters + bucket and rank per cell.
@@ -645,6 +646,7 @@ __device__ void k3d_bin_body(const Bin3D<R>& a) {
     all_images = R(2) * r2 >= R(0.999f) * L;
   }
   reach = (reach * R(1.0001f) + R(1e-6f)) * R(1.001f);  // the step's slack, and more
+  reach *= (R)a.wscale;  // wider bins: fewer, fuller buckets (any width >= the reach is exact)
   int nb = (int)floor(L / reach);
   if (!a.tpbc[t] || all_images || nb < 3 || !(do_contact || do_adh) || nbuck == 0) nb = 0;
   if (nb > 1024) nb = 1024;

@@ -135,6 +135,7 @@ struct dpm_tissue {
   // cell list (counting sort of the centres of mass into hashed bins)
   int cell_list_mode = -1;  // -1 automatic, 0 off, 1 on
   bool cells_on = false;
+  float bin_scale = 1.0f;  // cell-list bin width over the step's largest candidate reach
   int *bin_start = nullptr, *bin_list = nullptr, *tbin_nb = nullptr, *tbin_nbuck = nullptr,
       *tbin_hoff = nullptr;
   size_t bin_lds = 0;
@@ -549,6 +550,19 @@ static int create3d(const dpm3d_desc* d, dpm_tissue** out, int rs) {
   return DPM_OK;
 }
 
+extern "C" int dpm3d_set_bin_width(dpm_tissue* h, float factor) {
+  if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
+  if (!(factor >= 1.0f && factor <= 4.0f)) return fail(DPM_EINVAL, "the bin width factor must be in [1, 4]");
+  h->bin_scale = factor;
+  if (h->graph) {  // captured with the old arguments
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return DPM_OK;
+}
+
+extern "C" float dpm3d_bin_width(dpm_tissue* h) { return h && h->dim == 3 ? h->bin_scale : -1.0f; }
+
 extern "C" int dpm3d_set_cell_list(dpm_tissue* h, int mode) {
   if (!h || h->dim != 3) return fail(DPM_EINVAL, "not a 3D tissue");
   if (mode < -1 || mode > 1) return fail(DPM_EINVAL, "mode must be -1 (automatic), 0 or 1");
@@ -846,6 +860,7 @@ static int launch3d(dpm_tissue* h, Kind3D kind, int in, float dt, int flags, Edi
       b.tbin_hoff = h->tbin_hoff;
       b.bin_start = h->bin_start;
       b.bin_list = h->bin_list;
+      b.wscale = h->bin_scale;
       void (*kb)(Bin3D<R>) = nullptr;
       if constexpr (sizeof(R) == 8) kb = k3d_bin_f64; else kb = k3d_bin_f32;
       hipLaunchKernelGGL(kb, dim3(h->T), dim3(256), h->bin_lds, h->stream, b);
@@ -990,6 +1005,10 @@ extern "C" int dpm3d_autotune(dpm_tissue* h, int steps, float dt, int* best, flo
   const int cfg[8][3] = {{1, 0, 0}, {1, 0, 1}, {2, 1, 0}, {2, 2, 0}, {2, 4, 0}, {2, 1, 1}, {2, 2, 1}, {2, 4, 1}};
   const int kind0 = h->step_kernel, cpw0 = h->wave_cpw, mode0 = h->cell_list_mode;
   const bool cells0 = h->cells_on;
+  // the cell-list configurations each take the fastest of three bin widths
+  const float scales[3] = {1.0f, 1.5f, 2.0f};
+  const float scale0 = h->bin_scale;
+  float sbest = scale0;
   float tbest = 0.0f;
   int ib = -1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1001,27 +1020,33 @@ extern "C" int dpm3d_autotune(dpm_tissue* h, int steps, float dt, int* best, flo
     h->step_kernel = cfg[i][0];
     if (cfg[i][0] == 2) rc = set_wave_cpw(h, cfg[i][1]);
     h->cells_on = cfg[i][2] == 1;
-    for (int rep = 0; rep < 2 && !rc; rep++) {  // a warm-up launch, then the timed steps
-      rc = copy_all(false);
-      if (rc) break;
-      if (rep == 0) {
-        rc = launch_step3d(h, h->cur, dt, F_MOVE);
-        continue;
-      }
-      if (hipEventRecord(e0, h->stream) != hipSuccess) rc = fail(DPM_EDEVICE, "hipEventRecord failed");
-      for (int k = 0; k < steps && !rc; k++) {
-        rc = launch_step3d(h, h->cur, dt, F_MOVE);
-        h->cur = 1 - h->cur;
-      }
-      if (!rc && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
-        rc = fail(DPM_EDEVICE, "timing failed");
-      float ms = 0.0f;
-      if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(DPM_EDEVICE, "timing failed");
-      if (!rc) {
-        if (times_ms) times_ms[i] = ms / (float)steps;
-        if (ib < 0 || ms < tbest) {
-          tbest = ms;
-          ib = i;
+    const int nscale = cfg[i][2] == 1 ? 3 : 1;
+    for (int si = 0; si < nscale && !rc; si++) {
+      h->bin_scale = cfg[i][2] == 1 ? scales[si] : scale0;
+      for (int rep = 0; rep < 2 && !rc; rep++) {  // a warm-up launch, then the timed steps
+        rc = copy_all(false);
+        if (rc) break;
+        if (rep == 0) {
+          rc = launch_step3d(h, h->cur, dt, F_MOVE);
+          continue;
+        }
+        if (hipEventRecord(e0, h->stream) != hipSuccess) rc = fail(DPM_EDEVICE, "hipEventRecord failed");
+        for (int k = 0; k < steps && !rc; k++) {
+          rc = launch_step3d(h, h->cur, dt, F_MOVE);
+          h->cur = 1 - h->cur;
+        }
+        if (!rc && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+          rc = fail(DPM_EDEVICE, "timing failed");
+        float ms = 0.0f;
+        if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(DPM_EDEVICE, "timing failed");
+        if (!rc) {
+          const float per = ms / (float)steps;
+          if (times_ms && (times_ms[i] < 0.0f || per < times_ms[i])) times_ms[i] = per;
+          if (ib < 0 || ms < tbest) {
+            tbest = ms;
+            ib = i;
+            sbest = h->bin_scale;
+          }
         }
       }
     }
@@ -1037,8 +1062,10 @@ extern "C" int dpm3d_autotune(dpm_tissue* h, int steps, float dt, int* best, flo
     if (cpw0) (void)set_wave_cpw(h, cpw0);
     h->cells_on = cells0;
     h->cell_list_mode = mode0;
+    h->bin_scale = scale0;
     return rc ? rc : fail(DPM_EDEVICE, "no configuration ran");
   }
+  h->bin_scale = sbest;
   h->step_kernel = cfg[ib][0];
   if (cfg[ib][0] == 2) TRY(set_wave_cpw(h, cfg[ib][1]));
   h->cells_on = cfg[ib][2] == 1;

@@ -405,6 +405,8 @@ def test_3d_autotune_restores_state_and_keeps_a_configuration():
     u = handle()
     u.set_step_kernel(best["form"])
     u.set_cell_list(best["cell_list"])
+    assert best["bin_width"] in (1.0, 1.5, 2.0)
+    u.set_bin_width(best["bin_width"])
     u.set_positions(X0)
     u.euler(5, 0.002)
     u.euler(20, 0.002)
@@ -600,25 +602,29 @@ def test_3d_cell_list_equals_scan(n_side, spacing, subdiv, kernel):
     # bins) against scanning every cell: identical pair sets, forces and
     # trajectories, bit for bit (the 3 x 3 box is too small for bins: the
     # cell list falls back to the scan)
+    # at every bin width the autotuner tries, and a wider one
     m = W.monolayer3d(n_side, spacing, subdiv=subdiv, seed=9)
     if kernel == 2 and m.mesh.nv > 256:
         pytest.skip("the wave-per-cell step takes meshes of at most 256 vertices")
     out = []
-    for mode in (0, 1):
+    for mode, width in ((0, 1.0), (1, 1.0), (1, 1.5), (1, 2.0), (1, 3.0)):
         t = tissue3d(m)
         t.set_step_kernel(kernel)
         t.set_cell_list(mode)
+        t.set_bin_width(width)
+        assert t.bin_width() == width
         t.set_positions(m.X)
         g = t.probe()
         t.euler(20, 0.002)
         out.append((g, t.positions(), t.forces()))
-    (g0, X0, F0), (g1, X1, F1) = out
-    np.testing.assert_array_equal(g0["forces"], g1["forces"])
-    np.testing.assert_array_equal(g0["contact"], g1["contact"])
-    np.testing.assert_array_equal(g0["adhesion"], g1["adhesion"])
-    np.testing.assert_array_equal(g0["adhesion_hash"], g1["adhesion_hash"])
-    np.testing.assert_array_equal(X0, X1)
-    np.testing.assert_array_equal(F0, F1)
+    g0, X0, F0 = out[0]
+    for g1, X1, F1 in out[1:]:
+        np.testing.assert_array_equal(g0["forces"], g1["forces"])
+        np.testing.assert_array_equal(g0["contact"], g1["contact"])
+        np.testing.assert_array_equal(g0["adhesion"], g1["adhesion"])
+        np.testing.assert_array_equal(g0["adhesion_hash"], g1["adhesion_hash"])
+        np.testing.assert_array_equal(X0, X1)
+        np.testing.assert_array_equal(F0, F1)
     print(f"cell list {n_side}x{n_side} s={subdiv} kernel={kernel}: contacts {int(g0['contact'].sum())}, "
           f"adhesion {int(g0['adhesion'].sum())}", flush=True)
 
