@@ -588,6 +588,41 @@ __device__ __forceinline__ unsigned bin_bucket(int bx, int by, int bz, int nb, i
   return h & (unsigned)(nbuck - 1);
 }
 
+// The distinct buckets among the 27 around a cell, ascending, with their
+// cells' starts (gst[u]) and the prefix of their sizes (gof[u]; gof[nu] =
+// the total), from one whole wave: lane l < 27 brings bucket id g (the
+// other lanes' g is not read).  Hashed buckets may repeat an id; a repeat is
+// kept at its first lane only.  Every lane gets nu.  Replaces a serial sort
+// on one lane: 27 uniform rounds of lane reads instead of a chain of
+// dependent LDS accesses.
+__device__ __forceinline__ int distinct_buckets27(int g, const int* bs, int* gst, int* gof) {
+  const int lane = threadIdx.x & (DPM_WAVE - 1);
+  const bool in = lane < 27;
+  bool dup = false;
+#pragma unroll
+  for (int k = 0; k < 27; k++) dup = dup || (k < lane && __builtin_amdgcn_readlane(g, k) == g);
+  const bool keep = in && !dup;
+  const unsigned long long km = __ballot(keep);
+  const int start = keep ? bs[g] : 0;
+  const int cnt = keep ? bs[g + 1] - start : 0;
+  int rank = 0, off = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < 27; k++) {
+    const int gk = __builtin_amdgcn_readlane(g, k), ck = __builtin_amdgcn_readlane(cnt, k);
+    const bool below = ((km >> k) & 1ull) && gk < g;
+    rank += below ? 1 : 0;
+    off += below ? ck : 0;
+    total += ck;
+  }
+  if (keep) {
+    gst[rank] = start;
+    gof[rank] = off;
+  }
+  const int nu = (int)__popcll(km);
+  if (lane == 0) gof[nu] = total;
+  return nu;
+}
+
 template <typename R>
 __device__ __forceinline__ int bin_coord(R x, R L, R wb, int nb) {
   const R xw = x - L * floor(x / L);
@@ -1004,33 +1039,16 @@ __device__ void k3d_step_body(const Step3D<R>& a) {
       int* gid = l.gath;        // [27] bucket ids
       int* gst = l.gath + 32;   // [27] bucket starts
       int* gof = l.gath + 64;   // [28] prefix of bucket sizes
-      if (tid < 27) {
-        const int bx = bin_coord(Ci.x, L, wb, nb), by = bin_coord(Ci.y, L, wb, nb),
-                  bz = bin_coord(Ci.z, L, wb, nb);
-        const int dx = tid % 3 - 1, dy = (tid / 3) % 3 - 1, dz = tid / 9 - 1;
-        gid[tid] = (int)bin_bucket((bx + dx + nb) % nb, (by + dy + nb) % nb, (bz + dz + nb) % nb, nb, nbuck);
-      }
-      __syncthreads();
-      if (tid == 0) {  // distinct buckets, their cells' offsets
-        for (int i = 1; i < 27; i++) {
-          const int v = gid[i];
-          int j = i - 1;
-          while (j >= 0 && gid[j] > v) {
-            gid[j + 1] = gid[j];
-            j--;
-          }
-          gid[j + 1] = v;
+      if (tid < DPM_WAVE) {  // the whole first wave: distinct buckets, their cells' offsets
+        int g = 0;
+        if (tid < 27) {
+          const int bx = bin_coord(Ci.x, L, wb, nb), by = bin_coord(Ci.y, L, wb, nb),
+                    bz = bin_coord(Ci.z, L, wb, nb);
+          const int dx = tid % 3 - 1, dy = (tid / 3) % 3 - 1, dz = tid / 9 - 1;
+          g = (int)bin_bucket((bx + dx + nb) % nb, (by + dy + nb) % nb, (bz + dz + nb) % nb, nb, nbuck);
         }
-        int nu = 0, run = 0;
-        for (int i = 0; i < 27; i++) {
-          if (i > 0 && gid[i] == gid[i - 1]) continue;
-          gst[nu] = bs[gid[i]];
-          gof[nu] = run;
-          run += bs[gid[i] + 1] - bs[gid[i]];
-          nu++;
-        }
-        gof[nu] = run;
-        gid[31] = nu;
+        const int nu = distinct_buckets27(g, bs, gst, gof);
+        if (tid == 0) gid[31] = nu;
       }
       __syncthreads();
       const int nu = gid[31], n = gof[nu];

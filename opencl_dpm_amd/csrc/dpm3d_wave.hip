@@ -398,38 +398,18 @@ __device__ void k3d_wstep_body(const Step3D<R>& a) {
       const int nb = a.tbin_nb[t], nbuck = a.tbin_nbuck[t];
       const int* bs = a.bin_start + a.tbin_hoff[t];
       const R wb = L / (R)nb;
-      int* gid = gath;
       int* gst = gath + 32;
       int* gof = gath + 64;
+      int g = 0;
       if (lane < 27) {
         const int bx = bin_coord(Ci.x, L, wb, nb), by = bin_coord(Ci.y, L, wb, nb), bz = bin_coord(Ci.z, L, wb, nb);
         const int dx = lane % 3 - 1, dy = (lane / 3) % 3 - 1, dz = lane / 9 - 1;
-        gid[lane] = (int)bin_bucket((bx + dx + nb) % nb, (by + dy + nb) % nb, (bz + dz + nb) % nb, nb, nbuck);
+        g = (int)bin_bucket((bx + dx + nb) % nb, (by + dy + nb) % nb, (bz + dz + nb) % nb, nb, nbuck);
       }
+      // distinct buckets, their cells' offsets
+      const int nu = distinct_buckets27(g, bs, gst, gof);
       wave_lds_sync();
-      if (lane == 0) {  // distinct buckets, their cells' offsets
-        for (int i = 1; i < 27; i++) {
-          const int v = gid[i];
-          int j = i - 1;
-          while (j >= 0 && gid[j] > v) {
-            gid[j + 1] = gid[j];
-            j--;
-          }
-          gid[j + 1] = v;
-        }
-        int nu = 0, run = 0;
-        for (int i = 0; i < 27; i++) {
-          if (i > 0 && gid[i] == gid[i - 1]) continue;
-          gst[nu] = bs[gid[i]];
-          gof[nu] = run;
-          run += bs[gid[i] + 1] - bs[gid[i]];
-          nu++;
-        }
-        gof[nu] = run;
-        gid[31] = nu;
-      }
-      wave_lds_sync();
-      const int nu = gid[31], n = gof[nu];
+      const int n = gof[nu];
       if (n <= kWaveGlist) {  // wave-uniform; otherwise scan every cell
         int val0 = -1, val1 = -1;
         if (lane < n) {
