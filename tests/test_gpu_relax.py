@@ -178,6 +178,32 @@ def test_fire_batch_equals_individual():
         off += len(p.X)
 
 
+def test_fire_on_the_full_512_packing_ensemble():
+    """The batching form's whole ensemble (512 packings of 8-128 cells with
+    16-64 vertices, 1.39 M vertices) in one double handle: 2,000 FIRE
+    iterations leave every packing finite and no higher in energy, and three
+    packings (the smallest, one from the middle, the largest) give the same
+    bits and iteration counts run alone as in the batch (equal workgroup
+    size, so the same reduction order)."""
+    ps = W.packings2d(512, seed=0)
+    sizes = np.array([len(p.X) for p in ps])
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    b = handle(ps, threads=512)
+    e0 = b.energy()["energy"]
+    sb = b.fire(max_iter=2000)
+    Xb = b.positions()
+    print(f"512 packings, {int(offs[-1])} vertices: iterations {int(sb['iterations'].min())}-"
+          f"{int(sb['iterations'].max())}, converged {int(sb['converged'].sum())}", flush=True)
+    assert len(sb["energy"]) == 512
+    assert np.all(np.isfinite(Xb))
+    assert np.all(sb["energy"] <= e0 * (1 + 1e-9) + 1e-12)
+    for k in (int(np.argmin(sizes)), 256, int(np.argmax(sizes))):
+        t = handle(ps[k], threads=512)
+        s1 = t.fire(max_iter=2000)
+        np.testing.assert_array_equal(Xb[offs[k]:offs[k + 1]], t.positions())
+        assert s1["iterations"][0] == sb["iterations"][k]
+
+
 def test_jam_graph_replay_and_autotune():
     ps = W.packings2d(4, cells=(4, 8), verts=(16, 24), seed=6)
     t = handle(ps)
