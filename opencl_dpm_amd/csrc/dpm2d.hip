@@ -333,26 +333,60 @@ __device__ __forceinline__ bool outside_box_even(const Lds2D<R>& l, int cj, R px
   return px < x0 - m || px > x1 + m || py < y0 - m || py > y1 + m;
 }
 
-// Crossing parity of every vertex against every candidate cell, one
-// (vertex, candidate) pair per thread and iteration, so that a few vertices
-// with many wrapped candidates spread over the whole workgroup: bit k of
+// Crossing parity of every vertex against every candidate cell: bit k of
 // `bits` (cleared beforehand) is set when vertex k lies inside some
 // candidate by the odd-crossing rule (Cell2D_kernel.cl:165-202).  That is
 // the OR force_vertex forms over the same candidates, so the two agree bit
-// for bit.  Pairs q = w / nvt beyond a cell's candidate count do nothing.
+// for bit.  First one (vertex, candidate) pair per thread and iteration
+// takes the cheap tests (a candidate slot in range, not the own cell, not
+// outside the candidate's box); the pairs that pass are appended to `list`
+// (wave-aggregated LDS atomics on *count, zeroed beforehand), and after a
+// workgroup barrier the crossing loops run on the list over full waves —
+// left in place, the few passing pairs of each wave would hold all its
+// lanes through their loops.  Pairs beyond `cap` entries (cap 0: no list)
+// are tested where they are found.  The bits are an OR, whatever the order.
 template <typename R>
-__device__ void over_pass(const Tissue2DCtx<R>& T, const Lds2D<R>& l, unsigned* bits) {
-  const int nvt = T.nvt, nc = T.nc;
+__device__ void over_pass(const Tissue2DCtx<R>& T, const Lds2D<R>& l, unsigned* bits, int* list, int cap,
+                          int* count) {
+  const int nvt = T.nvt, nc = T.nc, lane = threadIdx.x & (DPM_WAVE - 1);
   for (int w = threadIdx.x; w < nvt * nc; w += blockDim.x) {
     const int k = w % nvt, q = w / nvt;
     const int ci = cell_of(l.vb, nc, k);
     const int ncd = l.ncand[ci];
     const bool scan_all = ncd > DPM2D_MAXCAND;
-    if (q >= (scan_all ? nc : ncd)) continue;
-    const int cj = scan_all ? q : (int)l.cand[ci * DPM2D_MAXCAND + q];
-    const R px = l.x[k], py = l.y[k];
-    if (cj == ci || outside_box_even(l, cj, px, py, T.pbc, T.L)) continue;
-    if (crossing_odd(l, cj, px, py, T.pbc, T.L)) atomicOr(&bits[k >> 5], 1u << (k & 31));
+    bool pass = false;
+    int cj = 0;
+    R px = R(0), py = R(0);
+    if (q < (scan_all ? nc : ncd)) {
+      cj = scan_all ? q : (int)l.cand[ci * DPM2D_MAXCAND + q];
+      px = l.x[k];
+      py = l.y[k];
+      pass = cj != ci && !outside_box_even(l, cj, px, py, T.pbc, T.L);
+    }
+    if (cap == 0) {  // block-uniform: no list (count and list are not allocated)
+      if (pass && crossing_odd(l, cj, px, py, T.pbc, T.L)) atomicOr(&bits[k >> 5], 1u << (k & 31));
+      continue;
+    }
+    const unsigned long long m = __ballot(pass);
+    if (m == 0ull) continue;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, (int)__popcll(m));
+    base = __shfl(base, leader);
+    if (pass) {
+      const int slot = base + (int)__popcll(m & ((1ull << lane) - 1ull));
+      if (slot < cap)
+        list[slot] = k | (cj << 16);
+      else if (crossing_odd(l, cj, px, py, T.pbc, T.L))
+        atomicOr(&bits[k >> 5], 1u << (k & 31));
+    }
+  }
+  if (cap == 0) return;
+  __syncthreads();
+  const int n = min(*count, cap);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int e = list[i], k = e & 0xffff, cj = e >> 16;
+    if (crossing_odd(l, cj, l.x[k], l.y[k], T.pbc, T.L)) atomicOr(&bits[k >> 5], 1u << (k & 31));
   }
 }
 
@@ -550,7 +584,13 @@ struct Run2D {
   int* p_overlap;                // diagnostics, per vertex (nullptr normally)
   int* p_attr;
   unsigned* p_attr_key;  // attraction partner-set key sums (vertex_key)
+  int over_cap;          // entries of the crossing-parity pair list after the LDS arrays (0: none)
 };
+
+// The crossing-parity pair list of k2d_run (a count, then the entries; the
+// entries pack vertex | cell << 16): its LDS after a tissue's Lds2D arrays.
+#define DPM2D_OVER_CAP 2048
+__host__ __device__ inline size_t lds2d_over_bytes() { return sizeof(int) * (size_t)(DPM2D_OVER_CAP + 1); }
 
 // Forces stay in registers: up to DPM2D_VPT vertices per thread.
 #define DPM2D_VPT 16
@@ -573,6 +613,8 @@ extern "C" __global__ void __launch_bounds__(1024) k2d_run(const Run2D a) {
   const bool split_over = (g.mask & M2_REPULSION) && T.nvt <= 32 * DPM2D_CSPLIT * T.nc &&
                           T.nvt * T.nc <= 16 * NT;
   unsigned* obits = l.cbits;
+  int* ocount = reinterpret_cast<int*>(smem + lds2d_bytes(T.nvt, T.nc, (int)sizeof(float)));
+  int* olist = ocount + 1;
   float rfx[DPM2D_VPT], rfy[DPM2D_VPT];
   for (int s = 0; s < nevals; s++) {
     cell_pass(g, T, l);
@@ -581,8 +623,9 @@ extern "C" __global__ void __launch_bounds__(1024) k2d_run(const Run2D a) {
     __syncthreads();
     if (split_over) {  // block-uniform; the candidate bits are dead here
       for (int i = tid; i < (T.nvt + 31) / 32; i += NT) obits[i] = 0u;
+      if (tid == 0 && a.over_cap > 0) *ocount = 0;
       __syncthreads();
-      over_pass(T, l, obits);
+      over_pass(T, l, obits, olist, a.over_cap, ocount);
       __syncthreads();
     }
 #pragma unroll

@@ -143,6 +143,8 @@ struct dpm_tissue {
   // ------------------------------------------------------------------ 2D
   int V = 0, NT2 = 0, NTr = 0;  // Euler threads, relax threads per tissue
   size_t lds2 = 0;
+  size_t lds_run = 0;  // k2d_run: lds2 and, where it fits, the crossing-parity pair list
+  int over_cap = 0;
   // arrays of the real type (rs bytes)
   void *x2 = nullptr, *y2 = nullptr, *fx2 = nullptr, *fy2 = nullptr;
   void *vx2 = nullptr, *vy2 = nullptr, *sx2 = nullptr, *sy2 = nullptr;  // FIRE / jamming
@@ -1632,6 +1634,14 @@ static int create2d(const dpm2d_desc* d, dpm_tissue** out, int rs) {
     return fail(DPM_ERANGE, "a double 2D tissue must fit the LDS of one workgroup");
   }
   if (!h->fits_wg) h->lds2 = 0;
+  // the crossing-parity pair list where a tissue can take the pass and the
+  // LDS allows it (without it the pass tests every pair where it finds it)
+  h->lds_run = h->lds2;
+  h->over_cap = 0;
+  if (h->fits_wg && any_pass && (int)(h->lds2 + lds2d_over_bytes()) <= max_lds_per_block(h->device)) {
+    h->lds_run = h->lds2 + lds2d_over_bytes();
+    h->over_cap = DPM2D_OVER_CAP;
+  }
   h->max_cells = maxc;
   h->cvb_h = cvb;
   h->tissue_begin_h.assign(d->cell_begin, d->cell_begin + T + 1);
@@ -1666,7 +1676,7 @@ static int create2d(const dpm2d_desc* d, dpm_tissue** out, int rs) {
   if (!rc) rc = dpm2d_set_tissue_params(h, d->L, d->pbc, d->Kre, d->Kat, d->mask);
   if (rc) { delete h; return rc; }
   if (h->fits_wg) {
-    (void)hipFuncSetAttribute((const void*)k2d_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds2);
+    (void)hipFuncSetAttribute((const void*)k2d_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_run);
     (void)hipFuncSetAttribute(rs == 8 ? (const void*)k2d_relax_f64 : (const void*)k2d_relax_f32,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds2);
   }
@@ -1965,6 +1975,7 @@ static Run2D run2d_args(dpm_tissue* h, int nsteps, float dt) {
   a.nsteps = nsteps;
   a.dt = dt;
   a.counters = h->counters;
+  a.over_cap = h->over_cap;
   return a;
 }
 
@@ -2075,7 +2086,7 @@ static int euler2d(dpm_tissue* h, int nsteps, float dt) {
   }
   if (use_wide(h)) return euler2d_wide(h, nsteps, dt);
   Run2D a = run2d_args(h, nsteps, dt);
-  hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds2, h->stream, a);
+  hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds_run, h->stream, a);
   HIPCHECK(hipGetLastError());
   return DPM_OK;
 }
@@ -2256,7 +2267,7 @@ extern "C" int dpm2d_probe(dpm_tissue* h, float* forces, int* overlap, int* attr
       a.counters = nullptr;
       a.p_overlap = dov;
       a.p_attr = dat;
-      hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds2, s, a);
+      hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds_run, s, a);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) rc = fail(DPM_EDEVICE, std::string("k2d_run: ") + hipGetErrorString(e));
     }
@@ -2292,7 +2303,7 @@ extern "C" int dpm2d_probe_attraction(dpm_tissue* h, unsigned* keys) {
       Run2D a = run2d_args(h, 0, 0.0f);
       a.counters = nullptr;
       a.p_attr_key = dk;
-      hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds2, s, a);
+      hipLaunchKernelGGL(k2d_run, dim3(h->T), dim3(h->NT2), h->lds_run, s, a);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) rc = fail(DPM_EDEVICE, std::string("k2d_run: ") + hipGetErrorString(e));
     }
