@@ -292,27 +292,71 @@ __device__ __forceinline__ bool vertex_far_from_box(const Lds2D<R>& l, const Tis
   return fabs(dx) - hx > reach || fabs(dy) - hy > reach;
 }
 
-// Crossing parity of point p against cell cj, Cell2D_kernel.cl:171-196.
+// floor(x / L) and round(x / L) as the reference evaluates them (a
+// correctly rounded quotient), from q = x (1/L) where that is certain: q is
+// within |q| 2^-22 of the rounded quotient, so where q is more than
+// 1e-6 (1 + |q|) away from every integer (for round, every half-integer)
+// both give the same result; closer, the division is taken.
 template <typename R>
-__device__ __forceinline__ bool crossing_odd(const Lds2D<R>& l, int cj, R px, R py, int pbc, R L) {
+__device__ __forceinline__ R floor_quot(R x, R L, R invL) {
+  if constexpr (sizeof(R) == 4) {
+    const R q = x * invL, f = floor(q), m = R(1e-6f) * (R(1) + fabs(q));
+    if (q - f > m && (f + R(1)) - q > m) return f;
+  }
+  return floor(x / L);
+}
+template <typename R>
+__device__ __forceinline__ R round_quot(R x, R L, R invL) {
+  if constexpr (sizeof(R) == 4) {
+    const R q = x * invL, m = R(1e-6f) * (R(1) + fabs(q));
+    const R h = fabs(q) - floor(fabs(q));  // distance above the integer below |q|
+    if (fabs(h - R(0.5f)) > m) return round(q);
+  }
+  return round(x / L);
+}
+
+// Crossing parity of point p against cell cj, Cell2D_kernel.cl:171-196.
+// The crossing abscissa tt = (djx - dix)(0 - diy)/(djy - diy) + dix only
+// decides 0 < tt: for float its sign is taken from the product with the
+// hardware reciprocal (relative error below 2^-21) where |tt| exceeds
+// 1e-6 of its terms, else from the correctly rounded division.  (The
+// rounded sum of two floats is 0 only when the exact sum is, so the sign of
+// tt is the sign of q + dix for the rounded quotient q.)
+template <typename R>
+__device__ __forceinline__ bool crossing_odd(const Lds2D<R>& l, int cj, R px, R py, int pbc, R L,
+                                             int first = 0, int stride = 1) {
+  // the edges (i, i - 1) for i = first, first + stride, ... (all of them by
+  // default; a share of them when several lanes split one polygon)
   const int b = l.vb[cj], e = l.vb[cj + 1], n = e - b;
+  const R invL = R(1) / L;
   bool odd = false;
-  for (int i = 0, j = n - 1; i < n; j = i++) {
+  for (int i = first; i < n; i += stride) {
+    const int j = i == 0 ? n - 1 : i - 1;
     R dix = px - l.x[b + i], diy = py - l.y[b + i];
     R djx = px - l.x[b + j], djy = py - l.y[b + j];
     if (pbc) {
       if (fabs(dix) > L || fabs(djx) > L) {
-        dix -= L * floor(dix / L);
-        djx -= L * floor(djx / L);
+        dix -= L * floor_quot(dix, L, invL);
+        djx -= L * floor_quot(djx, L, invL);
       }
       if (fabs(diy) > L || fabs(djy) > L) {
-        diy -= L * round(diy / L);
-        djy -= L * round(djy / L);
+        diy -= L * round_quot(diy, L, invL);
+        djy -= L * round_quot(djy, L, invL);
       }
     }
     if ((diy > R(0)) != (djy > R(0))) {
-      const R tt = (djx - dix) * (R(0) - diy) / (djy - diy) + dix;
-      if (R(0) < tt) odd = !odd;
+      const R a = (djx - dix) * (R(0) - diy), c = djy - diy;
+      bool pos;
+      if constexpr (sizeof(R) == 4) {
+        const R qa = a * __builtin_amdgcn_rcpf(c), ta = qa + dix;
+        if (fabs(ta) > R(1e-6f) * (fabs(qa) + fabs(dix)))
+          pos = ta > R(0);
+        else
+          pos = R(0) < a / c + dix;
+      } else {
+        pos = R(0) < a / c + dix;
+      }
+      if (pos) odd = !odd;
     }
   }
   return odd;
@@ -383,10 +427,16 @@ __device__ void over_pass(const Tissue2DCtx<R>& T, const Lds2D<R>& l, unsigned* 
   }
   if (cap == 0) return;
   __syncthreads();
+  // eight lanes per listed pair (aligned groups, active together), each
+  // testing every eighth edge; the crossing parity is the XOR over the group
   const int n = min(*count, cap);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int e = list[i], k = e & 0xffff, cj = e >> 16;
-    if (crossing_odd(l, cj, l.x[k], l.y[k], T.pbc, T.L)) atomicOr(&bits[k >> 5], 1u << (k & 31));
+  for (int i = threadIdx.x; i < 8 * n; i += blockDim.x) {
+    const int e = list[i >> 3], g = i & 7, k = e & 0xffff, cj = e >> 16;
+    int odd = crossing_odd(l, cj, l.x[k], l.y[k], T.pbc, T.L, g, 8) ? 1 : 0;
+    odd ^= __shfl_xor(odd, 1);
+    odd ^= __shfl_xor(odd, 2);
+    odd ^= __shfl_xor(odd, 4);
+    if (g == 0 && odd) atomicOr(&bits[k >> 5], 1u << (k & 31));
   }
 }
 

@@ -81,3 +81,85 @@ def test_no_float_square_lands_on_a_midpoint():
     for c in thresholds():
         b = bound_lt(c)
         assert np.float64(np.float32(b)) != b, c
+
+
+# ------------------------------------------------------------------ 2D crossing
+# The 2D crossing-parity test (crossing_odd in dpm2d.hip) takes floor(x / L),
+# round(x / L) and the sign of (djx - dix)(0 - diy) / (djy - diy) + dix from
+# products with reciprocals where the result is certain, else from the
+# correctly rounded division.  Here the fast forms, with the reciprocal's
+# error (up to one ulp on the device) pushed to either end, against the
+# divisions themselves, on values packed around every boundary.
+
+F = np.float32
+
+
+def fast_floor(x, L):
+    q = F(x * F(F(1) / L))
+    f = F(np.floor(q))
+    m = F(F(1e-6) * F(F(1) + F(abs(q))))
+    if F(q - f) > m and F(F(f + F(1)) - q) > m:
+        return f
+    return F(np.floor(F(x / L)))
+
+
+def fast_round(x, L):
+    q = F(x * F(F(1) / L))
+    m = F(F(1e-6) * F(F(1) + F(abs(q))))
+    h = F(F(abs(q)) - F(np.floor(F(abs(q)))))
+    if abs(F(h - F(0.5))) > m:
+        return F(np.sign(q) * np.floor(abs(q) + 0.5))
+    r = F(x / L)
+    return F(np.sign(r) * np.floor(abs(r) + F(0.5)))
+
+
+def fast_sign(a, c, dix, rcp_err):
+    rc = F(F(F(1) / c) * F(1 + rcp_err))
+    qa = F(a * rc)
+    ta = F(qa + dix)
+    if abs(ta) > F(F(1e-6) * F(abs(qa) + abs(dix))):
+        return bool(ta > 0)
+    return bool(F(0) < F(F(a / c) + dix))
+
+
+def c_round(v):
+    return F(np.sign(v) * np.floor(abs(v) + F(0.5)))
+
+
+def test_quotient_floor_and_round_equal_the_divisions():
+    rng = np.random.default_rng(11)
+    for L in F([9.969277, 38.4, 1.0, 153.6, 7.25]):
+        # x near k L and near (k + 1/2) L, walked over floats
+        for k in np.arange(-3, 4):
+            for base in (k * L, (k + 0.5) * L):
+                if base == 0:
+                    continue  # (the bit walk would leave the floats; the samples below cover 0)
+                c = np.array([base], F).view(np.int32)[0]
+                xs = (np.arange(c - 300, c + 301).astype(np.int32)).view(F)
+                for x in xs:
+                    assert fast_floor(x, L) == F(np.floor(F(x / L))), (x, L)
+                    assert fast_round(x, L) == c_round(F(x / L)), (x, L)
+        for x in F(rng.uniform(-4 * L, 4 * L, 2000)):
+            assert fast_floor(x, L) == F(np.floor(F(x / L)))
+            assert fast_round(x, L) == c_round(F(x / L))
+
+
+def test_crossing_sign_equals_the_division():
+    rng = np.random.default_rng(12)
+    eps = 2.0 ** -23
+    n = 0
+    for _ in range(20000):
+        a = F(rng.uniform(-3, 3))
+        c = F(rng.uniform(0.01, 3) * rng.choice([-1, 1]))
+        q = F(a / c)
+        # dix close to -q, so that tt sits near 0, and some far away
+        dix = F(-q + F(rng.normal() * abs(q) * 10.0 ** rng.integers(-9, 0)))
+        exact = bool(F(0) < F(q + dix))
+        for err in (-eps, 0.0, eps):
+            assert fast_sign(a, c, dix, err) == exact, (a, c, dix, err)
+        n += 1
+    # exact ties and zeros
+    for a, c, dix in ((F(0), F(1), F(0)), (F(1), F(1), F(-1)), (F(3), F(-2), F(1.5)), (F(1e-30), F(1e-8), F(0))):
+        exact = bool(F(0) < F(F(a / c) + dix))
+        for err in (-eps, 0.0, eps):
+            assert fast_sign(a, c, dix, err) == exact
