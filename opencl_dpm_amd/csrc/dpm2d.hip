@@ -389,12 +389,20 @@ __device__ __forceinline__ bool outside_box_even(const Lds2D<R>& l, int cj, R px
 // left in place, the few passing pairs of each wave would hold all its
 // lanes through their loops.  Pairs beyond `cap` entries (cap 0: no list)
 // are tested where they are found.  The bits are an OR, whatever the order.
+// Slots q run below qn, the largest candidate count of the tissue's cells
+// (nc where a list overflowed), and (k, q) = (w mod nvt, w / nvt) is
+// stepped without divisions.
 template <typename R>
 __device__ void over_pass(const Tissue2DCtx<R>& T, const Lds2D<R>& l, unsigned* bits, int* list, int cap,
-                          int* count) {
-  const int nvt = T.nvt, nc = T.nc, lane = threadIdx.x & (DPM_WAVE - 1);
-  for (int w = threadIdx.x; w < nvt * nc; w += blockDim.x) {
-    const int k = w % nvt, q = w / nvt;
+                          int* count, int qn) {
+  const int nvt = T.nvt, nc = T.nc, lane = threadIdx.x & (DPM_WAVE - 1), NT = blockDim.x;
+  const int dq = NT / nvt, dk = NT - dq * nvt;
+  int k = (int)threadIdx.x % nvt, q = (int)threadIdx.x / nvt;
+  for (int w = threadIdx.x; w < nvt * qn; w += NT, k += dk, q += dq) {
+    if (k >= nvt) {
+      k -= nvt;
+      q++;
+    }
     const int ci = cell_of(l.vb, nc, k);
     const int ncd = l.ncand[ci];
     const bool scan_all = ncd > DPM2D_MAXCAND;
@@ -637,10 +645,11 @@ struct Run2D {
   int over_cap;          // entries of the crossing-parity pair list after the LDS arrays (0: none)
 };
 
-// The crossing-parity pair list of k2d_run (a count, then the entries; the
-// entries pack vertex | cell << 16): its LDS after a tissue's Lds2D arrays.
+// The crossing-parity pair list of k2d_run (a count, the largest candidate
+// count, then the entries; the entries pack vertex | cell << 16): its LDS
+// after a tissue's Lds2D arrays.
 #define DPM2D_OVER_CAP 2048
-__host__ __device__ inline size_t lds2d_over_bytes() { return sizeof(int) * (size_t)(DPM2D_OVER_CAP + 1); }
+__host__ __device__ inline size_t lds2d_over_bytes() { return sizeof(int) * (size_t)(DPM2D_OVER_CAP + 2); }
 
 // Forces stay in registers: up to DPM2D_VPT vertices per thread.
 #define DPM2D_VPT 16
@@ -664,7 +673,9 @@ extern "C" __global__ void __launch_bounds__(1024) k2d_run(const Run2D a) {
                           T.nvt * T.nc <= 16 * NT;
   unsigned* obits = l.cbits;
   int* ocount = reinterpret_cast<int*>(smem + lds2d_bytes(T.nvt, T.nc, (int)sizeof(float)));
-  int* olist = ocount + 1;
+  int* oqmax = ocount + 1;
+  int* olist = ocount + 2;
+  if (tid == 0 && a.over_cap > 0) *oqmax = 0;  // (the first step's barriers precede its use)
   float rfx[DPM2D_VPT], rfy[DPM2D_VPT];
   for (int s = 0; s < nevals; s++) {
     cell_pass(g, T, l);
@@ -673,10 +684,17 @@ extern "C" __global__ void __launch_bounds__(1024) k2d_run(const Run2D a) {
     __syncthreads();
     if (split_over) {  // block-uniform; the candidate bits are dead here
       for (int i = tid; i < (T.nvt + 31) / 32; i += NT) obits[i] = 0u;
-      if (tid == 0 && a.over_cap > 0) *ocount = 0;
+      if (a.over_cap > 0) {
+        if (tid == 0) *ocount = 0;
+        for (int c = tid; c < T.nc; c += NT) {
+          const int nd = l.ncand[c];
+          atomicMax(oqmax, nd > DPM2D_MAXCAND ? T.nc : nd);
+        }
+      }
       __syncthreads();
-      over_pass(T, l, obits, olist, a.over_cap, ocount);
+      over_pass(T, l, obits, olist, a.over_cap, ocount, a.over_cap > 0 ? *oqmax : T.nc);
       __syncthreads();
+      if (tid == 0 && a.over_cap > 0) *oqmax = 0;  // every thread read it before the barrier above
     }
 #pragma unroll
     for (int r = 0; r < DPM2D_VPT; r++) {
