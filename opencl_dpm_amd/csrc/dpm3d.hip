@@ -494,7 +494,9 @@ struct AdhCut {
 };
 
 // Kat/2 (d/L0 - 1) delta/d with delta = (b - a) - L n, the adhesion term of
-// Cell3D_Kernel.cl:354-357, evaluated in double.  Float steps take it as
+// Cell3D_Kernel.cl:354-357, evaluated in double (the step kernels call it
+// for pairs across the periodic boundary; a float step's pairs without an
+// image take adhesion_term_near).  Float steps take it as
 // Kat/2 (1/L0 - 1/d) delta with 1/d from the hardware reciprocal square root
 // refined by two Newton steps (each squares the relative error) and kh =
 // Kat/2, inv_l0 = 1/L0 taken once per candidate pair — the same double value

@@ -21,7 +21,9 @@ static_assert(kWaveGlist <= kWaveCand, "the gathered list is sorted into the can
 
 // LDS: packed faces shared by the workgroup, then one slice per wave.  The
 // winding directions (or adhesion slices) share their bytes with the
-// cell-list gather, which is finished before the first candidate.
+// cell-list gather, which is finished before the first candidate, and with
+// the per-vertex facing-down counts of the shape phase, finished before the
+// candidate search.
 struct Lds3DWave {
   size_t vf, adj, img, fend, slice, total;
   size_t sx, nx, fx, il, down, cl, ut, gl, gath;
