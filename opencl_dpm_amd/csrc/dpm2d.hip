@@ -58,13 +58,13 @@ struct Geom2D {
 };
 
 // LDS per tissue: x, y [nvt] R | vb [nc+1] int | per cell strain, orient,
-// comx, comy, ax0, ax1, ay0, ay1 [nc] R | ncand [nc] int |
+// comx, comy, ax0, ax1, ay0, ay1, l0 [nc] R | ncand [nc] int |
 // cand [nc*MAXCAND] short | reduction scratch.
 __host__ __device__ inline size_t lds2d_bytes(int nvt, int nc, int rs) {
   size_t b = (size_t)rs * 2u * (size_t)nvt;
   b += sizeof(int) * (size_t)(nc + 1);
   b = (b + 7u) & ~(size_t)7u;
-  b += (size_t)rs * 8u * (size_t)nc + sizeof(int) * (size_t)nc;
+  b += (size_t)rs * 9u * (size_t)nc + sizeof(int) * (size_t)nc;
   b += sizeof(short) * (size_t)nc * DPM2D_MAXCAND;
   b = (b + 3u) & ~(size_t)3u;
   b += sizeof(unsigned) * (size_t)nc * DPM2D_CSPLIT;
@@ -77,6 +77,7 @@ struct Lds2D {
   R *x, *y;
   int* vb;
   R *strain, *orient, *comx, *comy, *ax0, *ax1, *ay0, *ay1;
+  R* l0c;  // the cells' l0 (staged from the parameters; the pair guards read it per pair)
   int* ncand;
   short* cand;
   unsigned* cbits;  // [nc * CSPLIT] candidate bits of one chunk of cells
@@ -101,7 +102,8 @@ __device__ __forceinline__ Lds2D<R> carve2d(unsigned char* smem, int nvt, int nc
   l.ax1 = cf + 5 * nc;
   l.ay0 = cf + 6 * nc;
   l.ay1 = cf + 7 * nc;
-  o += sizeof(R) * 8u * (size_t)nc;
+  l.l0c = cf + 8 * nc;
+  o += sizeof(R) * 9u * (size_t)nc;
   l.ncand = reinterpret_cast<int*>(smem + o);
   o += sizeof(int) * (size_t)nc;
   l.cand = reinterpret_cast<short*>(smem + o);
@@ -153,17 +155,25 @@ __device__ __forceinline__ void cell_one(const Geom2D<R>& g, const Tissue2DCtx<R
   const int b = l.vb[c], e = l.vb[c + 1], n = e - b;
   double area = 0.0, sx = 0.0, sy = 0.0;
   R x0 = l.x[b], x1 = l.x[b], y0 = l.y[b], y1 = l.y[b];
+  // vertex k's predecessor (the last vertex before the first) carried in
+  // registers, the loads of several vertices issued ahead: the same
+  // operations in the same order as one vertex after the other
+  R xp = l.x[e - 1], yp = l.y[e - 1];
+#pragma unroll 4
   for (int j = 0; j < n; j++) {
-    const int k = b + j, km = (j == 0) ? (e - 1) : (k - 1);
-    const double xs = (double)l.x[km] + (double)l.x[k];
-    const double yd = (double)l.y[km] - (double)l.y[k];
+    const int k = b + j;
+    const R xk = l.x[k], yk = l.y[k];
+    const double xs = (double)xp + (double)xk;
+    const double yd = (double)yp - (double)yk;
     area += xs * yd;
-    sx += (double)l.x[k];
-    sy += (double)l.y[k];
-    x0 = fmin(x0, l.x[k]);
-    x1 = fmax(x1, l.x[k]);
-    y0 = fmin(y0, l.y[k]);
-    y1 = fmax(y1, l.y[k]);
+    sx += (double)xk;
+    sy += (double)yk;
+    x0 = fmin(x0, xk);
+    x1 = fmax(x1, xk);
+    y0 = fmin(y0, yk);
+    y1 = fmax(y1, yk);
+    xp = xk;
+    yp = yk;
   }
   l.strain[c] = (R)(0.5 * fabs(area) / (double)g.a0[T.c0 + c] - 1.0);
   // the shoelace sum in this order is negative for counter-clockwise cells
@@ -207,7 +217,7 @@ __device__ bool maybe_pair(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Ld
     if (!disjoint) return true;
   }
   if (g.mask & (M2_ATTRACTION | M2_VREPULSION)) {
-    const R l0i = (R)g.l0[T.c0 + ci], l0j = (R)g.l0[T.c0 + cj];
+    const R l0i = l.l0c[ci], l0j = l.l0c[cj];
     const R cut = (g.mask & M2_VREPULSION) ? fmax(l0i, l0j) : l0i;
     const R hx = R(0.5f) * ((l.ax1[ci] - l.ax0[ci]) + (l.ax1[cj] - l.ax0[cj]));
     const R hy = R(0.5f) * ((l.ay1[ci] - l.ay0[ci]) + (l.ay1[cj] - l.ay0[cj]));
@@ -620,7 +630,10 @@ __device__ __forceinline__ void force_vertex(const Geom2D<R>& g, const Tissue2DC
 template <typename R>
 __device__ __forceinline__ void stage_tissue(const Geom2D<R>& g, const Tissue2DCtx<R>& T, const Lds2D<R>& l,
                                              const R* x, const R* y) {
-  for (int c = threadIdx.x; c <= T.nc; c += blockDim.x) l.vb[c] = g.cell_vert_begin[T.c0 + c] - T.v0;
+  for (int c = threadIdx.x; c <= T.nc; c += blockDim.x) {
+    l.vb[c] = g.cell_vert_begin[T.c0 + c] - T.v0;
+    if (c < T.nc) l.l0c[c] = (R)g.l0[T.c0 + c];
+  }
   for (int k = threadIdx.x; k < T.nvt; k += blockDim.x) {
     l.x[k] = x[T.v0 + k];
     l.y[k] = y[T.v0 + k];

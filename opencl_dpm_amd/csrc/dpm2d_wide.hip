@@ -65,6 +65,7 @@ __device__ __forceinline__ Lds2D<float> wide_view(const Wide2D& a, const Tissue2
   l.ax1 = q + 5 * (size_t)a.C;
   l.ay0 = q + 6 * (size_t)a.C;
   l.ay1 = q + 7 * (size_t)a.C;
+  l.l0c = const_cast<float*>(a.g.l0) + T.c0;
   l.ncand = a.ncand + T.c0;
   l.cand = a.cand + (size_t)T.c0 * DPM2D_MAXCAND;
   l.cbits = nullptr;
