@@ -236,7 +236,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--plain-steps", type=int, default=None,
                     help="steps timed on the plain tissue (default: --steps, the batched window)")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the batched handle's autotuner (dpm3d_autotune) and use --step-kernel")
